@@ -1,0 +1,292 @@
+"""gpupi_amd: Python host bindings for libgpupi.so, the MI355X (gfx950) build
+of the gpupi batched big-integer and pi library.
+
+The library's interface is the C ABI declared in include/*.h (batch_arith.h,
+batch_kernels.h, convert_decimal.h, gpupi_pi.h); this module mirrors it with
+ctypes so tests and benchmarks can drive it from Python.  Device memory is
+plain hipMalloc memory from the HIP runtime; there is no fallback path: if
+libgpupi.so is missing or no GPU is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpupi.so")
+CLI_PATH = os.path.join(_HERE, "pi_bs_gpu")
+
+HIP_SUCCESS = 0
+HIP_ERROR_INVALID_VALUE = 1
+PREC_REQUESTED = 0
+PREC_ROUNDED = 1
+
+_H2D, _D2H, _D2D = 1, 2, 3
+
+
+class BatchMPArray(ctypes.Structure):
+    """reference batch_arith.h:10-17 (data is a device pointer)."""
+
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("length", ctypes.c_uint32),
+        ("batch_size", ctypes.c_uint32),
+        ("stride", ctypes.c_uint32),
+    ]
+
+    def view(self, offset_words: int = 0, length: Optional[int] = None,
+             batch_size: Optional[int] = None, stride: Optional[int] = None) -> "BatchMPArray":
+        base = (self.data or 0) + 4 * offset_words
+        return BatchMPArray(base,
+                            self.length if length is None else length,
+                            self.batch_size if batch_size is None else batch_size,
+                            self.stride if stride is None else stride)
+
+
+class GpupiPiResult(ctypes.Structure):
+    """include/gpupi_pi.h GpupiPiResult."""
+
+    _fields_ = [
+        ("requested_digits", ctypes.c_uint32),
+        ("convert_digits", ctypes.c_uint32),
+        ("leaf_digits", ctypes.c_uint32),
+        ("prec_limbs", ctypes.c_uint32),
+        ("groups", ctypes.c_uint32),
+        ("ret_length", ctypes.c_uint32),
+        ("ret", ctypes.c_void_p),
+        ("digits", ctypes.c_void_p),
+        ("elapsed_ms", ctypes.c_float),
+        ("inverse_bits", ctypes.c_uint64),
+    ]
+
+
+class ConvertDecimalPowerTable(ctypes.Structure):
+    _fields_ = [
+        ("powers", ctypes.POINTER(BatchMPArray)),
+        ("levels", ctypes.c_uint32),
+        ("leaf_digits", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+_hip = None
+
+
+def _declare(lib):
+    A = BatchMPArray
+    c_ctx = ctypes.c_void_p
+    u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
+    err = ctypes.c_int
+    sig = {
+        "batch_mp_init": (c_ctx, []),
+        "batch_mp_destroy": (None, [c_ctx]),
+        "batch_mp_array_create": (A, [u32, u32, u32]),
+        "batch_mp_array_release": (None, [A]),
+        "batch_mp_workspace_size": (ctypes.c_size_t, [c_ctx]),
+        "batch_mp_ensure_workspace": (err, [c_ctx, ctypes.c_size_t]),
+        "batch_mp_mul": (err, [c_ctx, A, A, A]),
+        "batch_mp_mul_small": (err, [c_ctx, A, u32, A]),
+        "batch_mp_exactdiv_small": (err, [c_ctx, A, u32]),
+        "batch_mp_add": (err, [c_ctx, A, A, A]),
+        "batch_mp_add_small": (err, [c_ctx, A, u32, A]),
+        "batch_mp_shift_add": (err, [c_ctx, A, A, u32, A]),
+        "batch_mp_shift_sub": (err, [c_ctx, A, A, u32, A]),
+        "batch_mp_sub": (err, [c_ctx, A, A, A]),
+        "batch_mp_sub_small": (err, [c_ctx, A, u32, A]),
+        "batch_mp_shift_bits": (err, [c_ctx, A, A, i32]),
+        "batch_mp_bitlength_max": (err, [c_ctx, A, ctypes.POINTER(u32)]),
+        "batch_mp_limblength_max": (err, [c_ctx, A, ctypes.POINTER(u32)]),
+        "batch_mp_array_compact": (err, [c_ctx, ctypes.POINTER(A)]),
+        "batch_mp_set_stream": (err, [c_ctx, ctypes.c_void_p]),
+        "batch_mp_set_ntt_threshold": (err, [c_ctx, u32]),
+        "gpupi_pi_run": (err, [c_ctx, u32, ctypes.c_int, ctypes.POINTER(GpupiPiResult)]),
+        "gpupi_pi_free": (None, [ctypes.POINTER(GpupiPiResult)]),
+        "gpupi_binary_split": (err, [c_ctx, u32, ctypes.POINTER(A), ctypes.POINTER(A)]),
+        "gpupi_sqrt_10005": (err, [c_ctx, ctypes.c_int, ctypes.POINTER(A), ctypes.POINTER(A)]),
+        "gpupi_recursive_inverse": (err, [c_ctx, A, u64, ctypes.POINTER(A)]),
+        "gpupi_pi_fractional": (err, [c_ctx, ctypes.c_int, ctypes.POINTER(A), ctypes.POINTER(u32),
+                                      ctypes.POINTER(u64)]),
+        "gpupi_construct_leaf_17": (err, [c_ctx, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, u32]),
+        "gpupi_round_up_decimal_digits": (u32, [u32, ctypes.POINTER(u32)]),
+        "gpupi_prec_limbs_for_digits": (u32, [u32]),
+        "convert_decimal_precompute_powers": (err, [c_ctx, ctypes.POINTER(ConvertDecimalPowerTable), u32, u32]),
+        "convert_decimal_release_powers": (None, [ctypes.POINTER(ConvertDecimalPowerTable)]),
+        "convert_decimal_equal_split": (err, [c_ctx, A, u32, u32, u32,
+                                              ctypes.POINTER(ConvertDecimalPowerTable), ctypes.c_void_p]),
+        "batch_decimal_naive": (None, [ctypes.c_void_p, ctypes.c_void_p, u32, ctypes.c_int, u32,
+                                       ctypes.c_int, u32]),
+        "batch_mul_ntt_workspace_size": (ctypes.c_size_t, [u32, u32, u32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """The loaded libgpupi.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so", mode=ctypes.RTLD_GLOBAL)
+        _hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        _hip.hipFree.argtypes = [ctypes.c_void_p]
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+        _hip.hipGetErrorString.restype = ctypes.c_char_p
+        _hip.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    return _hip
+
+
+def check(err: int, what: str = "HIP call") -> None:
+    if err != HIP_SUCCESS:
+        msg = hip().hipGetErrorString(err).decode()
+        raise RuntimeError(f"{what} failed: {err} ({msg})")
+
+
+def gpu_available() -> bool:
+    try:
+        n = ctypes.c_int(0)
+        return hip().hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    except OSError:
+        return False
+
+
+def synchronize() -> None:
+    check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class DeviceBuffer:
+    """A hipMalloc'd buffer of uint32 words (or bytes)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = max(int(nbytes), 4)
+        p = ctypes.c_void_p()
+        check(hip().hipMalloc(ctypes.byref(p), self.nbytes), "hipMalloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        if arr.nbytes:
+            check(hip().hipMemcpy(buf.ptr, arr.ctypes.data, arr.nbytes, _H2D), "hipMemcpy H2D")
+        return buf
+
+    def fill(self, byte: int) -> None:
+        check(hip().hipMemset(self.ptr, byte, self.nbytes), "hipMemset")
+
+    def download(self, dtype=np.uint32, count: Optional[int] = None, offset_bytes: int = 0) -> np.ndarray:
+        itemsize = np.dtype(dtype).itemsize
+        if count is None:
+            count = (self.nbytes - offset_bytes) // itemsize
+        out = np.empty(count, dtype=dtype)
+        if count:
+            check(hip().hipMemcpy(out.ctypes.data, self.ptr + offset_bytes, count * itemsize, _D2H),
+                  "hipMemcpy D2H")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def download_words(ptr: int, count: int) -> np.ndarray:
+    out = np.empty(count, dtype=np.uint32)
+    if count:
+        check(hip().hipMemcpy(out.ctypes.data, ptr, count * 4, _D2H), "hipMemcpy D2H")
+    return out
+
+
+def download_bytes(ptr: int, count: int) -> bytes:
+    out = ctypes.create_string_buffer(max(count, 1))
+    if count:
+        check(hip().hipMemcpy(out, ptr, count, _D2H), "hipMemcpy D2H")
+    return out.raw[:count]
+
+
+class Context:
+    """Owns a BatchMPContext (reference batch_arith.h:33-43)."""
+
+    def __init__(self):
+        self._lib = lib()
+        self.ptr = self._lib.batch_mp_init()
+        if not self.ptr:
+            raise RuntimeError("batch_mp_init failed (no GPU, or the HIP runtime could not start)")
+
+    def close(self) -> None:
+        if self.ptr:
+            self._lib.batch_mp_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __getattr__(self, name):
+        # ctx.mul(A, B, C) -> batch_mp_mul(ctx, A, B, C), etc.
+        fn = getattr(lib(), "batch_mp_" + name, None)
+        if fn is None:
+            raise AttributeError(name)
+        return lambda *args: fn(self.ptr, *args)
+
+    def limblength_max_value(self, A: BatchMPArray) -> int:
+        r = ctypes.c_uint32(0)
+        check(self._lib.batch_mp_limblength_max(self.ptr, A, ctypes.byref(r)), "limblength_max")
+        return r.value
+
+    def pi(self, digits: int, convention: int = PREC_REQUESTED):
+        """Run the pipeline; returns (RET limbs, digit string, result struct)."""
+        res = GpupiPiResult()
+        check(self._lib.gpupi_pi_run(self.ptr, digits, convention, ctypes.byref(res)), "gpupi_pi_run")
+        try:
+            ret = download_words(res.ret, res.ret_length)
+            dec = download_bytes(res.digits, digits).decode() if digits else ""
+        finally:
+            self._lib.gpupi_pi_free(ctypes.byref(res))
+        return ret, dec, res
+
+
+def upload_rows(rows: np.ndarray, stride: Optional[int] = None, pad_value: int = 0):
+    """rows: (N, L) uint32.  Returns (buffer, BatchMPArray) with the given stride."""
+    rows = np.asarray(rows, dtype=np.uint32)
+    n, length = rows.shape
+    stride = length if stride is None else stride
+    host = np.full((n, max(stride, 1)), pad_value, dtype=np.uint32)
+    host[:, :length] = rows
+    buf = DeviceBuffer.from_array(host)
+    return buf, BatchMPArray(buf.ptr, length, n, stride)
+
+
+def download_rows(arr: BatchMPArray, buf: DeviceBuffer) -> np.ndarray:
+    """The (batch, length) limbs of an array view living in buf."""
+    stride = arr.stride if arr.stride else arr.length
+    total = (arr.batch_size - 1) * stride + arr.length if arr.batch_size else 0
+    off = arr.data - buf.ptr
+    flat = buf.download(np.uint32, count=total, offset_bytes=off)
+    out = np.zeros((arr.batch_size, arr.length), dtype=np.uint32)
+    for i in range(arr.batch_size):
+        out[i] = flat[i * stride: i * stride + arr.length]
+    return out

@@ -1,0 +1,257 @@
+// Context, arrays, validation and op dispatch: the C ABI of batch_arith.h
+// (reference batch_arith.cu:18-424).
+#include <hip/hip_runtime.h>
+
+#include <new>
+
+#include "common.hpp"
+
+namespace {
+
+// reference batch_arith.cu:89-100
+bool valid_array(const BatchMPArray &a) {
+    if (a.batch_size == 0) return a.length <= a.stride;
+    if (a.data == nullptr) return false;
+    if (a.stride == 0) return true;
+    return a.length <= a.stride;
+}
+
+// Outputs must not broadcast (every row would be written at one address).
+bool valid_output(const BatchMPArray &c) {
+    return valid_array(c) && (c.batch_size <= 1 || c.stride != 0);
+}
+
+bool same_batch(const BatchMPArray &a, const BatchMPArray &b) { return a.batch_size == b.batch_size; }
+
+// In-place use of an operand is accepted only with C's exact view.
+bool alias_ok(const BatchMPArray &x, const BatchMPArray &c) {
+    if (x.data != c.data || x.length == 0) return true;  // a zero-length operand is never read
+    return x.length == c.length && x.stride == c.stride;
+}
+
+hipError_t ensure(BatchMPContext *ctx, size_t bytes) {
+    if (bytes <= ctx->workspace_size_bytes) return hipSuccess;
+    uint32_t *fresh = nullptr;
+    GPUPI_RETURN_IF(hipMalloc(&fresh, bytes));
+    if (ctx->workspace != nullptr) {
+        hipError_t err = hipFree(ctx->workspace);
+        if (err != hipSuccess) {
+            (void)hipFree(fresh);
+            return err;
+        }
+    }
+    ctx->workspace = fresh;
+    ctx->workspace_size_bytes = bytes;
+    return hipSuccess;
+}
+
+hipError_t run_addsub(BatchMPContext *ctx, gpupi::AddSubKind kind, const BatchMPArray &A, const BatchMPArray *B,
+                      uint32_t scalar, uint32_t shift, const BatchMPArray &C) {
+    gpupi::AddSubArgs a{};
+    a.kind = kind;
+    a.A = A.data;
+    a.B = B ? B->data : nullptr;
+    a.C = C.data;
+    a.scalar = scalar;
+    a.shift = shift;
+    a.N = C.batch_size;
+    a.L_a = A.length;
+    a.L_b = B ? B->length : 0;
+    a.L_c = C.length;
+    a.stride_A = A.stride;
+    a.stride_B = B ? B->stride : 0;
+    a.stride_C = C.stride;
+    if (a.N == 0 || a.L_c == 0) return hipSuccess;
+    GPUPI_RETURN_IF(ensure(ctx, gpupi::carry_flags_bytes(a.N, a.L_c)));
+    return gpupi::launch_addsub(a, reinterpret_cast<uint8_t *>(ctx->workspace), ctx->stream);
+}
+
+hipError_t length_max(BatchMPContext *ctx, const BatchMPArray &A, bool bits, uint32_t *result) {
+    if (ctx == nullptr || !valid_array(A) || result == nullptr) return hipErrorInvalidValue;
+    if (A.batch_size == 0 || A.length == 0) {
+        *result = 0;
+        return hipSuccess;
+    }
+    GPUPI_RETURN_IF(gpupi::launch_fill_u32(ctx->d_scalar, 1, 0u, ctx->stream));
+    GPUPI_RETURN_IF(gpupi::launch_length_max(A.data, A.batch_size, A.length, A.stride, bits, ctx->d_scalar,
+                                             ctx->stream));
+    GPUPI_RETURN_IF(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+    *result = *ctx->h_scalar;
+    return hipSuccess;
+}
+
+// Default schoolbook -> NTT crossover on L_a + L_b (tuned by measurement).
+constexpr int kDefaultNttMinLimbs = 384;
+
+}  // namespace
+
+extern "C" {
+
+BatchMPArray batch_mp_array_create(uint32_t batch_size, uint32_t length, uint32_t stride) {
+    BatchMPArray a{};
+    a.length = length;
+    a.batch_size = batch_size;
+    a.stride = stride == 0 ? length : stride;
+    if (a.length > a.stride) {
+        a.stride = 0;
+        a.length = 0;
+        a.batch_size = 0;
+        return a;
+    }
+    const size_t bytes = (size_t)a.batch_size * a.stride * sizeof(uint32_t);
+    if (bytes == 0) return a;
+    if (hipMalloc(&a.data, bytes) != hipSuccess) a = BatchMPArray{};
+    return a;
+}
+
+void batch_mp_array_release(BatchMPArray array) {
+    if (array.data != nullptr) (void)hipFree(array.data);
+}
+
+BatchMPContext *batch_mp_init(void) {
+    BatchMPContext *ctx = new (std::nothrow) BatchMPContext{};
+    if (ctx == nullptr) return nullptr;
+    ctx->stream = nullptr;
+    ctx->ntt_min_limbs = kDefaultNttMinLimbs;
+    if (gpupi::ntt_tables_init(&ctx->ntt_tables) != hipSuccess ||
+        hipMalloc(&ctx->d_scalar, sizeof(uint32_t)) != hipSuccess ||
+        hipHostMalloc(&ctx->h_scalar, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        batch_mp_destroy(ctx);
+        return nullptr;
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+        batch_mp_destroy(ctx);
+        return nullptr;
+    }
+    return ctx;
+}
+
+void batch_mp_destroy(BatchMPContext *ctx) {
+    if (ctx == nullptr) return;
+    if (ctx->workspace) (void)hipFree(ctx->workspace);
+    if (ctx->d_scalar) (void)hipFree(ctx->d_scalar);
+    if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
+    gpupi::ntt_tables_release(&ctx->ntt_tables);
+    delete ctx;
+}
+
+size_t batch_mp_workspace_size(const BatchMPContext *ctx) { return ctx ? ctx->workspace_size_bytes : 0; }
+
+hipError_t batch_mp_ensure_workspace(BatchMPContext *ctx, size_t required_bytes) {
+    if (ctx == nullptr) return hipErrorInvalidValue;
+    return ensure(ctx, required_bytes);
+}
+
+hipError_t batch_mp_set_stream(BatchMPContext *ctx, hipStream_t stream) {
+    if (ctx == nullptr) return hipErrorInvalidValue;
+    ctx->stream = stream;
+    return hipSuccess;
+}
+
+hipStream_t batch_mp_get_stream(const BatchMPContext *ctx) { return ctx ? ctx->stream : nullptr; }
+
+// Extension: schoolbook/NTT crossover (L_a + L_b at or above uses the NTT).
+hipError_t batch_mp_set_ntt_threshold(BatchMPContext *ctx, uint32_t min_total_limbs) {
+    if (ctx == nullptr) return hipErrorInvalidValue;
+    ctx->ntt_min_limbs = (int)min_total_limbs;
+    return hipSuccess;
+}
+
+hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, BatchMPArray C) {
+    if (ctx == nullptr || !valid_array(A) || !valid_array(B) || !valid_output(C)) return hipErrorInvalidValue;
+    if (!same_batch(A, B) || !same_batch(A, C)) return hipErrorInvalidValue;
+    if ((uint64_t)C.length != (uint64_t)A.length + B.length) return hipErrorInvalidValue;
+    if (C.batch_size > 0 && (C.data == A.data || C.data == B.data)) return hipErrorInvalidValue;
+    if (C.batch_size == 0 || C.length == 0) return hipSuccess;
+    const uint32_t total = A.length + B.length;
+    const bool small = A.length == 0 || B.length == 0 ||
+                       ((int64_t)total < ctx->ntt_min_limbs && gpupi::mul_naive_supports(A.length, B.length));
+    if (small || !gpupi::ntt_supports(A.length, B.length)) {
+        if (!gpupi::mul_naive_supports(A.length, B.length)) return hipErrorInvalidValue;
+        return gpupi::launch_mul_naive(A.data, B.data, C.data, C.batch_size, A.length, B.length, A.stride,
+                                       B.stride, C.stride, ctx->stream);
+    }
+    GPUPI_RETURN_IF(ensure(ctx, gpupi::ntt_workspace_bytes(C.batch_size, A.length, B.length)));
+    return gpupi::launch_mul_ntt(A.data, B.data, C.data, ctx->workspace, ctx->workspace_size_bytes,
+                                 ctx->ntt_tables, C.batch_size, A.length, B.length, A.stride, B.stride,
+                                 C.stride, ctx->stream);
+}
+
+hipError_t batch_mp_mul_small(BatchMPContext *ctx, BatchMPArray A, uint32_t B, BatchMPArray C) {
+    if (ctx == nullptr || !valid_array(A) || !valid_output(C) || !same_batch(A, C)) return hipErrorInvalidValue;
+    if (C.batch_size > 0 && A.data == C.data) return hipErrorInvalidValue;  // reference asserts A != C
+    return run_addsub(ctx, gpupi::AddSubKind::MulSmall, A, nullptr, B, 0, C);
+}
+
+hipError_t batch_mp_exactdiv_small(BatchMPContext *ctx, BatchMPArray A, uint32_t B) {
+    if (ctx == nullptr || !valid_output(A)) return hipErrorInvalidValue;
+    if (B == 0u || (B & 1u) == 0u || A.length > 128u) return hipErrorInvalidValue;
+    if (A.batch_size == 0 || A.length == 0) return hipSuccess;
+    return gpupi::launch_exactdiv_small(A.data, B, A.batch_size, A.length, A.stride, ctx->stream);
+}
+
+hipError_t batch_mp_add(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, BatchMPArray C) {
+    if (ctx == nullptr || !valid_array(A) || !valid_array(B) || !valid_output(C)) return hipErrorInvalidValue;
+    if (!same_batch(A, B) || !same_batch(A, C) || !alias_ok(A, C) || !alias_ok(B, C)) return hipErrorInvalidValue;
+    return run_addsub(ctx, gpupi::AddSubKind::Add, A, &B, 0, 0, C);
+}
+
+hipError_t batch_mp_add_small(BatchMPContext *ctx, BatchMPArray A, uint32_t B, BatchMPArray C) {
+    if (ctx == nullptr || !valid_array(A) || !valid_output(C) || !same_batch(A, C) || !alias_ok(A, C))
+        return hipErrorInvalidValue;
+    return run_addsub(ctx, gpupi::AddSubKind::AddSmall, A, nullptr, B, 0, C);
+}
+
+hipError_t batch_mp_shift_add(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, uint32_t shift, BatchMPArray C) {
+    if (ctx == nullptr || !valid_array(A) || !valid_array(B) || !valid_output(C)) return hipErrorInvalidValue;
+    if (!same_batch(A, B) || !same_batch(A, C)) return hipErrorInvalidValue;
+    if (C.data == A.data || C.data == B.data) return hipErrorInvalidValue;
+    return run_addsub(ctx, gpupi::AddSubKind::ShiftAdd, A, &B, 0, shift, C);
+}
+
+hipError_t batch_mp_shift_sub(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, uint32_t shift, BatchMPArray C) {
+    if (ctx == nullptr || !valid_array(A) || !valid_array(B) || !valid_output(C)) return hipErrorInvalidValue;
+    if (!same_batch(A, B) || !same_batch(A, C)) return hipErrorInvalidValue;
+    if (C.data == A.data || C.data == B.data) return hipErrorInvalidValue;
+    return run_addsub(ctx, gpupi::AddSubKind::ShiftSub, A, &B, 0, shift, C);
+}
+
+hipError_t batch_mp_sub(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, BatchMPArray C) {
+    if (ctx == nullptr || !valid_array(A) || !valid_array(B) || !valid_output(C)) return hipErrorInvalidValue;
+    if (!same_batch(A, B) || !same_batch(A, C) || !alias_ok(A, C) || !alias_ok(B, C)) return hipErrorInvalidValue;
+    return run_addsub(ctx, gpupi::AddSubKind::Sub, A, &B, 0, 0, C);
+}
+
+hipError_t batch_mp_sub_small(BatchMPContext *ctx, BatchMPArray A, uint32_t B, BatchMPArray C) {
+    if (ctx == nullptr || !valid_array(A) || !valid_output(C) || !same_batch(A, C) || !alias_ok(A, C))
+        return hipErrorInvalidValue;
+    return run_addsub(ctx, gpupi::AddSubKind::SubSmall, A, nullptr, B, 0, C);
+}
+
+hipError_t batch_mp_shift_bits(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, int32_t shift_bits) {
+    if (ctx == nullptr || !valid_array(A) || !valid_output(B) || !same_batch(A, B)) return hipErrorInvalidValue;
+    if (B.batch_size > 0 && A.data == B.data && shift_bits != 0) return hipErrorInvalidValue;
+    if (B.batch_size == 0 || B.length == 0) return hipSuccess;
+    return gpupi::launch_shift_bits(A.data, B.data, B.batch_size, A.length, B.length, A.stride, B.stride,
+                                    shift_bits, ctx->stream);
+}
+
+hipError_t batch_mp_bitlength_max(BatchMPContext *ctx, BatchMPArray A, uint32_t *result) {
+    return length_max(ctx, A, true, result);
+}
+
+hipError_t batch_mp_limblength_max(BatchMPContext *ctx, BatchMPArray A, uint32_t *result) {
+    return length_max(ctx, A, false, result);
+}
+
+hipError_t batch_mp_array_compact(BatchMPContext *ctx, BatchMPArray *array) {
+    if (array == nullptr) return hipErrorInvalidValue;
+    uint32_t len = 0;
+    GPUPI_RETURN_IF(batch_mp_limblength_max(ctx, *array, &len));
+    array->length = len;
+    return hipSuccess;
+}
+
+}  // extern "C"

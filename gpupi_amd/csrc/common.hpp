@@ -1,0 +1,112 @@
+// gpupi-mi355x internal header: device helpers shared by the HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "batch_arith.h"
+#include "batch_kernels.h"
+
+namespace gpupi {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+#define GPUPI_RETURN_IF(err)                     \
+    do {                                         \
+        hipError_t _e = (err);                   \
+        if (_e != hipSuccess) return _e;         \
+    } while (0)
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+inline uint64_t ceil_div64(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot64(bool p) { return (uint64_t)__ballot(p); }
+
+// Resolve a 64-lane carry (or borrow) chain.  Bit l of G: lane l produces a
+// carry with no carry-in; bit l of P: lane l passes an incoming carry on
+// (G and P never share a bit).  Treating (G|P) and G as two 64-bit numbers,
+// the carry into lane l of their sum is exactly the carry lane l receives.
+// Returns the carry-in mask; *cout receives the carry out of lane 63.
+__device__ __forceinline__ uint64_t resolve_carries(uint64_t G, uint64_t P, uint32_t cin,
+                                                    uint32_t *cout) {
+    const uint64_t X = G | P;
+    const uint64_t s1 = X + G;
+    const uint32_t c1 = s1 < X;
+    const uint64_t s2 = s1 + (uint64_t)cin;
+    const uint32_t c2 = s2 < s1;
+    *cout = c1 | c2;
+    return s2 ^ X ^ G;
+}
+
+// Internal stream-aware launchers (defined across the .hip sources).
+// `flags` is scratch of at least carry_flags_bytes(N, L_c) bytes.
+size_t carry_flags_bytes(uint32_t N, uint32_t L_c);
+
+enum class AddSubKind : int { Add, Sub, AddSmall, SubSmall, MulSmall, ShiftAdd, ShiftSub, SpillAdd };
+
+struct AddSubArgs {
+    AddSubKind kind;
+    const uint32_t *A;
+    const uint32_t *B;
+    uint32_t *C;
+    uint32_t scalar;   // AddSmall/SubSmall/MulSmall operand
+    uint32_t shift;    // ShiftAdd/ShiftSub bit shift of A
+    uint32_t N, L_a, L_b, L_c;
+    uint32_t stride_A, stride_B, stride_C;
+};
+
+hipError_t launch_addsub(const AddSubArgs &args, uint8_t *flags, hipStream_t stream);
+
+// C (limbs) += spill words: for each run r (run_len limbs), the 64-bit
+// value spill[r] is added at limb (r + 1) * run_len of the same row.
+hipError_t launch_add_spills(uint32_t *C, const uint64_t *spill, uint32_t N, uint32_t L_c,
+                             uint32_t stride_C, uint32_t run_len, uint32_t runs_per_row,
+                             uint8_t *flags, hipStream_t stream);
+
+hipError_t launch_shift_bits(const uint32_t *A, uint32_t *B, uint32_t N, uint32_t L_in,
+                             uint32_t L_out, uint32_t stride_in, uint32_t stride_out,
+                             int64_t shift_bits, hipStream_t stream);
+
+hipError_t launch_exactdiv_small(uint32_t *A, uint32_t B, uint32_t N, uint32_t L,
+                                 uint32_t stride_A, hipStream_t stream);
+
+// Writes max(limb length) (or bit length) over the batch into *d_result,
+// which the caller zeroed.
+hipError_t launch_length_max(const uint32_t *A, uint32_t N, uint32_t L, uint32_t stride_A,
+                             bool bits, uint32_t *d_result, hipStream_t stream);
+
+hipError_t launch_mul_naive(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t N,
+                            uint32_t L_a, uint32_t L_b, uint32_t stride_A, uint32_t stride_B,
+                            uint32_t stride_C, hipStream_t stream);
+bool mul_naive_supports(uint32_t L_a, uint32_t L_b);
+
+hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, void *workspace,
+                          size_t workspace_bytes, const NTTPrecomputedTables &tables, uint32_t N,
+                          uint32_t L_a, uint32_t L_b, uint32_t stride_A, uint32_t stride_B,
+                          uint32_t stride_C, hipStream_t stream);
+size_t ntt_workspace_bytes(uint32_t N, uint32_t L_a, uint32_t L_b);
+bool ntt_supports(uint32_t L_a, uint32_t L_b);
+hipError_t ntt_tables_init(NTTPrecomputedTables *tables);
+void ntt_tables_release(NTTPrecomputedTables *tables);
+
+hipError_t launch_decimal_leaf(const uint32_t *A, char *B, uint32_t N, uint32_t L_limb,
+                               uint32_t stride_A, uint32_t L_dec, uint32_t stride_B,
+                               bool natural_order, hipStream_t stream);
+
+hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStream_t stream);
+
+}  // namespace gpupi
+
+// Context internals shared between arith.hip and the pipeline sources.
+struct BatchMPContext {
+    NTTPrecomputedTables ntt_tables;
+    uint32_t *workspace;
+    size_t workspace_size_bytes;
+    uint32_t *d_scalar;       // device word for length reductions
+    uint32_t *h_scalar;       // pinned host mirror
+    hipStream_t stream;
+    int ntt_min_limbs;        // schoolbook -> NTT crossover on L_a + L_b
+};

@@ -1,0 +1,845 @@
+// The Chudnovsky pipeline on the device: leaf terms, the batched binary
+// splitting tree, sqrt(10005), the inverse, the final division and the
+// equal-split radix conversion.  The sequence of batched operations (and so
+// every intermediate integer) follows the reference pi_bs_gpu.cu and
+// convert_decimal.cu line for line where the values depend on it:
+// truncations keyed on compacted lengths, the 2^15-per-term shift, the
+// divisors of the 17-term block, the Pell doubling, and the approximate
+// inverse.  Buffers are sized on the host from the compacted lengths, which
+// every compaction (a synchronizing length reduction) brings back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "convert_decimal.h"
+#include "gpupi_pi.h"
+
+namespace gpupi {
+hipError_t launch_extract_shifted(const BatchMPArray &src, BatchMPArray &dst, uint32_t rows, uint32_t shift_bits,
+                                  uint32_t dst_row_offset, uint32_t dst_row_step, hipStream_t stream);
+hipError_t launch_extract_low(const BatchMPArray &src, BatchMPArray &dst, uint32_t rows, uint32_t valid_bits,
+                              hipStream_t stream);
+}  // namespace gpupi
+
+namespace {
+
+using gpupi::ceil_div;
+
+// ------------------------------------------------------------------ leaf
+__device__ __forceinline__ void reduce_pair(uint32_t &a, uint32_t &b, uint32_t d) {
+    if (a % d == 0u && b % d == 0u) {
+        a /= d;
+        b /= d;
+    }
+}
+
+__device__ __forceinline__ void reduce_3_5_23_29(uint32_t &a, uint32_t &b) {
+    reduce_pair(a, b, 3u);
+    reduce_pair(a, b, 5u);
+    reduce_pair(a, b, 23u);
+    reduce_pair(a, b, 29u);
+}
+
+template <int W>
+__device__ __forceinline__ void mul_word(uint32_t (&v)[W], uint32_t m) {
+    uint32_t carry = 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint64_t t = (uint64_t)v[j] * m + carry;
+        v[j] = (uint32_t)t;
+        carry = (uint32_t)(t >> 32);
+    }
+}
+
+// Term i: p = (6i+1)(2i+1)(6i+5), q = (i+1)^3 * 10005^2 * 3335 (= (i+1)^3
+// 640320^3 / 24 / 2^15), r = (13591409 + 545140134 i) * q / 3335, with the
+// common factors of p and q cancelled as in pi_bs_gpu.cu:17-74.  Head rows
+// are terms 17t + 16, body rows terms t + t/16 (pi_bs_gpu.cu:40-46).
+__global__ void __launch_bounds__(256) leaf17_kernel(int is_head, uint32_t *P, uint32_t *Q, uint32_t *R,
+                                                     uint32_t count) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += blockDim.x * gridDim.x) {
+        const int64_t i = is_head ? (int64_t)t * 17 + 16 : (int64_t)t + (int64_t)(t >> 4);
+        uint32_t p1 = (uint32_t)(6 * i + 1), p2 = (uint32_t)(2 * i + 1), p3 = (uint32_t)(6 * i + 5);
+        uint32_t q1 = (uint32_t)(i + 1), q2 = q1, q3 = q1;
+        uint32_t q4 = 10005u, q5 = 10005u;
+        const uint32_t q6 = 3335u;
+        const uint64_t r1 = (uint64_t)(545140134ll * i + 13591409ll);
+        reduce_pair(p1, q1, 5u);
+        reduce_pair(p1, q2, 5u);
+        reduce_pair(p1, q3, 5u);
+        reduce_3_5_23_29(p1, q4);
+        reduce_3_5_23_29(p1, q5);
+        reduce_3_5_23_29(p2, q4);
+        reduce_3_5_23_29(p2, q5);
+        reduce_3_5_23_29(p3, q4);
+        reduce_3_5_23_29(p3, q5);
+        uint32_t pv[3] = {p1, 0, 0};
+        mul_word(pv, p2);
+        mul_word(pv, p3);
+        uint32_t qv[4] = {q1, 0, 0, 0};
+        mul_word(qv, q2);
+        mul_word(qv, q3);
+        mul_word(qv, q4);
+        mul_word(qv, q5);
+        mul_word(qv, q6);
+        uint32_t rv[6] = {(uint32_t)r1, (uint32_t)(r1 >> 32), 0, 0, 0, 0};
+        mul_word(rv, q1);
+        mul_word(rv, q2);
+        mul_word(rv, q3);
+        mul_word(rv, q4);
+        mul_word(rv, q5);
+        uint32_t *po = P + (size_t)t * 3, *qo = Q + (size_t)t * 4, *ro = R + (size_t)t * 6;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) po[j] = pv[j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) qo[j] = qv[j];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) ro[j] = rv[j];
+    }
+}
+
+// ------------------------------------------------------- buffer helpers
+
+// A grow-only device buffer holding one batch array.
+struct Buf {
+    uint32_t *ptr = nullptr;
+    size_t cap = 0;  // words
+    Buf() = default;
+    Buf(const Buf &) = delete;
+    Buf &operator=(const Buf &) = delete;
+    ~Buf() {
+        if (ptr) (void)hipFree(ptr);
+    }
+    hipError_t reserve(size_t words) {
+        if (words <= cap) return hipSuccess;
+        if (ptr) {
+            GPUPI_RETURN_IF(hipFree(ptr));
+            ptr = nullptr;
+            cap = 0;
+        }
+        GPUPI_RETURN_IF(hipMalloc(&ptr, std::max<size_t>(words, 1) * sizeof(uint32_t)));
+        cap = words;
+        return hipSuccess;
+    }
+    // A contiguous array view (stride = length) of `batch` rows.
+    hipError_t view(uint32_t batch, uint32_t length, BatchMPArray &out) {
+        GPUPI_RETURN_IF(reserve((size_t)batch * length));
+        out.data = ptr;
+        out.length = length;
+        out.batch_size = batch;
+        out.stride = length;
+        return hipSuccess;
+    }
+    void swap(Buf &o) {
+        std::swap(ptr, o.ptr);
+        std::swap(cap, o.cap);
+    }
+    uint32_t *release() {
+        uint32_t *p = ptr;
+        ptr = nullptr;
+        cap = 0;
+        return p;
+    }
+};
+
+BatchMPArray view_of(uint32_t *data, uint32_t length, uint32_t batch, uint32_t stride) {
+    BatchMPArray a;
+    a.data = data;
+    a.length = length;
+    a.batch_size = batch;
+    a.stride = stride;
+    return a;
+}
+
+// pi_bs_gpu.cu:1095-1110
+BatchMPArray make_subview(const BatchMPArray &a, uint32_t start, uint32_t length) {
+    if (start > a.length) start = a.length;
+    const uint32_t max_len = a.length - start;
+    if (length > max_len) length = max_len;
+    return view_of(a.data + start, length, a.batch_size, a.stride >= start ? a.stride - start : 0u);
+}
+
+hipError_t compact(BatchMPContext *ctx, BatchMPArray &a) { return batch_mp_array_compact(ctx, &a); }
+
+// Host-side guard: a batch-1 view must lie inside [base, base + words).
+bool inside(const BatchMPArray &v, const uint32_t *base, size_t words) {
+    if (v.length == 0) return true;
+    return v.data >= base && v.data + v.length <= base + words;
+}
+
+// pi_bs_gpu.cu:1068-1093
+hipError_t shift_right_large(BatchMPContext *ctx, const BatchMPArray &A, uint64_t shift_bits, BatchMPArray B) {
+    const uint64_t limb_shift = shift_bits / 32u;
+    const uint32_t bit_shift = (uint32_t)(shift_bits % 32u);
+    if (limb_shift >= A.length) return hipErrorInvalidValue;
+    const BatchMPArray pre = view_of(A.data + limb_shift, A.length - (uint32_t)limb_shift, A.batch_size,
+                                     A.stride - (uint32_t)limb_shift);
+    return batch_mp_shift_bits(ctx, pre, B, -(int32_t)bit_shift);
+}
+
+hipError_t launch_leaf(BatchMPContext *ctx, int is_head, uint32_t *P, uint32_t *Q, uint32_t *R, uint32_t count) {
+    if (count == 0) return hipSuccess;
+    uint32_t blocks = ceil_div(count, 256);
+    if (blocks > 65536u * 4) blocks = 65536u * 4;
+    hipLaunchKernelGGL(leaf17_kernel, dim3(blocks), dim3(256), 0, ctx->stream, is_head, P, Q, R, count);
+    return hipGetLastError();
+}
+
+// --------------------------------------------- host big-integer helpers
+// The inverse's base case is one division of 2^L0 (L0 <= 32768 bits) by a
+// number of at most ~1024 limbs, done on the host as in the reference
+// (pi_bs_gpu.cu:1237-1253, there with GMP).  Here: Knuth's algorithm D on
+// 32-bit limbs.
+typedef std::vector<uint32_t> Limbs;
+
+void trim(Limbs &v) {
+    while (!v.empty() && v.back() == 0) v.pop_back();
+}
+
+// q = floor(u / v), r = u mod v; v non-zero.
+void divmod(const Limbs &u_in, const Limbs &v_in, Limbs &q, Limbs &r) {
+    Limbs u = u_in, v = v_in;
+    trim(u);
+    trim(v);
+    q.assign(u.size() > v.size() ? u.size() - v.size() + 1 : 1, 0);
+    if (u.size() < v.size() || (u.size() == v.size() && std::lexicographical_compare(u.rbegin(), u.rend(), v.rbegin(), v.rend()))) {
+        r = u;
+        q.assign(1, 0);
+        trim(q);
+        return;
+    }
+    const size_t n = v.size(), m = u.size() - n;
+    if (n == 1) {
+        uint64_t rem = 0;
+        for (size_t i = u.size(); i-- > 0;) {
+            const uint64_t cur = (rem << 32) | u[i];
+            q[i] = (uint32_t)(cur / v[0]);
+            rem = cur % v[0];
+        }
+        r.assign(1, (uint32_t)rem);
+        trim(q);
+        trim(r);
+        return;
+    }
+    const int s = __builtin_clz(v[n - 1]);
+    Limbs vn(n), un(u.size() + 1);
+    for (size_t i = n - 1; i > 0; --i) vn[i] = (v[i] << s) | (s ? (uint32_t)((uint64_t)v[i - 1] >> (32 - s)) : 0);
+    vn[0] = v[0] << s;
+    un[u.size()] = s ? (uint32_t)((uint64_t)u[u.size() - 1] >> (32 - s)) : 0;
+    for (size_t i = u.size() - 1; i > 0; --i) un[i] = (u[i] << s) | (s ? (uint32_t)((uint64_t)u[i - 1] >> (32 - s)) : 0);
+    un[0] = u[0] << s;
+    q.assign(m + 1, 0);
+    for (size_t j = m + 1; j-- > 0;) {
+        const uint64_t num = ((uint64_t)un[j + n] << 32) | un[j + n - 1];
+        uint64_t qhat = num / vn[n - 1];
+        uint64_t rhat = num % vn[n - 1];
+        while (qhat >= (1ull << 32) || qhat * vn[n - 2] > ((rhat << 32) | un[j + n - 2])) {
+            --qhat;
+            rhat += vn[n - 1];
+            if (rhat >= (1ull << 32)) break;
+        }
+        int64_t borrow = 0;
+        uint64_t carry = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const uint64_t p = qhat * vn[i] + carry;
+            carry = p >> 32;
+            const int64_t t = (int64_t)un[i + j] - (int64_t)(uint32_t)p + borrow;
+            un[i + j] = (uint32_t)t;
+            borrow = t >> 32;
+        }
+        const int64_t t = (int64_t)un[j + n] - (int64_t)carry + borrow;
+        un[j + n] = (uint32_t)t;
+        if (t < 0) {
+            --qhat;
+            uint64_t c = 0;
+            for (size_t i = 0; i < n; ++i) {
+                const uint64_t sum = (uint64_t)un[i + j] + vn[i] + c;
+                un[i + j] = (uint32_t)sum;
+                c = sum >> 32;
+            }
+            un[j + n] += (uint32_t)c;
+        }
+        q[j] = (uint32_t)qhat;
+    }
+    r.assign(n, 0);
+    for (size_t i = 0; i < n; ++i) r[i] = (un[i] >> s) | (s ? (uint32_t)((uint64_t)un[i + 1] << (32 - s)) : 0);
+    trim(q);
+    trim(r);
+}
+
+hipError_t download(BatchMPContext *ctx, const BatchMPArray &a, Limbs &out) {
+    out.assign(a.length, 0);
+    if (a.length == 0) return hipSuccess;
+    GPUPI_RETURN_IF(hipMemcpyAsync(out.data(), a.data, a.length * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    return hipStreamSynchronize(ctx->stream);
+}
+
+// pi_bs_gpu.cu:1128-1141: write `stride` words (value zero-extended); the
+// view's length becomes the used length (at least 1).
+hipError_t upload(BatchMPContext *ctx, const Limbs &v, BatchMPArray &a) {
+    Limbs t = v;
+    trim(t);
+    uint32_t used = t.empty() ? 1u : (uint32_t)t.size();
+    if (used > a.stride) return hipErrorInvalidValue;
+    t.resize(a.stride, 0);
+    GPUPI_RETURN_IF(hipMemcpyAsync(a.data, t.data(), a.stride * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                   ctx->stream));
+    GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+    a.length = used;
+    return hipSuccess;
+}
+
+}  // namespace
+
+// ====================================================================== API
+extern "C" {
+
+hipError_t gpupi_construct_leaf_17(BatchMPContext *ctx, int is_head, uint32_t *P, uint32_t *Q, uint32_t *R,
+                                   uint32_t count) {
+    if (ctx == nullptr || (count > 0 && (!P || !Q || !R))) return hipErrorInvalidValue;
+    return launch_leaf(ctx, is_head, P, Q, R, count);
+}
+
+// pi_bs_gpu.cu:503-792
+hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t N, BatchMPArray *Q_out, BatchMPArray *R_out) {
+    if (ctx == nullptr || Q_out == nullptr || R_out == nullptr || N == 0 || (N & (N - 1)) != 0) return hipErrorInvalidValue;
+    const uint32_t n = N * 16u;
+    const uint32_t div_lo = 830297u, div_hi = 156590819u;  // 13^2 17^3, 7^6 11^3 (pi_bs_gpu.cu:151-152)
+    Buf bP0, bQ0, bR0, bP, bQ, bR, bPn, bQn, bRn, bR1, bR2;
+    BatchMPArray P0, Q0, R0, P, Q, R;
+    GPUPI_RETURN_IF(bP0.view(N, 3, P0));
+    GPUPI_RETURN_IF(bQ0.view(N, 4, Q0));
+    GPUPI_RETURN_IF(bR0.view(N, 6, R0));
+    GPUPI_RETURN_IF(bP.view(n, 3, P));
+    GPUPI_RETURN_IF(bQ.view(n, 4, Q));
+    GPUPI_RETURN_IF(bR.view(n, 6, R));
+    // capacities as the reference pre-allocates (pi_bs_gpu.cu:611-617)
+    GPUPI_RETURN_IF(bPn.reserve((size_t)(n / 2) * 6));
+    GPUPI_RETURN_IF(bQn.reserve((size_t)(n / 2) * 10));
+    GPUPI_RETURN_IF(bRn.reserve((size_t)(n / 2) * 12));
+    GPUPI_RETURN_IF(bR1.reserve((size_t)(n / 2) * 11));
+    GPUPI_RETURN_IF(bR2.reserve((size_t)(n / 2) * 9));
+    GPUPI_RETURN_IF(launch_leaf(ctx, 1, P0.data, Q0.data, R0.data, N));
+    GPUPI_RETURN_IF(launch_leaf(ctx, 0, P.data, Q.data, R.data, n));
+
+    for (uint32_t n1 = n; n1 > 1; n1 >>= 1) {
+        const bool subtract = (n1 == n) || (n1 == (n >> 4));
+        const uint32_t ignore_last = (n1 <= N) ? 1u : 0u;
+        const uint32_t half = n1 / 2;
+        BatchMPArray Pn, Qn, Rn, R1, R2;
+        // P: the last group's P is never used once rows are whole groups
+        GPUPI_RETURN_IF(bPn.view(half - ignore_last, P.length * 2, Pn));
+        BatchMPArray Pe = view_of(P.data, P.length, half - ignore_last, P.stride * 2);
+        BatchMPArray Po = view_of(P.data + P.stride, P.length, half - ignore_last, P.stride * 2);
+        if (Po.batch_size > 0) {
+            GPUPI_RETURN_IF(batch_mp_mul(ctx, Pe, Po, Pn));
+            GPUPI_RETURN_IF(compact(ctx, Pn));
+        }
+        GPUPI_RETURN_IF(bQn.view(half, Q.length * 2, Qn));
+        const BatchMPArray Qe = view_of(Q.data, Q.length, half, Q.stride * 2);
+        const BatchMPArray Qo = view_of(Q.data + Q.stride, Q.length, half, Q.stride * 2);
+        GPUPI_RETURN_IF(batch_mp_mul(ctx, Qe, Qo, Qn));
+        GPUPI_RETURN_IF(compact(ctx, Qn));
+        const BatchMPArray Re = view_of(R.data, R.length, half, R.stride * 2);
+        const BatchMPArray Ro = view_of(R.data + R.stride, R.length, half, R.stride * 2);
+        GPUPI_RETURN_IF(bR1.view(half, R.length + Q.length, R1));
+        GPUPI_RETURN_IF(batch_mp_mul(ctx, Re, Qo, R1));
+        // R(a,c) = R(a,b) Q(b,c) 2^(15 terms(b,c)) +- P(a,b) R(b,c)
+        const uint32_t terms_right = (n / n1) + ((n / n1) >> 4);
+        const uint32_t shift = 15u * terms_right;
+        Pe.batch_size = half;
+        GPUPI_RETURN_IF(bR2.view(half, R.length + P.length, R2));
+        GPUPI_RETURN_IF(batch_mp_mul(ctx, Pe, Ro, R2));
+        const uint32_t shifted_len = R1.length + (shift >> 5) + ((shift & 31u) != 0);
+        const uint32_t rlen = std::max(shifted_len, R2.length) + (subtract ? 0u : 1u);
+        GPUPI_RETURN_IF(bRn.view(half, rlen, Rn));
+        GPUPI_RETURN_IF(subtract ? batch_mp_shift_sub(ctx, R1, R2, shift, Rn) : batch_mp_shift_add(ctx, R1, R2, shift, Rn));
+        GPUPI_RETURN_IF(compact(ctx, Rn));
+        bP.swap(bPn);
+        bQ.swap(bQn);
+        bR.swap(bRn);
+        P = Pn;
+        Q = Qn;
+        R = Rn;
+
+        if (n1 == (n >> 3)) {
+            // append the head term 17g+16 of each group, then divide out the
+            // block's common factor (pi_bs_gpu.cu:725-779)
+            BatchMPArray Ph, Qh, Rh, Rp1, Rp2;
+            GPUPI_RETURN_IF(bPn.view(half, P.length + P0.length, Ph));
+            GPUPI_RETURN_IF(batch_mp_mul(ctx, P, P0, Ph));
+            GPUPI_RETURN_IF(compact(ctx, Ph));
+            GPUPI_RETURN_IF(bQn.view(half, Q.length + Q0.length, Qh));
+            GPUPI_RETURN_IF(batch_mp_mul(ctx, Q, Q0, Qh));
+            GPUPI_RETURN_IF(compact(ctx, Qh));
+            GPUPI_RETURN_IF(bR1.view(half, R.length + Q0.length, Rp1));
+            GPUPI_RETURN_IF(batch_mp_mul(ctx, R, Q0, Rp1));
+            GPUPI_RETURN_IF(bR2.view(half, P.length + R0.length, Rp2));
+            GPUPI_RETURN_IF(batch_mp_mul(ctx, P, R0, Rp2));
+            const uint32_t hl = std::max(Rp1.length + 1, Rp2.length) + 1;
+            GPUPI_RETURN_IF(bRn.view(half, hl, Rh));
+            GPUPI_RETURN_IF(batch_mp_shift_add(ctx, Rp1, Rp2, 15u, Rh));
+            GPUPI_RETURN_IF(compact(ctx, Rh));
+            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Ph, div_lo));
+            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Ph, div_hi));
+            GPUPI_RETURN_IF(compact(ctx, Ph));
+            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Qh, div_lo));
+            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Qh, div_hi));
+            GPUPI_RETURN_IF(compact(ctx, Qh));
+            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Rh, div_lo));
+            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Rh, div_hi));
+            GPUPI_RETURN_IF(compact(ctx, Rh));
+            bP.swap(bPn);
+            bQ.swap(bQn);
+            bR.swap(bRn);
+            P = Ph;
+            Q = Qh;
+            R = Rh;
+        }
+    }
+    // Q and R now hold one row each; hand their buffers to the caller.
+    *Q_out = view_of(bQ.release(), Q.length, 1, Q.stride);
+    *R_out = view_of(bR.release(), R.length, 1, R.stride);
+    return hipSuccess;
+}
+
+// pi_bs_gpu.cu:1346-1522
+hipError_t gpupi_sqrt_10005(BatchMPContext *ctx, int target_digits, BatchMPArray *P_out, BatchMPArray *Q_out) {
+    if (ctx == nullptr || P_out == nullptr || Q_out == nullptr) return hipErrorInvalidValue;
+    Buf bP, bQ, bPQ, bPP, bQQ, bQQ5, bPn, bQn;
+    BatchMPArray P, Q;
+    GPUPI_RETURN_IF(bP.view(1, 1, P));
+    GPUPI_RETURN_IF(bQ.view(1, 1, Q));
+    const uint32_t init[2] = {4001u, 40u};
+    GPUPI_RETURN_IF(hipMemcpyAsync(P.data, &init[0], 4, hipMemcpyHostToDevice, ctx->stream));
+    GPUPI_RETURN_IF(hipMemcpyAsync(Q.data, &init[1], 4, hipMemcpyHostToDevice, ctx->stream));
+    GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+    double valid_digits = 7.5;
+    while (valid_digits < target_digits) {
+        valid_digits *= 2;
+        BatchMPArray PQ, Qn, PP, QQ, QQ5, Pn;
+        GPUPI_RETURN_IF(bPQ.view(1, P.length + Q.length, PQ));
+        GPUPI_RETURN_IF(batch_mp_mul(ctx, P, Q, PQ));
+        GPUPI_RETURN_IF(compact(ctx, PQ));
+        GPUPI_RETURN_IF(bQn.view(1, PQ.length + 1, Qn));
+        GPUPI_RETURN_IF(batch_mp_shift_bits(ctx, PQ, Qn, 1));
+        GPUPI_RETURN_IF(compact(ctx, Qn));
+        GPUPI_RETURN_IF(bPP.view(1, P.length * 2, PP));
+        GPUPI_RETURN_IF(batch_mp_mul(ctx, P, P, PP));
+        GPUPI_RETURN_IF(compact(ctx, PP));
+        GPUPI_RETURN_IF(bQQ.view(1, Q.length * 2, QQ));
+        GPUPI_RETURN_IF(batch_mp_mul(ctx, Q, Q, QQ));
+        GPUPI_RETURN_IF(compact(ctx, QQ));
+        GPUPI_RETURN_IF(bQQ5.view(1, QQ.length + 1, QQ5));
+        GPUPI_RETURN_IF(batch_mp_mul_small(ctx, QQ, 10005u, QQ5));
+        GPUPI_RETURN_IF(compact(ctx, QQ5));
+        GPUPI_RETURN_IF(bPn.view(1, std::max(PP.length, QQ5.length) + 1, Pn));
+        GPUPI_RETURN_IF(batch_mp_add(ctx, PP, QQ5, Pn));
+        GPUPI_RETURN_IF(compact(ctx, Pn));
+        bP.swap(bPn);
+        bQ.swap(bQn);
+        P = Pn;
+        Q = Qn;
+    }
+    *P_out = view_of(bP.release(), P.length, 1, P.stride);
+    *Q_out = view_of(bQ.release(), Q.length, 1, Q.stride);
+    return hipSuccess;
+}
+
+// pi_bs_gpu.cu:1143-1344: y ~ 2^L / x by precision doubling from a host base
+// case.  One flat workspace holds res | res2 | y2 exactly as the reference
+// lays it out, so `res << m` is a view that extends downwards.
+hipError_t gpupi_recursive_inverse(BatchMPContext *ctx, BatchMPArray x, uint64_t L, BatchMPArray *y_out) {
+    if (ctx == nullptr || y_out == nullptr || x.batch_size != 1 || x.data == nullptr) return hipErrorInvalidValue;
+    const uint64_t cpu_limit = 32u * 1024u;
+    std::vector<uint64_t> levels{L};
+    while (levels.back() > cpu_limit) {
+        const uint64_t m = ((levels.back() - 2u) / 4u) & ~31ull;
+        levels.push_back(levels.back() - m * 2u);
+    }
+    const size_t L_res = (size_t)((L + 31) / 32 + 2), L_res2 = (size_t)((L + 31) / 32 + 1),
+                 L_y2 = (size_t)x.length + 1;
+    Buf by, bws;
+    BatchMPArray y;
+    GPUPI_RETURN_IF(by.view(1, x.length, y));
+    GPUPI_RETURN_IF(gpupi::launch_fill_u32(y.data, x.length, 0u, ctx->stream));
+    GPUPI_RETURN_IF(bws.reserve(L_res + L_res2 + L_y2));
+    uint32_t *res_base = bws.ptr, *res2_base = bws.ptr + L_res, *y2_base = res2_base + L_res2;
+    GPUPI_RETURN_IF(gpupi::launch_fill_u32(res_base, L_res, 0u, ctx->stream));
+    GPUPI_RETURN_IF(gpupi::launch_fill_u32(res2_base, L_res2 + L_y2, 0xA5A5A5A5u, ctx->stream));
+
+    BatchMPArray res;
+    {
+        const uint64_t L0 = levels.back();
+        const uint32_t start = (uint32_t)((L - L0) >> 6);
+        const BatchMPArray x0 = make_subview(x, start, x.length - start);
+        BatchMPArray y0 = make_subview(y, start, y.length - start);
+        res = view_of(res_base + (size_t)start * 2, x0.length + 1, 1, x0.length + 1);
+        if (!inside(res, bws.ptr, bws.cap)) return hipErrorInvalidValue;
+        Limbs xv, num((size_t)(L0 / 32) + 1, 0), qv, rv;
+        GPUPI_RETURN_IF(download(ctx, x0, xv));
+        num[L0 / 32] = 1u << (L0 % 32);
+        trim(xv);
+        if (xv.empty()) return hipErrorInvalidValue;
+        divmod(num, xv, qv, rv);
+        GPUPI_RETURN_IF(upload(ctx, qv, y0));
+        GPUPI_RETURN_IF(upload(ctx, rv, res));
+    }
+    GPUPI_RETURN_IF(compact(ctx, res));
+
+    for (size_t i = levels.size() - 1; i > 0; --i) {
+        const uint64_t L1 = levels[i], L2 = levels[i - 1];
+        const uint64_t m = (L2 - L1) >> 1;
+        const uint32_t mw = (uint32_t)(m >> 5);
+        const BatchMPArray x2 = make_subview(x, (uint32_t)((L - L2) >> 6), mw);
+        const BatchMPArray y1 = make_subview(y, (uint32_t)((L - L1) >> 6), y.length - (uint32_t)((L - L1) >> 6));
+        const BatchMPArray x2y1 = view_of(res2_base, x2.length + y1.length, 1, x2.length + y1.length);
+        BatchMPArray res1_m = view_of(res.data - mw, res.length + mw + 1, 1, res.length + mw + 1);
+        if (!inside(x2y1, bws.ptr, bws.cap) || !inside(res1_m, bws.ptr, bws.cap)) return hipErrorInvalidValue;
+        GPUPI_RETURN_IF(batch_mp_mul(ctx, x2, y1, x2y1));
+        GPUPI_RETURN_IF(batch_mp_sub(ctx, res1_m, x2y1, res1_m));
+        uint32_t top = 0;
+        GPUPI_RETURN_IF(hipMemcpyAsync(ctx->h_scalar, res1_m.data + res1_m.length - 1, 4, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+        GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+        top = *ctx->h_scalar;
+        const bool negative = top >= 0x80000000u;
+        if (negative) {
+            GPUPI_RETURN_IF(batch_mp_sub(ctx, view_of(res1_m.data, 0, 1, 0), res1_m, res1_m));
+        }
+        const BatchMPArray res2 = view_of(res2_base, res1_m.length + y1.length, 1, res1_m.length + y1.length);
+        GPUPI_RETURN_IF(batch_mp_mul(ctx, res1_m, y1, res2));
+        const uint32_t drop = (uint32_t)((L2 - m * 2) >> 5);
+        const BatchMPArray y2 = view_of(y2_base, res2.length - drop, 1, res2.length - drop);
+        if (res2.length < drop || !inside(res2, bws.ptr, bws.cap) || !inside(y2, bws.ptr, bws.cap)) return hipErrorInvalidValue;
+        GPUPI_RETURN_IF(shift_right_large(ctx, res2, L2 - m * 2u, y2));
+        if (negative) GPUPI_RETURN_IF(batch_mp_add_small(ctx, y2, 1u, y2));
+        const BatchMPArray y_new = make_subview(y, (uint32_t)((L - L2) >> 6), y.length - (uint32_t)((L - L2) >> 6));
+        if (!negative) GPUPI_RETURN_IF(batch_mp_add(ctx, y_new, y2, y_new));
+        else GPUPI_RETURN_IF(batch_mp_sub(ctx, y_new, y2, y_new));
+        if (L2 != L) {
+            const BatchMPArray x_next = make_subview(x, (uint32_t)((L - L2) >> 6), x.length - (uint32_t)((L - L2) >> 6));
+            res = view_of(res1_m.data - mw, res1_m.length + mw, 1, res1_m.length + mw);
+            const BatchMPArray y2x = view_of(res2_base, y2.length + x_next.length, 1, y2.length + x_next.length);
+            if (!inside(res, bws.ptr, bws.cap) || !inside(y2x, bws.ptr, bws.cap)) return hipErrorInvalidValue;
+            GPUPI_RETURN_IF(batch_mp_mul(ctx, y2, x_next, y2x));
+            if (!negative) GPUPI_RETURN_IF(batch_mp_sub(ctx, res, y2x, res));
+            else GPUPI_RETURN_IF(batch_mp_sub(ctx, y2x, res, res));
+            GPUPI_RETURN_IF(compact(ctx, res));
+        }
+    }
+    GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+    *y_out = view_of(by.release(), y.length, 1, y.stride);
+    return hipSuccess;
+}
+
+// pi_bs_gpu.cu:1524-1685
+hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPArray *ret_out, uint32_t *groups_out,
+                               uint64_t *inverse_bits_out) {
+    if (ctx == nullptr || ret_out == nullptr) return hipErrorInvalidValue;
+    int N = 1;
+    while (N * 17 * 14.3 < target_digits) N *= 2;
+    const int prec_limbs = (int)(((long)(target_digits * 3.322) + 31) / 32 + 1);
+
+    BatchMPArray Q_bs{}, R_bs{};
+    GPUPI_RETURN_IF(gpupi_binary_split(ctx, (uint32_t)N, &Q_bs, &R_bs));
+    Buf bQbs, bRbs;
+    bQbs.ptr = Q_bs.data;
+    bQbs.cap = Q_bs.stride;
+    bRbs.ptr = R_bs.data;
+    bRbs.cap = R_bs.stride;
+    const unsigned long long r_total_shift = 15ull * (unsigned long long)N * 17ull - 8ull;
+    const int rs_bits = (int)(r_total_shift % 32), rs_limbs = (int)(r_total_shift / 32);
+    if ((uint32_t)rs_limbs >= R_bs.length) return hipErrorInvalidValue;
+    const BatchMPArray R_pre = view_of(R_bs.data + rs_limbs, R_bs.length - (uint32_t)rs_limbs, 1,
+                                       R_bs.length - (uint32_t)rs_limbs);
+    Buf bRsh;
+    BatchMPArray R_sh;
+    GPUPI_RETURN_IF(bRsh.view(1, R_pre.length + 1, R_sh));
+    GPUPI_RETURN_IF(batch_mp_shift_bits(ctx, R_pre, R_sh, -rs_bits));
+    GPUPI_RETURN_IF(compact(ctx, R_sh));
+
+    long extra = std::max(0l, (long)std::min(Q_bs.length, R_sh.length) - prec_limbs);
+    const BatchMPArray Q_tr = view_of(Q_bs.data + extra, Q_bs.length - (uint32_t)extra, 1, Q_bs.length - (uint32_t)extra);
+    const BatchMPArray R_tr = view_of(R_sh.data + extra, R_sh.length - (uint32_t)extra, 1, R_sh.length - (uint32_t)extra);
+
+    BatchMPArray P15{}, Q15{};
+    GPUPI_RETURN_IF(gpupi_sqrt_10005(ctx, target_digits, &P15, &Q15));
+    Buf bP15, bQ15;
+    bP15.ptr = P15.data;
+    bP15.cap = P15.stride;
+    bQ15.ptr = Q15.data;
+    bQ15.cap = Q15.stride;
+    Buf bPf, bQf;
+    BatchMPArray P_final, Q_final;
+    GPUPI_RETURN_IF(bPf.view(1, P15.length + Q_tr.length, P_final));
+    GPUPI_RETURN_IF(bQf.view(1, Q15.length + R_tr.length, Q_final));
+    GPUPI_RETURN_IF(batch_mp_mul(ctx, P15, Q_tr, P_final));
+    GPUPI_RETURN_IF(batch_mp_mul(ctx, R_tr, Q15, Q_final));
+    GPUPI_RETURN_IF(compact(ctx, P_final));
+    GPUPI_RETURN_IF(compact(ctx, Q_final));
+
+    extra = std::max(0l, (long)std::min(P_final.length, Q_final.length) - prec_limbs);
+    const BatchMPArray P_tf = make_subview(P_final, (uint32_t)extra, P_final.length - (uint32_t)extra);
+    BatchMPArray Q_tf = make_subview(Q_final, (uint32_t)extra, Q_final.length - (uint32_t)extra);
+    GPUPI_RETURN_IF(compact(ctx, Q_tf));
+    uint32_t q_bits = 0;
+    if (Q_tf.length > 0) {
+        GPUPI_RETURN_IF(hipMemcpyAsync(ctx->h_scalar, Q_tf.data + Q_tf.length - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+        q_bits = (Q_tf.length - 1) * 32u + (32u - (uint32_t)__builtin_clz(*ctx->h_scalar));
+    }
+    const uint64_t inverse_bits = (uint64_t)q_bits * 2u - 1u;
+
+    BatchMPArray inv{};
+    GPUPI_RETURN_IF(gpupi_recursive_inverse(ctx, Q_tf, inverse_bits, &inv));
+    Buf binv;
+    binv.ptr = inv.data;
+    binv.cap = inv.stride;
+    Buf bprod;
+    BatchMPArray prod;
+    GPUPI_RETURN_IF(bprod.view(1, P_tf.length + inv.length, prod));
+    GPUPI_RETURN_IF(batch_mp_mul(ctx, P_tf, inv, prod));
+    GPUPI_RETURN_IF(compact(ctx, prod));
+
+    Buf bret;
+    BatchMPArray ret;
+    GPUPI_RETURN_IF(bret.view(1, prod.length - (uint32_t)(inverse_bits / 32u) + (uint32_t)prec_limbs, ret));
+    const uint64_t final_shift = inverse_bits - (uint64_t)prec_limbs * 32u;
+    GPUPI_RETURN_IF(shift_right_large(ctx, prod, final_shift, ret));
+    GPUPI_RETURN_IF(compact(ctx, ret));
+    ret.data = bret.release();
+    *ret_out = ret;
+    if (groups_out) *groups_out = (uint32_t)N;
+    if (inverse_bits_out) *inverse_bits_out = inverse_bits;
+    return hipSuccess;
+}
+
+// pi_bs_gpu.cu:924-939
+uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits) {
+    if (requested == 0) {
+        if (leaf_digits) *leaf_digits = 477u;
+        return 0;
+    }
+    uint32_t digits = requested, levels = 0;
+    while (digits >= 32u * 18u) {
+        digits >>= 1;
+        ++levels;
+    }
+    if ((digits << levels) < requested) ++digits;
+    if (leaf_digits) *leaf_digits = digits;
+    return digits << levels;
+}
+
+// pi_bs_gpu.cu:941-944
+uint32_t gpupi_prec_limbs_for_digits(uint32_t digits) {
+    const uint32_t prec_bits = (uint32_t)((double)digits * 3.322) + 31u;
+    return prec_bits / 32u + 1u;
+}
+
+}  // extern "C"
+
+// ========================================================= radix conversion
+namespace {
+
+uint32_t bits_for_digits(uint32_t digits) {  // convert_decimal.cu:36-39
+    return (uint32_t)(std::ceil((double)digits * 3.32192809488736234787) + 34.0);
+}
+
+uint32_t levels_for_shape(uint32_t total, uint32_t leaf) {  // convert_decimal.cu:41-52
+    if (leaf == 0 || total == 0 || total % leaf) return UINT32_MAX;
+    uint32_t ratio = total / leaf, lv = 0;
+    while (ratio > 1 && (ratio & 1u) == 0) {
+        ratio >>= 1;
+        ++lv;
+    }
+    return ratio == 1 ? lv : UINT32_MAX;
+}
+
+}  // namespace
+
+extern "C" {
+
+// convert_decimal.cu:198-269
+hipError_t convert_decimal_precompute_powers(BatchMPContext *ctx, ConvertDecimalPowerTable *table,
+                                             uint32_t leaf_digits, uint32_t levels) {
+    if (ctx == nullptr || table == nullptr || leaf_digits == 0) return hipErrorInvalidValue;
+    table->powers = nullptr;
+    table->levels = 0;
+    table->leaf_digits = leaf_digits;
+    if (levels == 0) return hipSuccess;
+    BatchMPArray *powers = new (std::nothrow) BatchMPArray[levels];
+    if (powers == nullptr) return hipErrorOutOfMemory;
+    for (uint32_t i = 0; i < levels; ++i) powers[i] = BatchMPArray{};
+    auto fail = [&](hipError_t e) {
+        for (uint32_t i = 0; i < levels; ++i) batch_mp_array_release(powers[i]);
+        delete[] powers;
+        return e;
+    };
+    // 5^leaf_digits on the host (at most ~1400 bits), then squarings.
+    Limbs five(1, 1);
+    for (uint32_t k = 0; k < leaf_digits; ++k) {
+        uint64_t carry = 0;
+        for (auto &w : five) {
+            const uint64_t t = (uint64_t)w * 5u + carry;
+            w = (uint32_t)t;
+            carry = t >> 32;
+        }
+        if (carry) five.push_back((uint32_t)carry);
+    }
+    powers[0] = batch_mp_array_create(1, (uint32_t)five.size());
+    if (powers[0].data == nullptr) return fail(hipErrorOutOfMemory);
+    hipError_t err = hipMemcpyAsync(powers[0].data, five.data(), five.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
+    if (err != hipSuccess) return fail(err);
+    for (uint32_t l = 1; l < levels; ++l) {
+        powers[l] = batch_mp_array_create(1, powers[l - 1].length * 2);
+        if (powers[l].data == nullptr) return fail(hipErrorOutOfMemory);
+        err = batch_mp_mul(ctx, powers[l - 1], powers[l - 1], powers[l]);
+        if (err == hipSuccess) err = batch_mp_array_compact(ctx, &powers[l]);
+        if (err != hipSuccess) return fail(err);
+    }
+    table->powers = powers;
+    table->levels = levels;
+    return hipSuccess;
+}
+
+void convert_decimal_release_powers(ConvertDecimalPowerTable *table) {
+    if (table == nullptr) return;
+    if (table->powers) {
+        for (uint32_t i = 0; i < table->levels; ++i) batch_mp_array_release(table->powers[i]);
+        delete[] table->powers;
+    }
+    table->powers = nullptr;
+    table->levels = 0;
+}
+
+// convert_decimal.cu:285-474.  The leaf digits go straight to their natural
+// positions (the reference writes reversed chunks and reorders them).
+hipError_t convert_decimal_equal_split(BatchMPContext *ctx, BatchMPArray input, uint32_t input_bits,
+                                       uint32_t total_digits, uint32_t leaf_digits,
+                                       const ConvertDecimalPowerTable *table, char *output_digits) {
+    if (ctx == nullptr || output_digits == nullptr || input.batch_size != 1 || input.data == nullptr)
+        return hipErrorInvalidValue;
+    const uint32_t levels = levels_for_shape(total_digits, leaf_digits);
+    if (levels == UINT32_MAX || table == nullptr || table->leaf_digits != leaf_digits || table->levels != levels)
+        return hipErrorInvalidValue;
+    std::vector<uint32_t> dig(levels + 1), bits(levels + 1), limbs(levels + 1);
+    dig[0] = total_digits;
+    bits[0] = input_bits;
+    limbs[0] = input.length;
+    for (uint32_t l = 1; l <= levels; ++l) {
+        dig[l] = dig[l - 1] >> 1;
+        bits[l] = bits_for_digits(dig[l]);
+        limbs[l] = ceil_div(bits[l], 32);
+    }
+    Buf bcur, bnext, bmant, bprod;
+    BatchMPArray cur = input;
+    if (levels == 0) {
+        const uint32_t need = bits_for_digits(total_digits);
+        if (input_bits < need) return hipErrorInvalidValue;
+        BatchMPArray t;
+        GPUPI_RETURN_IF(bcur.view(1, ceil_div(need, 32), t));
+        GPUPI_RETURN_IF(gpupi::launch_extract_shifted(input, t, 1, input_bits - need, 0, 2, ctx->stream));
+        cur = t;
+    } else {
+        for (uint32_t l = 0; l < levels; ++l) {
+            const uint32_t child_digits = dig[l + 1], child_bits = bits[l + 1];
+            if (cur.length < limbs[l]) return hipErrorInvalidValue;
+            if (bits[l] <= child_digits || bits[l] < child_bits) return hipErrorInvalidValue;
+            BatchMPArray next, mant, prod;
+            GPUPI_RETURN_IF(bnext.view(cur.batch_size * 2u, limbs[l + 1], next));
+            GPUPI_RETURN_IF(gpupi::launch_extract_shifted(cur, next, cur.batch_size, bits[l] - child_bits, 0, 2,
+                                                          ctx->stream));
+            const uint32_t mant_bits = bits[l] - child_digits;
+            GPUPI_RETURN_IF(bmant.view(cur.batch_size, ceil_div(mant_bits, 32), mant));
+            GPUPI_RETURN_IF(gpupi::launch_extract_low(cur, mant, cur.batch_size, mant_bits, ctx->stream));
+            const BatchMPArray &pw = table->powers[levels - 1 - l];
+            const BatchMPArray power = view_of(pw.data, pw.length, cur.batch_size, 0);
+            GPUPI_RETURN_IF(bprod.view(cur.batch_size, mant.length + power.length, prod));
+            GPUPI_RETURN_IF(batch_mp_mul(ctx, mant, power, prod));
+            GPUPI_RETURN_IF(gpupi::launch_extract_shifted(prod, next, cur.batch_size, mant_bits - child_bits, 1, 2,
+                                                          ctx->stream));
+            bcur.swap(bnext);
+            cur = next;
+        }
+    }
+    Buf baligned;
+    BatchMPArray aligned;
+    GPUPI_RETURN_IF(baligned.view(cur.batch_size, cur.length, aligned));
+    const uint32_t leaf_bits = levels == 0 ? bits_for_digits(total_digits) : bits[levels];
+    GPUPI_RETURN_IF(batch_mp_shift_bits(ctx, cur, aligned, (int32_t)(cur.length * 32u - leaf_bits)));
+    GPUPI_RETURN_IF(gpupi::launch_decimal_leaf(aligned.data, output_digits, aligned.batch_size, aligned.length,
+                                               aligned.stride, leaf_digits, leaf_digits, true, ctx->stream));
+    return hipStreamSynchronize(ctx->stream);
+}
+
+// pi_bs_gpu.cu:2073-2193: the reference's timed region.
+hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec_convention, GpupiPiResult *result) {
+    if (ctx == nullptr || result == nullptr) return hipErrorInvalidValue;
+    std::memset(result, 0, sizeof(*result));
+    uint32_t leaf = 477;
+    const uint32_t convert_digits = gpupi_round_up_decimal_digits(requested_digits, &leaf);
+    const uint32_t prec_digits = prec_convention == GPUPI_PREC_ROUNDED ? convert_digits : requested_digits;
+    const uint32_t prec_limbs = gpupi_prec_limbs_for_digits(prec_digits);
+    const uint32_t levels = convert_digits ? (uint32_t)__builtin_ctz(convert_digits / leaf) : 0u;
+    hipEvent_t start = nullptr, stop = nullptr;
+    GPUPI_RETURN_IF(hipEventCreate(&start));
+    GPUPI_RETURN_IF(hipEventCreate(&stop));
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{start, stop};
+    GPUPI_RETURN_IF(hipEventRecord(start, ctx->stream));
+    BatchMPArray ret{};
+    uint32_t groups = 0;
+    uint64_t inverse_bits = 0;
+    GPUPI_RETURN_IF(gpupi_pi_fractional(ctx, (int)prec_digits, &ret, &groups, &inverse_bits));
+    ConvertDecimalPowerTable powers{};
+    hipError_t err = convert_decimal_precompute_powers(ctx, &powers, leaf, levels);
+    char *digits = nullptr;
+    if (err == hipSuccess) err = hipMalloc(&digits, std::max<uint32_t>(convert_digits, 1));
+    if (err == hipSuccess && requested_digits > 0) {
+        const BatchMPArray frac = make_subview(ret, 0, std::min(prec_limbs, ret.length));
+        err = convert_decimal_equal_split(ctx, frac, prec_limbs * 32u, convert_digits, leaf, &powers, digits);
+    }
+    convert_decimal_release_powers(&powers);
+    if (err == hipSuccess) err = hipEventRecord(stop, ctx->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(stop);
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, start, stop);
+    if (err != hipSuccess) {
+        batch_mp_array_release(ret);
+        if (digits) (void)hipFree(digits);
+        return err;
+    }
+    result->requested_digits = requested_digits;
+    result->convert_digits = convert_digits;
+    result->leaf_digits = leaf;
+    result->prec_limbs = prec_limbs;
+    result->groups = groups;
+    result->ret_length = ret.length;
+    result->ret = ret.data;
+    result->digits = digits;
+    result->elapsed_ms = ms;
+    result->inverse_bits = inverse_bits;
+    return hipSuccess;
+}
+
+void gpupi_pi_free(GpupiPiResult *result) {
+    if (result == nullptr) return;
+    if (result->ret) (void)hipFree(result->ret);
+    if (result->digits) (void)hipFree(result->digits);
+    result->ret = nullptr;
+    result->digits = nullptr;
+}
+
+}  // extern "C"

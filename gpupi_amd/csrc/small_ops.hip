@@ -1,0 +1,224 @@
+// Bit shift, exact division by an odd word, and batch length reductions.
+#include "common.hpp"
+
+namespace gpupi {
+namespace {
+
+// Word j of (A << s) or (A >> -s) for one row (reference batch_shift.h:5-15).
+__device__ __forceinline__ uint32_t shifted_word(const uint32_t *a, uint32_t L_in, uint64_t j,
+                                                 int64_t shift) {
+    if (shift >= 0) {
+        const uint64_t w = (uint64_t)shift >> 5;
+        const uint32_t r = (uint32_t)shift & 31u;
+        if (j < w) return 0u;
+        const uint64_t i = j - w;
+        const uint32_t hi = i < L_in ? a[i] : 0u;
+        if (r == 0) return hi;
+        const uint32_t lo = (i >= 1 && i - 1 < L_in) ? a[i - 1] : 0u;
+        return (hi << r) | (lo >> (32u - r));
+    }
+    const uint64_t s = (uint64_t)(-shift);
+    const uint64_t i = j + (s >> 5);
+    const uint32_t r = (uint32_t)s & 31u;
+    const uint32_t lo = i < L_in ? a[i] : 0u;
+    if (r == 0) return lo;
+    const uint32_t hi = i + 1 < L_in ? a[i + 1] : 0u;
+    return (lo >> r) | (hi << (32u - r));
+}
+
+__global__ void __launch_bounds__(256) shift_bits_kernel(const uint32_t *A, uint32_t *B, uint32_t L_in,
+                                                         uint32_t L_out, uint32_t stride_in,
+                                                         uint32_t stride_out, int64_t shift,
+                                                         uint64_t total) {
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)blockDim.x * gridDim.x) {
+        const uint64_t row = idx / L_out;
+        const uint64_t j = idx - row * L_out;
+        B[row * stride_out + j] = shifted_word(A + row * stride_in, L_in, j, shift);
+    }
+}
+
+// 64 rows per block; rows staged through LDS so global accesses stay
+// contiguous, then one thread runs each row's Hensel division
+// (q_i = (a_i - borrow) * b^-1 mod 2^32, borrow = hi(q_i b) + underflow).
+constexpr int kDivRows = 64;
+constexpr int kDivMaxL = 128;
+
+__global__ void __launch_bounds__(64) exactdiv_kernel(uint32_t *A, uint32_t b, uint32_t binv, uint32_t N,
+                                                      uint32_t L, uint32_t stride_A) {
+    __shared__ uint32_t tile[kDivRows * (kDivMaxL + 1)];
+    const uint32_t r0 = blockIdx.x * kDivRows;
+    const uint32_t rows = min((uint32_t)kDivRows, N - r0);
+    const uint32_t pitch = L + 1;
+    for (uint32_t idx = threadIdx.x; idx < rows * L; idx += blockDim.x) {
+        const uint32_t r = idx / L, j = idx - r * L;
+        tile[r * pitch + j] = A[(size_t)(r0 + r) * stride_A + j];
+    }
+    __syncthreads();
+    if (threadIdx.x < rows) {
+        uint32_t *t = tile + threadIdx.x * pitch;
+        uint32_t borrow = 0;
+        for (uint32_t j = 0; j < L; ++j) {
+            const uint32_t a = t[j];
+            const uint32_t s = a - borrow;
+            const uint32_t under = a < borrow;
+            const uint32_t q = s * binv;
+            t[j] = q;
+            borrow = __umulhi(q, b) + under;
+        }
+    }
+    __syncthreads();
+    for (uint32_t idx = threadIdx.x; idx < rows * L; idx += blockDim.x) {
+        const uint32_t r = idx / L, j = idx - r * L;
+        A[(size_t)(r0 + r) * stride_A + j] = tile[r * pitch + j];
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t length_of(uint32_t top_index_plus_one, uint32_t top_word, bool bits) {
+    if (top_index_plus_one == 0) return 0u;
+    if (!bits) return top_index_plus_one;
+    return (top_index_plus_one - 1) * 32u + (32u - (uint32_t)__clz(top_word));
+}
+
+// Rows of at most 64 limbs: one thread per row scanning down from the top.
+__global__ void __launch_bounds__(256) length_thread_kernel(const uint32_t *A, uint32_t N, uint32_t L,
+                                                            uint32_t stride_A, int bits, uint32_t *result) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t len = 0;
+    if (row < N) {
+        const uint32_t *a = A + (size_t)row * stride_A;
+        uint32_t j = L;
+        while (j > 0 && a[j - 1] == 0u) --j;
+        len = length_of(j, j ? a[j - 1] : 0u, bits != 0);
+    }
+    len = wave_max(len);
+    if ((threadIdx.x & (kWave - 1)) == 0 && len != 0) atomicMax(result, len);
+}
+
+// Longer rows: one wave per row, 64 limbs per step from the top down.
+__global__ void __launch_bounds__(256) length_wave_kernel(const uint32_t *A, uint32_t N, uint32_t L,
+                                                          uint32_t stride_A, int bits, uint32_t *result) {
+    const uint32_t row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (row >= N) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t *a = A + (size_t)row * stride_A;
+    for (int64_t top = L; top > 0; top -= kWave) {
+        const int64_t j = top - kWave + (int64_t)lane;
+        const uint32_t v = (j >= 0) ? a[j] : 0u;
+        const uint64_t nz = ballot64(v != 0u);
+        if (nz != 0) {
+            const int hi_lane = 63 - __builtin_clzll(nz);
+            const uint32_t w = __shfl(v, hi_lane);
+            if (lane == 0) {
+                const uint32_t idx1 = (uint32_t)(top - kWave + hi_lane + 1);
+                atomicMax(result, length_of(idx1, w, bits != 0));
+            }
+            return;
+        }
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t count, uint32_t value) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)blockDim.x * gridDim.x)
+        p[i] = value;
+}
+
+uint32_t grid_for(uint64_t total, uint32_t block) {
+    uint64_t g = ceil_div64(total, block);
+    if (g > 65536ull * 16) g = 65536ull * 16;
+    return (uint32_t)(g ? g : 1);
+}
+
+uint32_t odd_inverse_u32(uint32_t b) {
+    uint32_t x = b;  // correct to 3 bits for odd b
+    for (int i = 0; i < 5; ++i) x *= 2u - b * x;
+    return x;
+}
+
+}  // namespace
+
+hipError_t launch_shift_bits(const uint32_t *A, uint32_t *B, uint32_t N, uint32_t L_in, uint32_t L_out,
+                             uint32_t stride_in, uint32_t stride_out, int64_t shift_bits,
+                             hipStream_t stream) {
+    const uint64_t total = (uint64_t)N * L_out;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(shift_bits_kernel, dim3(grid_for(total, 256)), dim3(256), 0, stream, A, B, L_in,
+                       L_out, stride_in, stride_out, shift_bits, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_exactdiv_small(uint32_t *A, uint32_t B, uint32_t N, uint32_t L, uint32_t stride_A,
+                                 hipStream_t stream) {
+    if (N == 0 || L == 0) return hipSuccess;
+    if (L > (uint32_t)kDivMaxL || (B & 1u) == 0u) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(exactdiv_kernel, dim3(ceil_div(N, kDivRows)), dim3(kDivRows), 0, stream, A, B,
+                       odd_inverse_u32(B), N, L, stride_A);
+    return hipGetLastError();
+}
+
+hipError_t launch_length_max(const uint32_t *A, uint32_t N, uint32_t L, uint32_t stride_A, bool bits,
+                             uint32_t *d_result, hipStream_t stream) {
+    if (N == 0 || L == 0) return hipSuccess;
+    if (L <= 64) {
+        hipLaunchKernelGGL(length_thread_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, stream, A, N, L,
+                           stride_A, bits ? 1 : 0, d_result);
+    } else {
+        hipLaunchKernelGGL(length_wave_kernel, dim3(ceil_div(N, 4)), dim3(256), 0, stream, A, N, L,
+                           stride_A, bits ? 1 : 0, d_result);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(count, 256)), dim3(256), 0, stream, p, count, value);
+    return hipGetLastError();
+}
+
+}  // namespace gpupi
+
+// ---------------------------------------------------------------------------
+// C entry points (reference batch_shift.h, batch_exactdiv_small.h,
+// batch_bitlength.h); null stream.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void batch_shift_bits(const uint32_t *A, uint32_t *B, uint32_t N, uint32_t L_in, uint32_t L_out,
+                      uint32_t stride_in, uint32_t stride_out, int32_t shift_bits) {
+    (void)gpupi::launch_shift_bits(A, B, N, L_in, L_out, stride_in, stride_out, shift_bits, 0);
+}
+
+void batch_exactdiv_small(uint32_t *A, uint32_t B, uint32_t N, uint32_t L, uint32_t stride_A) {
+    (void)gpupi::launch_exactdiv_small(A, B, N, L, stride_A, 0);
+}
+
+size_t batch_bitlength_workspace_size(uint32_t, uint32_t) { return sizeof(uint32_t); }
+
+static uint32_t length_entry(const uint32_t *A, uint32_t *ws, uint32_t N, uint32_t L, uint32_t sA,
+                             bool bits) {
+    uint32_t h = 0;
+    if (ws == nullptr) return 0;
+    if (gpupi::launch_fill_u32(ws, 1, 0u, 0) != hipSuccess) return 0;
+    if (gpupi::launch_length_max(A, N, L, sA, bits, ws, 0) != hipSuccess) return 0;
+    if (hipMemcpy(&h, ws, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    return h;
+}
+
+uint32_t batch_bitlength_max(const uint32_t *A, uint32_t *workspace, uint32_t N, uint32_t L,
+                             uint32_t stride_A) {
+    return length_entry(A, workspace, N, L, stride_A, true);
+}
+
+uint32_t batch_limblength_max(const uint32_t *A, uint32_t *workspace, uint32_t N, uint32_t L,
+                              uint32_t stride_A) {
+    return length_entry(A, workspace, N, L, stride_A, false);
+}
+
+}  // extern "C"

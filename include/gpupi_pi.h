@@ -1,0 +1,67 @@
+/*
+ * gpupi-mi355x: the pi pipeline as a library call, C ABI.
+ *
+ * The reference exposes the pipeline only through its CLI
+ * (pi_bs_gpu.cu:2073-2431, `pi_bs_gpu <digits> [--benchmark|--profile-stages|
+ * --full-print]`).  These entry points expose the same computation to other
+ * hosts (the CLI in this library, tests, benchmarks):
+ *   gpupi_pi_run   = main_fractional's timed region (pi_bs_gpu.cu:2137-2193):
+ *                    pi_fractional + convert_decimal_precompute_powers +
+ *                    convert_decimal_equal_split;
+ *   the stage entry points mirror pi_bs_gpu.cu's binary_split_batched (503),
+ *   sqrt_10005 (1346), recursive_inverse (1179) and pi_fractional (1524).
+ */
+#ifndef GPUPI_PI_H
+#define GPUPI_PI_H
+
+#include <stdint.h>
+
+#include "batch_arith.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision convention.  GPUPI_PREC_REQUESTED sizes the precision from the
+ * requested digit count (the reference's current code, pi_bs_gpu.cu:2081);
+ * GPUPI_PREC_ROUNDED sizes it from the rounded-up conversion digit count,
+ * which reproduces the reference README's published RET length and hash. */
+enum { GPUPI_PREC_REQUESTED = 0, GPUPI_PREC_ROUNDED = 1 };
+
+typedef struct GpupiPiResult {
+    uint32_t requested_digits;
+    uint32_t convert_digits;   /* leaf_digits << levels */
+    uint32_t leaf_digits;
+    uint32_t prec_limbs;
+    uint32_t groups;           /* 17-term groups N */
+    uint32_t ret_length;       /* limbs of RET */
+    uint32_t *ret;             /* device, RET limbs (owned; gpupi_pi_free) */
+    char *digits;              /* device, convert_digits ASCII digits after "3." */
+    float elapsed_ms;          /* HIP-event time of the timed region */
+    uint64_t inverse_bits;
+} GpupiPiResult;
+
+/* Runs the pipeline for `requested_digits`.  Device outputs stay resident. */
+hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec_convention,
+                        GpupiPiResult *result);
+void gpupi_pi_free(GpupiPiResult *result);
+
+/* Stage entry points (batch-1 outputs are created arrays; release them). */
+hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t groups, BatchMPArray *Q, BatchMPArray *R);
+hipError_t gpupi_sqrt_10005(BatchMPContext *ctx, int target_digits, BatchMPArray *P, BatchMPArray *Q);
+hipError_t gpupi_recursive_inverse(BatchMPContext *ctx, BatchMPArray x, uint64_t L, BatchMPArray *y);
+hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPArray *ret,
+                               uint32_t *groups, uint64_t *inverse_bits);
+/* Leaf terms (pi_bs_gpu.cu:36-131): head rows 17t+16, body rows t + t/16. */
+hipError_t gpupi_construct_leaf_17(BatchMPContext *ctx, int is_head, uint32_t *P, uint32_t *Q,
+                                   uint32_t *R, uint32_t count);
+
+/* Sizing helpers (pi_bs_gpu.cu:924-944) */
+uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits);
+uint32_t gpupi_prec_limbs_for_digits(uint32_t digits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

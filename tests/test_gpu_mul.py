@@ -1,0 +1,122 @@
+"""GPU: batched products (schoolbook kernels and the two-prime binary64 NTT)
+bit-exact against GMP's mpn_mul through the oracle.
+
+Shapes: the schoolbook size classes, every NTT plan class (one tile; column +
+row; two columns + row), the 1024-limb boundary of the reference
+(batch_arith_test.cu:99-103), random shapes with random strides in the style
+of batch_mul_ntt_test.cu:531-556, and broadcast operands."""
+import numpy as np
+import pytest
+
+import gpupi_amd as g
+import oracle_ffi as of
+from helpers import rand_rows
+
+pytestmark = pytest.mark.gpu
+PAD = 0xDEADBEEF
+
+
+def oracle_mul(A, B, sa=None, sb=None):
+    n = A.shape[0]
+    hc = of.HostArray.empty(n, A.shape[1] + B.shape[1])
+    assert of.lib().o_mul(of.HostArray(A, sa).oarray(), of.HostArray(B, sb).oarray(), hc.oarray()) == 0
+    return hc.rows()
+
+
+def device_mul(ctx, A, B, sa=None, sb=None, sc_extra=1, b_broadcast=False):
+    n = A.shape[0]
+    lc = A.shape[1] + B.shape[1]
+    ba, Aa = g.upload_rows(A, sa)
+    bb, Ba = g.upload_rows(B if not b_broadcast else B[:1], sb)
+    if b_broadcast:
+        Ba = Ba.view(batch_size=n, stride=0)
+    sc = lc + sc_extra
+    bc, Ca = g.upload_rows(np.full((n, sc), PAD, dtype=np.uint32), sc)
+    Ca.length = lc
+    g.check(ctx.mul(Aa, Ba, Ca), "mul")
+    g.synchronize()
+    full = bc.download(np.uint32, count=n * sc).reshape(n, sc)
+    assert np.all(full[:, lc:] == PAD), "write past the product"
+    return full[:, :lc]
+
+
+SCHOOLBOOK = [(1000, 1, 1), (777, 3, 3), (500, 4, 6), (300, 16, 9), (64, 100, 16), (40, 17, 17),
+              (33, 64, 64), (9, 200, 150), (5, 512, 512), (3, 1000, 24), (2, 5000, 3000), (2, 8192, 8192)]
+NTT = [(1, 8, 8), (100, 9, 7), (64, 32, 32), (17, 100, 100), (9, 513, 512), (5, 1024, 1024), (3, 2048, 2047),
+       (2, 2049, 2047), (4, 3000, 1000), (3, 4097, 4095), (2, 10000, 9000), (2, 65536, 65536),
+       (1, 300000, 200000), (1, 1048576, 1048576), (3, 2097152, 1), (1, 2097153, 2097152)]
+
+
+@pytest.mark.parametrize("n,la,lb", SCHOOLBOOK, ids=[f"N{n}_{a}x{b}" for n, a, b in SCHOOLBOOK])
+def test_schoolbook(ctx, n, la, lb):
+    g.check(ctx.set_ntt_threshold(1 << 30))
+    try:
+        rng = np.random.default_rng(100)
+        for mode in ("random", "ones"):
+            A = rand_rows(rng, n, la, mode)
+            B = rand_rows(rng, n, lb, mode)
+            assert np.array_equal(device_mul(ctx, A, B, la + 3, lb + 1), oracle_mul(A, B, la + 3, lb + 1))
+    finally:
+        g.check(ctx.set_ntt_threshold(384))
+
+
+@pytest.mark.parametrize("n,la,lb", NTT, ids=[f"N{n}_{a}x{b}" for n, a, b in NTT])
+def test_ntt(ctx, n, la, lb):
+    g.check(ctx.set_ntt_threshold(2))
+    try:
+        rng = np.random.default_rng(1234)
+        for mode in ("random", "ones"):
+            A = rand_rows(rng, n, la, mode)
+            B = rand_rows(rng, n, lb, mode)
+            assert np.array_equal(device_mul(ctx, A, B), oracle_mul(A, B)), mode
+    finally:
+        g.check(ctx.set_ntt_threshold(384))
+
+
+def test_boundary_shapes(ctx):
+    # batch_arith_test.cu:99-103: 512+512 (naive there), 513+512, 2200+2200
+    rng = np.random.default_rng(7)
+    for la, lb in [(512, 512), (513, 512), (2200, 2200)]:
+        A = rand_rows(rng, 6, la)
+        B = rand_rows(rng, 6, lb)
+        assert np.array_equal(device_mul(ctx, A, B), oracle_mul(A, B))
+
+
+def test_random_shapes_and_strides(ctx):
+    # batch_mul_ntt_test.cu:531-556 style: log-uniform lengths, random split, strides L+1..8
+    rng = np.random.default_rng(100)
+    for _ in range(24):
+        total = int(2 ** rng.uniform(1, 15))
+        la = max(1, int(rng.integers(1, total + 1)))
+        lb = max(1, total - la) if total > la else 1
+        n = max(1, int(50000 // (la + lb)))
+        n = min(n, 200)
+        sa = la + int(rng.integers(0, 8))
+        sb = lb + int(rng.integers(0, 8))
+        A = rand_rows(rng, n, la)
+        B = rand_rows(rng, n, lb)
+        assert np.array_equal(device_mul(ctx, A, B, sa, sb, sc_extra=int(rng.integers(1, 5))),
+                              oracle_mul(A, B, sa, sb)), (n, la, lb)
+
+
+def test_broadcast_operand(ctx):
+    # convert_decimal multiplies every row by one power (stride 0)
+    rng = np.random.default_rng(3)
+    for la, lb in [(10, 7), (300, 300), (1500, 2500)]:
+        A = rand_rows(rng, 33, la)
+        B = np.repeat(rand_rows(rng, 1, lb), 33, axis=0)
+        assert np.array_equal(device_mul(ctx, A, B, b_broadcast=True), oracle_mul(A, B))
+
+
+@pytest.mark.slow
+def test_ntt_three_level(ctx):
+    # K = 2^23 and 2^25: two column levels + the contiguous level
+    g.check(ctx.set_ntt_threshold(2))
+    try:
+        rng = np.random.default_rng(5)
+        for la, lb in [(1 << 22, (1 << 22) - 5), (1 << 24, 1 << 24)]:
+            A = rand_rows(rng, 1, la)
+            B = rand_rows(rng, 1, lb)
+            assert np.array_equal(device_mul(ctx, A, B), oracle_mul(A, B))
+    finally:
+        g.check(ctx.set_ntt_threshold(384))

@@ -1,0 +1,109 @@
+"""GPU: the pi pipeline and each of its stages through the C ABI, bit-exact
+against the oracle's restatement of the reference; convert_decimal against
+the reference's own known-answer cases."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import gpupi_amd as g
+import oracle_ffi as of
+import refcases
+from helpers import rand_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def take_device(arr: g.BatchMPArray) -> np.ndarray:
+    out = g.download_words(arr.data, arr.length)
+    g.hip().hipFree(ctypes.c_void_p(arr.data))
+    return out
+
+
+@pytest.mark.parametrize("is_head,count", [(1, 1), (1, 4096), (0, 16), (0, 65536 * 16 + 5)])
+def test_leaf_terms(ctx, is_head, count):
+    bufs = [g.DeviceBuffer(count * w * 4) for w in (3, 4, 6)]
+    g.check(g.lib().gpupi_construct_leaf_17(ctx.ptr, is_head, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, count))
+    g.synchronize()
+    host = [np.zeros(count * w, dtype=np.uint32) for w in (3, 4, 6)]
+    of.lib().o_leaf17(is_head, host[0].ctypes.data, host[1].ctypes.data, host[2].ctypes.data, count)
+    for b, h in zip(bufs, host):
+        assert np.array_equal(b.download(np.uint32, count=h.size), h)
+
+
+@pytest.mark.parametrize("groups", [1, 2, 8, 64, 1024])
+def test_binary_split(ctx, groups):
+    Q, R = g.BatchMPArray(), g.BatchMPArray()
+    g.check(g.lib().gpupi_binary_split(ctx.ptr, groups, ctypes.byref(Q), ctypes.byref(R)))
+    oq, orr = of.OArray(), of.OArray()
+    assert of.lib().o_binary_split(groups, ctypes.byref(oq), ctypes.byref(orr)) == 0
+    assert np.array_equal(take_device(Q), of.take_array(oq))
+    assert np.array_equal(take_device(R), of.take_array(orr))
+
+
+@pytest.mark.parametrize("digits", [5, 100, 10000, 300000])
+def test_sqrt_10005(ctx, digits):
+    P, Q = g.BatchMPArray(), g.BatchMPArray()
+    g.check(g.lib().gpupi_sqrt_10005(ctx.ptr, digits, ctypes.byref(P), ctypes.byref(Q)))
+    op, oq = of.OArray(), of.OArray()
+    assert of.lib().o_sqrt_10005(digits, ctypes.byref(op), ctypes.byref(oq)) == 0
+    assert np.array_equal(take_device(P), of.take_array(op))
+    assert np.array_equal(take_device(Q), of.take_array(oq))
+
+
+@pytest.mark.parametrize("limbs", [10, 1000, 1100, 5000, 40000])
+def test_recursive_inverse(ctx, limbs):
+    rng = np.random.default_rng(limbs)
+    x = rand_rows(rng, 1, limbs)
+    x[0, -1] |= 0x80000000
+    L = 2 * (limbs * 32) - 1
+    bx, X = g.upload_rows(x)
+    Y = g.BatchMPArray()
+    g.check(g.lib().gpupi_recursive_inverse(ctx.ptr, X, L, ctypes.byref(Y)))
+    hx = of.HostArray(x)
+    oy = of.OArray()
+    assert of.lib().o_recursive_inverse(hx.oarray(), L, ctypes.byref(oy)) == 0
+    assert np.array_equal(take_device(Y), of.take_array(oy))
+
+
+@pytest.mark.parametrize("total,bits,words,expected", refcases.convert_cases(),
+                         ids=[str(c[0]) for c in refcases.convert_cases()])
+def test_convert_reference_cases(ctx, total, bits, words, expected):
+    leaf = 477
+    levels = (total // leaf).bit_length() - 1
+    table = g.ConvertDecimalPowerTable()
+    g.check(g.lib().convert_decimal_precompute_powers(ctx.ptr, ctypes.byref(table), leaf, levels))
+    try:
+        b, X = g.upload_rows(np.array([words], dtype=np.uint32))
+        out = g.DeviceBuffer(total)
+        g.check(g.lib().convert_decimal_equal_split(ctx.ptr, X, bits, total, leaf, ctypes.byref(table), out.ptr))
+        g.synchronize()
+        assert out.download(np.uint8, count=total).tobytes().decode() == expected
+    finally:
+        g.lib().convert_decimal_release_powers(ctypes.byref(table))
+
+
+@pytest.mark.parametrize("digits", [0, 1, 9, 100, 575, 576, 1000, 5000, 20000, 100000, 1000000])
+def test_pi_matches_oracle(ctx, digits):
+    ret, dec, res = ctx.pi(digits)
+    oret, odec, ocd = of.pi_digits(digits)
+    assert res.convert_digits == ocd
+    assert np.array_equal(ret, oret)
+    assert dec == odec
+
+
+@pytest.mark.parametrize("digits", [1000, 123457])
+def test_pi_readme_convention(ctx, digits):
+    ret, dec, res = ctx.pi(digits, g.PREC_ROUNDED)
+    oret, odec, _ = of.pi_digits(digits, readme_prec=True)
+    assert np.array_equal(ret, oret)
+    assert dec == odec
+
+
+@pytest.mark.slow
+def test_pi_1e7_matches_oracle(ctx):
+    ret, dec, res = ctx.pi(10_000_000)
+    oret, odec, _ = of.pi_digits(10_000_000)
+    assert len(ret) == 1038127
+    assert np.array_equal(ret, oret)
+    assert dec == odec

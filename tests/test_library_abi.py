@@ -1,0 +1,55 @@
+"""CPU: the C-ABI library loads, exports every entry point include/*.h
+declares, and does not link the oracle or GMP; the CLI rejects bad
+arguments before touching a device."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "gpupi_amd", "libgpupi.so")
+CLI = os.path.join(ROOT, "gpupi_amd", "pi_bs_gpu")
+CXX_ONLY = {"compact"}  # BatchMPArray::compact, an inline C++ member
+
+
+def declared_functions():
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"^\s*(?:[A-Za-z_][\w]*[\s\*]+)+\**([A-Za-z_]\w*)\s*\(([^;{}]*)\)\s*;", text, re.M):
+            if m.group(1) not in CXX_ONLY and not m.group(0).lstrip().startswith(("typedef", "return")):
+                names.add(m.group(1))
+    return names
+
+
+def test_headers_declare_the_reference_surface():
+    names = declared_functions()
+    for required in ["batch_mp_init", "batch_mp_mul", "batch_mp_shift_add", "batch_mp_exactdiv_small",
+                     "batch_mul_ntt", "batch_mul_naive", "batch_mul_small", "convert_decimal_equal_split",
+                     "convert_decimal_precompute_powers", "batch_decimal_naive", "batch_shift_bits",
+                     "batch_limblength_max", "gpupi_pi_run"]:
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_library_does_not_link_the_oracle():
+    out = subprocess.run(["readelf", "-d", LIB], capture_output=True, text=True, check=True).stdout
+    needed = re.findall(r"\(NEEDED\).*\[(.*)\]", out)
+    assert needed and not any("gmp" in n or "oracle" in n for n in needed), needed
+
+
+@pytest.mark.parametrize("args", [[], ["abc"], ["10", "--bogus"], ["-5"], ["10", "--benchmark", "--full-print"]])
+def test_cli_rejects_bad_arguments(args):
+    r = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Usage" in r.stderr
