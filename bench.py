@@ -1,0 +1,186 @@
+"""Benchmark: milliseconds to 10^9 decimal digits of pi on one MI355X.
+
+One step = the reference's timed region (pi_bs_gpu.cu:2137-2193): the
+binary-splitting series, sqrt(10005), the inverse and final division, the
+5^k power table and the radix conversion to decimal, with the result left in
+HBM.  The default workload is 1,000,000,000 digits in the precision
+convention of the reference README (RET of 103,847,955 limbs); the last
+step's RET hash and decimal head/tail are checked against the published ones.
+
+Printed: one JSON line with the metric, the live roofline of the dominant NTT
+kernel (HIP events around every launch of it inside the timed steps) and a
+CPU baseline (the GMP oracle on this host, timed on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+README_1E9 = {
+    "ret_len": 103847955,
+    "ret_hash": 354748112,
+    "ret_preview": "3243f6a88...d6e80bbbf2a7f738",
+    "dec_head": "3141592653589793238462643383279502884197169399375105820974944592",
+    "dec_tail": "1990548327874671398682093196353628204612755715171395115275045519",
+}
+
+
+def mod_hash(limbs) -> int:
+    mod = 1000000007
+    base = (1 << 32) % mod
+    v = 0
+    for x in reversed(limbs.tolist()):
+        v = (v * base + x) % mod
+    return v
+
+
+def hex_preview(limbs) -> str:
+    n = len(limbs)
+    top = format(int(limbs[-1]), "x") + format(int(limbs[-2]), "08x")
+    low = format(int(limbs[1]), "08x") + format(int(limbs[0]), "08x")
+    return f"{top}...{low}" if n > 4 else top
+
+
+def cpu_baseline(digits_small: int, digits_large: int, target: int) -> dict:
+    """Times the oracle (single-threaded GMP restatement of the pipeline) on
+    this host at two sizes and scales to the target with the measured growth
+    factor per decade."""
+    cli = os.path.join(ROOT, "oracle", "pi_oracle")
+    if not os.path.exists(cli):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    times = {}
+    for d in (digits_small, digits_large):
+        out = subprocess.run([cli, str(d)], capture_output=True, text=True, check=True).stdout
+        times[d] = float(re.search(r"elapsed_ms=([0-9.]+)", out).group(1))
+    import math
+    decades_sample = math.log10(digits_large / digits_small)
+    growth = (times[digits_large] / times[digits_small]) ** (1.0 / decades_sample)
+    decades = math.log10(target / digits_large)
+    value = times[digits_large] * growth ** decades
+    return {
+        "value": round(value, 1),
+        "unit": f"ms to {target:.0e} digits (extrapolated)",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle/pi_oracle (GMP restatement, 1 thread) measured {times[digits_small]:.1f} ms at "
+                   f"{digits_small:.0e} and {times[digits_large]:.1f} ms at {digits_large:.0e} digits; scaled to "
+                   f"{target:.0e} by the measured growth factor {growth:.2f}x per decade"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--digits", type=int, default=1_000_000_000)
+    ap.add_argument("--prec", choices=["rounded", "requested"], default="rounded")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the pi pipeline here is single-device).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import gpupi_amd as g
+
+    conv = g.PREC_ROUNDED if args.prec == "rounded" else g.PREC_REQUESTED
+    lib = g.lib()
+    ctx = g.Context()
+
+    def one_step(keep: bool = False):
+        res = g.GpupiPiResult()
+        g.check(lib.gpupi_pi_run(ctx.ptr, args.digits, conv, ctypes.byref(res)), "gpupi_pi_run")
+        if keep:
+            return res
+        lib.gpupi_pi_free(ctypes.byref(res))
+        return None
+
+    for _ in range(args.warmup):
+        one_step()
+    g.synchronize()
+    g.kernel_stats_reset(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    last = None
+    for k in range(args.steps):
+        if last is not None:
+            lib.gpupi_pi_free(ctypes.byref(last))
+        last = one_step(keep=True)
+    g.synchronize()
+    t1 = time.perf_counter()
+    stats = g.kernel_stats()
+    g.kernel_stats_reset(False)
+    ms_per_step = (t1 - t0) * 1000.0 / max(args.steps, 1)
+
+    # bit-exactness of the last step
+    ret = g.download_words(last.ret, last.ret_length)
+    head = "3" + g.download_bytes(last.digits, min(63, args.digits)).decode()
+    tail_n = min(64, args.digits)
+    tail = g.download_bytes(last.digits + args.digits - tail_n, tail_n).decode()
+    event_ms = last.elapsed_ms
+    lib.gpupi_pi_free(ctypes.byref(last))
+    check = {"ret_len": int(len(ret)), "ret_hash": mod_hash(ret), "ret_preview": hex_preview(ret),
+             "dec_head": head, "dec_tail": tail}
+    bit_exact = None
+    if args.digits == 1_000_000_000 and conv == g.PREC_ROUNDED:
+        bit_exact = all(check[k] == README_1E9[k] for k in README_1E9)
+
+    roofline = None
+    if stats:
+        name, launches, total_ms, total_bytes = max(stats, key=lambda s: s[2])
+        achieved = total_bytes / (total_ms / 1e3) / 1e9
+        roofline = {"bound": "hbm", "kernel": name, "launches": int(launches),
+                    "avg_launch_ms": round(total_ms / launches, 4),
+                    "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None}
+    ntt = {s[0]: {"launches": int(s[1]), "ms": round(s[2], 2), "GBps": round(s[3] / (s[2] / 1e3) / 1e9, 1)}
+           for s in stats}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(1_000_000, 10_000_000, args.digits)
+
+    ctx.close()
+    line = {
+        "metric": "ms to 1e9 pi digits (bit-exact)" if args.digits == 1_000_000_000 else f"ms to {args.digits} pi digits",
+        "value": round(ms_per_step, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        # the reference publishes 4623 ms at 10^9 digits (RTX 5090, README.md:8); < 1 is faster
+        "vs_baseline": round(ms_per_step / 4623.0, 4) if args.digits == 1_000_000_000 else None,
+        "dtype": "u32 limbs, f64 NTT residues",
+        "data": "none (pi has no input; the computation is deterministic)",
+        "config": {"workload": f"pi to {args.digits} decimal digits, single GPU, "
+                               f"{'README' if conv == g.PREC_ROUNDED else 'requested-digit'} precision convention",
+                   "digits": args.digits, "prec_convention": args.prec, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "bit_exact_vs_reference_readme": bit_exact,
+        "check": check,
+        "event_ms_last_step": round(event_ms, 3),
+        "ntt_kernels": ntt,
+    }
+    print(json.dumps(line))
+    return 0 if bit_exact in (None, True) else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

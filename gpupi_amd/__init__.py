@@ -63,6 +63,15 @@ class GpupiPiResult(ctypes.Structure):
     ]
 
 
+class GpupiKernelStat(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 48),
+        ("launches", ctypes.c_uint64),
+        ("total_ms", ctypes.c_double),
+        ("total_bytes", ctypes.c_double),
+    ]
+
+
 class ConvertDecimalPowerTable(ctypes.Structure):
     _fields_ = [
         ("powers", ctypes.POINTER(BatchMPArray)),
@@ -120,6 +129,8 @@ def _declare(lib):
         "batch_decimal_naive": (None, [ctypes.c_void_p, ctypes.c_void_p, u32, ctypes.c_int, u32,
                                        ctypes.c_int, u32]),
         "batch_mul_ntt_workspace_size": (ctypes.c_size_t, [u32, u32, u32]),
+        "gpupi_kernel_stats_reset": (None, [ctypes.c_int]),
+        "gpupi_kernel_stats_query": (ctypes.c_int, [ctypes.POINTER(GpupiKernelStat), ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -290,3 +301,17 @@ def download_rows(arr: BatchMPArray, buf: DeviceBuffer) -> np.ndarray:
     for i in range(arr.batch_size):
         out[i] = flat[i * stride: i * stride + arr.length]
     return out
+
+
+def kernel_stats_reset(enable: bool) -> None:
+    lib().gpupi_kernel_stats_reset(1 if enable else 0)
+
+
+def kernel_stats() -> list:
+    """[(name, launches, total_ms, total_bytes)] for the NTT kernels timed
+    since the last kernel_stats_reset(True)."""
+    arr = (GpupiKernelStat * 16)()
+    n = lib().gpupi_kernel_stats_query(arr, 16)
+    if n < 0:
+        raise RuntimeError("gpupi_kernel_stats_query failed")
+    return [(arr[i].name.decode(), arr[i].launches, arr[i].total_ms, arr[i].total_bytes) for i in range(n)]

@@ -22,9 +22,11 @@
 // coefficients, leaving one 64-bit spill word per run for the carry pass.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <vector>
 
 #include "common.hpp"
+#include "gpupi_pi.h"
 #include "ntt_plan.hpp"
 
 namespace gpupi {
@@ -470,6 +472,51 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 bool g_attr_done = false;
 
+// Optional per-launch timing of the NTT kernels (HIP events on the launch
+// stream) with each launch's algorithmic bytes, for live roofline numbers.
+enum KernelKind { kColFwdLimbs, kColFwd, kRowFused, kColInv, kColInvLimbs, kLocal, kKindCount };
+const char *const kKindNames[kKindCount] = {"ntt_col_fwd_kernel<limbs>", "ntt_col_fwd_kernel",
+                                            "ntt_row_fused_kernel", "ntt_col_inv_kernel",
+                                            "ntt_col_inv_kernel<limbs>", "ntt_local_kernel"};
+struct StatRecord {
+    int kind;
+    hipEvent_t start, stop;
+    double bytes;
+};
+struct Stats {
+    bool enabled = false;
+    std::vector<StatRecord> records;
+    std::vector<hipEvent_t> pool;
+} g_stats;
+
+hipEvent_t stats_event() {
+    if (!g_stats.pool.empty()) {
+        hipEvent_t e = g_stats.pool.back();
+        g_stats.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Records `launch` between two events when stats are enabled.
+template <typename F>
+hipError_t timed(int kind, double bytes, hipStream_t stream, F &&launch) {
+    if (!g_stats.enabled) {
+        launch();
+        return hipGetLastError();
+    }
+    StatRecord r{kind, stats_event(), stats_event(), bytes};
+    if (r.start == nullptr || r.stop == nullptr) return hipErrorOutOfMemory;
+    GPUPI_RETURN_IF(hipEventRecord(r.start, stream));
+    launch();
+    GPUPI_RETURN_IF(hipGetLastError());
+    GPUPI_RETURN_IF(hipEventRecord(r.stop, stream));
+    g_stats.records.push_back(r);
+    return hipSuccess;
+}
+
 }  // namespace
 
 bool ntt_supports(uint32_t L_a, uint32_t L_b) {
@@ -589,8 +636,10 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
         a.B = B;
         a.C = C;
         a.N = N;
-        hipLaunchKernelGGL(ntt_local_kernel, dim3((uint32_t)blocks), dim3(256), dyn, stream, a);
-        return hipGetLastError();
+        const double bytes = 4.0 * N * ((double)L_a + L_b + Lc);
+        return timed(kLocal, bytes, stream, [&] {
+            hipLaunchKernelGGL(ntt_local_kernel, dim3((uint32_t)blocks), dim3(256), dyn, stream, a);
+        });
     }
     if (workspace == nullptr || workspace_bytes < ntt_workspace_bytes(N, L_a, L_b)) return hipErrorInvalidValue;
     const size_t nc = chunk_products(N, k);
@@ -610,20 +659,27 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
         a.N = n;
         const uint64_t fwd_blocks = (uint64_t)n * 2 * tiles, blocks = (uint64_t)n * tiles;
         if (fwd_blocks > 0x7fffffffull) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(ntt_col_fwd_kernel<true>, dim3((uint32_t)fwd_blocks), dim3(256), 0, stream, a, 0);
-        GPUPI_RETURN_IF(hipGetLastError());
+        // algorithmic bytes: residue pairs are 16 B, limbs 4 B
+        const double tb = 16.0 * (double)n * (double)K;
+        GPUPI_RETURN_IF(timed(kColFwdLimbs, 4.0 * n * ((double)L_a + L_b) + 2 * tb, stream, [&] {
+            hipLaunchKernelGGL(ntt_col_fwd_kernel<true>, dim3((uint32_t)fwd_blocks), dim3(256), 0, stream, a, 0);
+        }));
         if (p.levels == 3) {
-            hipLaunchKernelGGL(ntt_col_fwd_kernel<false>, dim3((uint32_t)fwd_blocks), dim3(256), 0, stream, a, 1);
-            GPUPI_RETURN_IF(hipGetLastError());
+            GPUPI_RETURN_IF(timed(kColFwd, 4 * tb, stream, [&] {
+                hipLaunchKernelGGL(ntt_col_fwd_kernel<false>, dim3((uint32_t)fwd_blocks), dim3(256), 0, stream, a, 1);
+            }));
         }
-        hipLaunchKernelGGL(ntt_row_fused_kernel, dim3((uint32_t)blocks), dim3(256), dyn, stream, a);
-        GPUPI_RETURN_IF(hipGetLastError());
+        GPUPI_RETURN_IF(timed(kRowFused, 3 * tb, stream, [&] {
+            hipLaunchKernelGGL(ntt_row_fused_kernel, dim3((uint32_t)blocks), dim3(256), dyn, stream, a);
+        }));
         if (p.levels == 3) {
-            hipLaunchKernelGGL(ntt_col_inv_kernel<false>, dim3((uint32_t)blocks), dim3(256), 0, stream, a, 1);
-            GPUPI_RETURN_IF(hipGetLastError());
+            GPUPI_RETURN_IF(timed(kColInv, 2 * tb, stream, [&] {
+                hipLaunchKernelGGL(ntt_col_inv_kernel<false>, dim3((uint32_t)blocks), dim3(256), 0, stream, a, 1);
+            }));
         }
-        hipLaunchKernelGGL(ntt_col_inv_kernel<true>, dim3((uint32_t)blocks), dim3(256), 0, stream, a, 0);
-        GPUPI_RETURN_IF(hipGetLastError());
+        GPUPI_RETURN_IF(timed(kColInvLimbs, tb + 4.0 * n * Lc + 8.0 * n * runs, stream, [&] {
+            hipLaunchKernelGGL(ntt_col_inv_kernel<true>, dim3((uint32_t)blocks), dim3(256), 0, stream, a, 0);
+        }));
         GPUPI_RETURN_IF(launch_add_spills(a.C, a.spill, n, Lc, stride_C, 1u << final_run_log2(p), (uint32_t)runs,
                                           flags, stream));
     }
@@ -658,6 +714,36 @@ void batch_mul_ntt(uint32_t *A, uint32_t *B, uint32_t *ret, uint32_t *workspace,
 
 size_t batch_mul_ntt_workspace_size(uint32_t N, uint32_t L_a, uint32_t L_b) {
     return gpupi::ntt_workspace_bytes(N, L_a, L_b);
+}
+
+void gpupi_kernel_stats_reset(int enable) {
+    using namespace gpupi;
+    for (auto &r : g_stats.records) {
+        g_stats.pool.push_back(r.start);
+        g_stats.pool.push_back(r.stop);
+    }
+    g_stats.records.clear();
+    g_stats.enabled = enable != 0;
+}
+
+int gpupi_kernel_stats_query(GpupiKernelStat *out, int max_entries) {
+    using namespace gpupi;
+    GpupiKernelStat agg[kKindCount] = {};
+    for (int k = 0; k < kKindCount; ++k) {
+        std::snprintf(agg[k].name, sizeof(agg[k].name), "%s", kKindNames[k]);
+    }
+    for (auto &r : g_stats.records) {
+        if (hipEventSynchronize(r.stop) != hipSuccess) return -1;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.start, r.stop) != hipSuccess) return -1;
+        agg[r.kind].launches += 1;
+        agg[r.kind].total_ms += ms;
+        agg[r.kind].total_bytes += r.bytes;
+    }
+    int n = 0;
+    for (int k = 0; k < kKindCount && n < max_entries; ++k)
+        if (agg[k].launches > 0) out[n++] = agg[k];
+    return n;
 }
 
 }  // extern "C"

@@ -1,4 +1,6 @@
 // Bit shift, exact division by an odd word, and batch length reductions.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace gpupi {
@@ -86,42 +88,89 @@ __device__ __forceinline__ uint32_t length_of(uint32_t top_index_plus_one, uint3
     return (top_index_plus_one - 1) * 32u + (32u - (uint32_t)__clz(top_word));
 }
 
+// Batch length reductions: a grid-stride loop over rows with the maximum
+// reduced per block (one atomic per block).  A thread or wave stops as soon
+// as it has seen the largest possible value.
+__device__ __forceinline__ void block_max_atomic(uint32_t v, uint32_t *result) {
+    __shared__ uint32_t wmax[4];
+    v = wave_max(v);
+    const uint32_t wave = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) wmax[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = wmax[0];
+        for (uint32_t w = 1; w < blockDim.x / kWave; ++w) m = max(m, wmax[w]);
+        if (m != 0) atomicMax(result, m);
+    }
+}
+
 // Rows of at most 64 limbs: one thread per row scanning down from the top.
 __global__ void __launch_bounds__(256) length_thread_kernel(const uint32_t *A, uint32_t N, uint32_t L,
                                                             uint32_t stride_A, int bits, uint32_t *result) {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t len = 0;
-    if (row < N) {
+    const uint32_t top = bits ? 32u * L : L;
+    uint32_t best = 0;
+    for (uint32_t row = blockIdx.x * blockDim.x + threadIdx.x; row < N && best < top;
+         row += blockDim.x * gridDim.x) {
         const uint32_t *a = A + (size_t)row * stride_A;
         uint32_t j = L;
         while (j > 0 && a[j - 1] == 0u) --j;
-        len = length_of(j, j ? a[j - 1] : 0u, bits != 0);
+        best = max(best, length_of(j, j ? a[j - 1] : 0u, bits != 0));
     }
-    len = wave_max(len);
-    if ((threadIdx.x & (kWave - 1)) == 0 && len != 0) atomicMax(result, len);
+    block_max_atomic(best, result);
 }
 
 // Longer rows: one wave per row, 64 limbs per step from the top down.
 __global__ void __launch_bounds__(256) length_wave_kernel(const uint32_t *A, uint32_t N, uint32_t L,
                                                           uint32_t stride_A, int bits, uint32_t *result) {
-    const uint32_t row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (row >= N) return;
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t *a = A + (size_t)row * stride_A;
-    for (int64_t top = L; top > 0; top -= kWave) {
-        const int64_t j = top - kWave + (int64_t)lane;
-        const uint32_t v = (j >= 0) ? a[j] : 0u;
-        const uint64_t nz = ballot64(v != 0u);
-        if (nz != 0) {
-            const int hi_lane = 63 - __builtin_clzll(nz);
-            const uint32_t w = __shfl(v, hi_lane);
-            if (lane == 0) {
-                const uint32_t idx1 = (uint32_t)(top - kWave + hi_lane + 1);
-                atomicMax(result, length_of(idx1, w, bits != 0));
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    const uint32_t topv = bits ? 32u * L : L;
+    uint32_t best = 0;
+    for (uint32_t row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; row < N && best < topv;
+         row += waves) {
+        const uint32_t *a = A + (size_t)row * stride_A;
+        for (int64_t top = L; top > 0; top -= kWave) {
+            const int64_t j = top - kWave + (int64_t)lane;
+            const uint32_t v = (j >= 0) ? a[j] : 0u;
+            const uint64_t nz = ballot64(v != 0u);
+            if (nz != 0) {
+                const int hi_lane = 63 - __builtin_clzll(nz);
+                const uint32_t w = __shfl(v, hi_lane);
+                best = max(best, length_of((uint32_t)(top - kWave + hi_lane + 1), w, bits != 0));
+                break;
             }
-            return;
         }
     }
+    block_max_atomic(best, result);
+}
+
+// Few long rows: blocks scan 4096-limb chunks counted from the top of each
+// row.  A block whose chunk lies wholly below the best length found so far
+// leaves at once, so the work is about one chunk per row plus the zero tail
+// actually present above the top limb.
+constexpr uint32_t kLenChunk = 4096;
+
+__global__ void __launch_bounds__(256) length_chunk_kernel(const uint32_t *A, uint32_t L, uint32_t stride_A,
+                                                           uint32_t chunks_per_row, int bits, uint32_t *result) {
+    __shared__ uint32_t skip;
+    const uint32_t row = blockIdx.x / chunks_per_row;
+    const uint32_t c = blockIdx.x - row * chunks_per_row;
+    const int64_t hi = (int64_t)L - (int64_t)c * kLenChunk;  // chunk covers [hi - 4096, hi)
+    const int64_t lo = hi - (int64_t)kLenChunk > 0 ? hi - (int64_t)kLenChunk : 0;
+    if (threadIdx.x == 0) {
+        const uint32_t cur = atomicAdd(result, 0u);
+        const uint64_t best_possible = bits ? 32ull * (uint64_t)hi : (uint64_t)hi;
+        skip = (uint64_t)cur >= best_possible ? 1u : 0u;
+    }
+    __syncthreads();
+    if (skip) return;
+    const uint32_t *a = A + (size_t)row * stride_A;
+    uint32_t best = 0;
+    for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+        const uint32_t v = a[j];
+        if (v != 0u) best = max(best, length_of((uint32_t)j + 1u, v, bits != 0));
+    }
+    block_max_atomic(best, result);
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, uint64_t count, uint32_t value) {
@@ -166,12 +215,16 @@ hipError_t launch_exactdiv_small(uint32_t *A, uint32_t B, uint32_t N, uint32_t L
 hipError_t launch_length_max(const uint32_t *A, uint32_t N, uint32_t L, uint32_t stride_A, bool bits,
                              uint32_t *d_result, hipStream_t stream) {
     if (N == 0 || L == 0) return hipSuccess;
-    if (L <= 64) {
-        hipLaunchKernelGGL(length_thread_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, stream, A, N, L,
-                           stride_A, bits ? 1 : 0, d_result);
+    if (L > 4 * kLenChunk && N <= 64) {
+        const uint32_t cpr = ceil_div(L, kLenChunk);
+        hipLaunchKernelGGL(length_chunk_kernel, dim3(N * cpr), dim3(256), 0, stream, A, L, stride_A, cpr,
+                           bits ? 1 : 0, d_result);
+    } else if (L <= 64) {
+        hipLaunchKernelGGL(length_thread_kernel, dim3(std::min<uint32_t>(ceil_div(N, 256), 2048)), dim3(256), 0,
+                           stream, A, N, L, stride_A, bits ? 1 : 0, d_result);
     } else {
-        hipLaunchKernelGGL(length_wave_kernel, dim3(ceil_div(N, 4)), dim3(256), 0, stream, A, N, L,
-                           stride_A, bits ? 1 : 0, d_result);
+        hipLaunchKernelGGL(length_wave_kernel, dim3(std::min<uint32_t>(ceil_div(N, 4), 4096)), dim3(256), 0,
+                           stream, A, N, L, stride_A, bits ? 1 : 0, d_result);
     }
     return hipGetLastError();
 }

@@ -56,6 +56,22 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
 hipError_t gpupi_construct_leaf_17(BatchMPContext *ctx, int is_head, uint32_t *P, uint32_t *Q,
                                    uint32_t *R, uint32_t count);
 
+/* Instrumentation (no reference counterpart).  While enabled, every NTT
+ * kernel launch is bracketed by HIP events on its stream; query aggregates
+ * per kernel: launches, summed event time and summed algorithmic bytes.  A
+ * launch's algorithmic bytes count each residue pair it must read or write
+ * once (16 B) and each input or output limb once (4 B): the minimum traffic
+ * of that pass.  query synchronizes on the recorded events and returns the
+ * number of entries written. */
+typedef struct GpupiKernelStat {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+    double total_bytes;
+} GpupiKernelStat;
+void gpupi_kernel_stats_reset(int enable);
+int gpupi_kernel_stats_query(GpupiKernelStat *out, int max_entries);
+
 /* Sizing helpers (pi_bs_gpu.cu:924-944) */
 uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits);
 uint32_t gpupi_prec_limbs_for_digits(uint32_t digits);
