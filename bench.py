@@ -78,6 +78,42 @@ def cpu_baseline(digits_small: int, digits_large: int, target: int) -> dict:
     }
 
 
+def bench_mul(args, g) -> int:
+    """Times batch_mp_mul on N random pairs of L_a x L_b limbs (inputs
+    resident in HBM) and reports per-kernel NTT timings."""
+    import numpy as np
+    rng = np.random.default_rng(1234)
+    ctx = g.Context()
+    A = rng.integers(0, 1 << 32, size=(args.n, args.la), dtype=np.uint64).astype(np.uint32)
+    B = rng.integers(0, 1 << 32, size=(args.n, args.lb), dtype=np.uint64).astype(np.uint32)
+    ba, Aa = g.upload_rows(A)
+    bb, Ba = g.upload_rows(B)
+    del A, B
+    bc = g.DeviceBuffer(4 * args.n * (args.la + args.lb))
+    Ca = g.BatchMPArray(bc.ptr, args.la + args.lb, args.n, args.la + args.lb)
+    for _ in range(args.warmup):
+        g.check(ctx.mul(Aa, Ba, Ca), "mul")
+    g.synchronize()
+    g.kernel_stats_reset(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.check(ctx.mul(Aa, Ba, Ca), "mul")
+    g.synchronize()
+    t1 = time.perf_counter()
+    stats = g.kernel_stats()
+    g.kernel_stats_reset(False)
+    ms = (t1 - t0) * 1000.0 / args.steps
+    # the reference's compulsory-traffic figure (batch_mul_ntt_test.cu:491-492)
+    abr = 4.0 * args.n * (2 * (args.la + args.lb)) / (ms / 1e3) / 1e9
+    ntt = {s[0]: {"launches": int(s[1]), "avg_ms": round(s[2] / s[1], 4),
+                  "GBps": round(s[3] / (s[2] / 1e3) / 1e9, 1)} for s in stats}
+    ctx.close()
+    print(json.dumps({"metric": f"ms per batched product N={args.n} {args.la}x{args.lb} limbs",
+                      "value": round(ms, 4), "unit": "ms", "steps": args.steps, "warmup": args.warmup,
+                      "a_b_ret_GBps": round(abr, 1), "ntt_kernels": ntt}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +122,11 @@ def main() -> int:
     ap.add_argument("--digits", type=int, default=1_000_000_000)
     ap.add_argument("--prec", choices=["rounded", "requested"], default="rounded")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["pi", "mul"], default="pi",
+                    help="pi: the headline run; mul: one batched product shape (kernel study)")
+    ap.add_argument("--la", type=int, default=1 << 27)
+    ap.add_argument("--lb", type=int, default=1 << 27)
+    ap.add_argument("--n", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -95,6 +136,9 @@ def main() -> int:
     import numpy as np
 
     import gpupi_amd as g
+
+    if args.workload == "mul":
+        return bench_mul(args, g)
 
     conv = g.PREC_ROUNDED if args.prec == "rounded" else g.PREC_REQUESTED
     lib = g.lib()

@@ -10,19 +10,28 @@
 // p0 p1 ~ 2^98 (a coefficient of a product of 32-bit limbs is < 2^91 for
 // lengths up to 2^27), so each limb is one coefficient and transforms reach
 // 2^30 points.  Values are kept centred (|v| <= 0.75 p) so no compare/select
-// is needed inside butterflies.
+// is needed inside butterflies; sums are re-centred every other stage.
 //
-// Each kernel works on a 4096-element tile in LDS (radix-16 steps, bit-
-// reversed slot order inside the tile); long transforms are split into
-// column levels (<= 512 points, >= 8 adjacent columns per tile so HBM
-// accesses stay in >= 128-byte runs) and a final contiguous level.  The last
-// forward level, the pointwise product and the first inverse level are fused
-// in one kernel; the final inverse level applies 1/K, recombines the primes
-// (Garner) and emits limbs with carries resolved inside runs of adjacent
+// Residues are stored planar (one array of binary64 per prime) and every
+// kernel runs the two primes one after the other through a 4096-element LDS
+// tile (34.8 KB).  Inside a tile the transform runs radix-16 steps
+// (bit-reversed slot order); the DFT size of a level is a template parameter
+// so every LDS address is a per-thread base plus a compile-time offset.
+// Long transforms are split into column levels (<= 512 points, 8..128
+// adjacent columns per tile) and a final contiguous level.  The last forward
+// level, the pointwise product and the first inverse level are one kernel;
+// the final inverse level applies 1/K, recombines the primes (Garner) and
+// emits limbs with carries resolved inside runs of <= 16 adjacent
 // coefficients, leaving one 64-bit spill word per run for the carry pass.
+//
+// Twiddles: in-tile steps read a per-step table laid out [j][n2] so that a
+// wave's loads are contiguous; the inter-level twiddle of a column element
+// is w^(c * bitrev(r)), generated per thread as base * step^v from two
+// lookups in a two-level table of w_{2^30} powers.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -37,156 +46,240 @@ using namespace ntt;
 constexpr uint64_t kPrime0 = 0x1ffffc0000001ull;  // 562948879679489 = 524287 * 2^30 + 1
 constexpr uint64_t kPrime1 = 0x1fffe00000001ull;  // 562941363486721 = 524280 * 2^30 + 1 (2^33 | p - 1)
 constexpr uint64_t kGen0 = 3, kGen1 = 13;          // primitive roots
-constexpr double kP0 = 562948879679489.0;
-constexpr double kP1 = 562941363486721.0;
-constexpr double kPinv0 = 1.0 / kP0;
-constexpr double kPinv1 = 1.0 / kP1;
 constexpr double kRoundMagic = 6755399441055744.0;  // 1.5 * 2^52
+constexpr double kP0 = 562948879679489.0, kP1 = 562941363486721.0;
 
-// Table layout (double2 = (prime0, prime1) pairs) inside tables.roots.
-constexpr size_t kOffTw = 0;                // w_4096^e, e < 4096
-constexpr size_t kOffTwInv = 4096;          // w_4096^-e
-constexpr size_t kOffLo = 8192;             // w_2^30^e, e < 2^15
-constexpr size_t kOffHi = kOffLo + 32768;   // w_2^30^(e 2^15)
+// Table layout (binary64) inside tables.roots, one block per prime.
+//   step tables: for Ms = 2^x (x = 5..12), at (2^x - 32): [j * Ms/16 + n2] =
+//                w_Ms^(n2 * bitrev(j, 4)), j < 16, n2 < Ms/16
+//   lo/hi: w_{2^30}^e and w_{2^30}^(e 2^15), e < 2^15 (and the inverses)
+constexpr size_t kOffStep = 0;
+constexpr size_t kOffStepInv = 8192;
+constexpr size_t kOffLo = 16384;
+constexpr size_t kOffHi = kOffLo + 32768;
 constexpr size_t kOffLoInv = kOffHi + 32768;
 constexpr size_t kOffHiInv = kOffLoInv + 32768;
-constexpr size_t kTableLen = kOffHiInv + 32768;
+constexpr size_t kPrimeTable = kOffHiInv + 32768;
+constexpr size_t kTableLen = 2 * kPrimeTable;
 
 struct NttConsts {
-    double2 w16[8];
-    double2 w16i[8];
-    double2 kinv[kMaxLog2 + 1];
+    double w16[2][8];
+    double w16i[2][8];
+    double kinv[2][kMaxLog2 + 1];
     double garner;  // p0^-1 mod p1, centred
 };
 __constant__ NttConsts c_ntt;
 
-__device__ __forceinline__ double qround(double x, double pinv) {
-    return __builtin_fma(x, pinv, kRoundMagic) - kRoundMagic;
+// ------------------------------------------------------- modular arithmetic
+struct Mod {
+    double p, pinv;
+};
+__device__ __forceinline__ Mod mod_of(int P) { return P ? Mod{kP1, 1.0 / kP1} : Mod{kP0, 1.0 / kP0}; }
+
+__device__ __forceinline__ double qround(double x, Mod m) {
+    return __builtin_fma(x, m.pinv, kRoundMagic) - kRoundMagic;
 }
-__device__ __forceinline__ double redp(double x, double p, double pinv) {
-    return __builtin_fma(-qround(x, pinv), p, x);
-}
-__device__ __forceinline__ double mulp(double a, double b, double p, double pinv) {
+__device__ __forceinline__ double redp(double x, Mod m) { return __builtin_fma(-qround(x, m), m.p, x); }
+__device__ __forceinline__ double mulp(double a, double b, Mod m) {
     const double h = a * b;
     const double l = __builtin_fma(a, b, -h);
-    return __builtin_fma(-qround(h, pinv), p, h) + l;
-}
-__device__ __forceinline__ double2 add2(double2 u, double2 v) {
-    return make_double2(redp(u.x + v.x, kP0, kPinv0), redp(u.y + v.y, kP1, kPinv1));
-}
-__device__ __forceinline__ double2 sub2(double2 u, double2 v) {
-    return make_double2(redp(u.x - v.x, kP0, kPinv0), redp(u.y - v.y, kP1, kPinv1));
-}
-__device__ __forceinline__ double2 mul2(double2 u, double2 w) {
-    return make_double2(mulp(u.x, w.x, kP0, kPinv0), mulp(u.y, w.y, kP1, kPinv1));
-}
-__device__ __forceinline__ double2 submul2(double2 u, double2 v, double2 w) {
-    return make_double2(mulp(u.x - v.x, w.x, kP0, kPinv0), mulp(u.y - v.y, w.y, kP1, kPinv1));
+    return __builtin_fma(-qround(h, m), m.p, h) + l;
 }
 
-__device__ __forceinline__ uint32_t brev(uint32_t x, int bits) {
-    return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
-}
+constexpr int log2c(int x) { return x <= 1 ? 0 : 1 + log2c(x / 2); }
+constexpr uint32_t brev4(uint32_t x) { return ((x & 1u) << 3) | ((x & 2u) << 1) | ((x & 4u) >> 1) | ((x & 8u) >> 3); }
 
 // In-register radix-R DIF (natural in, bit-reversed out) and its inverse DIT.
+// Sums are re-centred only every other radix-2 stage (always in the last):
+// with inputs |v| <= 0.75p no intermediate exceeds 3p, far inside the
+// exactness limits of the modular product (|a| < 2^52) and of the sums.
 template <int R>
-__device__ __forceinline__ void dif(double2 *v) {
+__device__ __forceinline__ void dif(double *v, const double *w16, Mod m) {
+    constexpr int S = log2c(R);
 #pragma unroll
-    for (int h = R / 2; h >= 1; h >>= 1) {
+    for (int st = 0; st < S; ++st) {
+        const int h = R >> (st + 1);
+        const bool reduce = ((S - 1 - st) & 1) == 0;  // log2(h) even
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             if (j & h) continue;
             const int e = (j & (h - 1)) * (8 / h);
-            const double2 a = v[j], b = v[j + h];
-            v[j] = add2(a, b);
-            v[j + h] = (e == 0) ? sub2(a, b) : submul2(a, b, c_ntt.w16[e]);
+            const double a = v[j], b = v[j + h];
+            v[j] = reduce ? redp(a + b, m) : a + b;
+            v[j + h] = (e != 0) ? mulp(a - b, w16[e], m) : (reduce ? redp(a - b, m) : a - b);
         }
     }
 }
 
 template <int R>
-__device__ __forceinline__ void dit(double2 *v) {
+__device__ __forceinline__ void dit(double *v, const double *w16i, Mod m) {
+    constexpr int S = log2c(R);
 #pragma unroll
-    for (int h = 1; h < R; h <<= 1) {
+    for (int st = 0; st < S; ++st) {
+        const int h = 1 << st;
+        const bool reduce = ((S - 1 - st) & 1) == 0;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             if (j & h) continue;
             const int e = (j & (h - 1)) * (8 / h);
-            const double2 t = (e == 0) ? v[j + h] : mul2(v[j + h], c_ntt.w16i[e]);
-            const double2 a = v[j];
-            v[j] = add2(a, t);
-            v[j + h] = sub2(a, t);
+            const double t = (e == 0) ? v[j + h] : mulp(v[j + h], w16i[e], m);
+            const double a = v[j];
+            v[j] = reduce ? redp(a + t, m) : a + t;
+            v[j + h] = reduce ? redp(a - t, m) : a - t;
         }
     }
 }
 
-// One radix-R step over the tile (4096 / R small DFTs, 16 / R per thread).
-template <int R, bool INV>
-__device__ __forceinline__ void tile_step(double2 *lds, int m, int s, const double2 *tw) {
-    constexpr int PER = 16 / R;
-    constexpr int LR = (R == 16) ? 4 : (R == 8) ? 3 : (R == 4) ? 2 : 1;
-    double2 v[16];
-    SmallDft d[PER];
+// In-tile steps.  Step S of the 2^M-point DFTs of a tile is 4096 / R small
+// DFTs of radix R; thread q = threadIdx.x handles 16 / R of them (q, q + 256,
+// ...), 16 values in registers v[u * R + j].  The first and last steps of a
+// transform run on values already in registers (the step-0 elements of a
+// thread are the ones it reads from HBM), the others go through LDS.
+template <int M, int S, bool COL>
+struct StepMap {
+    static constexpr int LR = step_log2radix(M, S);
+    static constexpr int R = 1 << LR;
+    static constexpr int PER = 16 / R;
+    static constexpr int LMS = step_ms_log2(M, S);
+    static constexpr int LSS = LMS - LR;
+    static constexpr uint32_t kStride = COL ? 1u << (LSS + kTileLog2 - M) : 1u << LSS;
+    uint32_t base[PER];  // tile index of element j = 0 of each small DFT
+    uint32_t n2[PER];
+    __device__ __forceinline__ StepMap() {
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        d[u] = small_dft(m, s, threadIdx.x + 256u * u);
-#pragma unroll
-        for (int j = 0; j < R; ++j) v[u * R + j] = lds[lds_index(d[u].base + j * d[u].stride)];
-    }
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int shift = kTileLog2 - d[u].log2Ms;
-        if (!INV) {
-            dif<R>(&v[u * R]);
-#pragma unroll
-            for (int j = 1; j < R; ++j) {
-                const uint32_t e = d[u].n2 * brev((uint32_t)j, LR);
-                v[u * R + j] = mul2(v[u * R + j], tw[e << shift]);
-            }
-        } else {
-#pragma unroll
-            for (int j = 1; j < R; ++j) {
-                const uint32_t e = d[u].n2 * brev((uint32_t)j, LR);
-                v[u * R + j] = mul2(v[u * R + j], tw[e << shift]);
-            }
-            dit<R>(&v[u * R]);
+        for (int u = 0; u < PER; ++u) {
+            const SmallDft d = small_dft(M, S, threadIdx.x + 256u * u, COL);
+            base[u] = d.base;
+            n2[u] = d.n2;
         }
     }
+    // tile index of value v[k]
+    __device__ __forceinline__ uint32_t index(int k) const { return base[k / R] + (k % R) * kStride; }
+};
+
+// Butterflies (and the step's twiddles) of step S on registers.
+template <int M, int S, bool INV, bool COL>
+__device__ __forceinline__ void step_regs(double *v, const StepMap<M, S, COL> &sm, const double *tab, int P, Mod m) {
+    using SM = StepMap<M, S, COL>;
+    constexpr int R = SM::R;
+    const double *w16 = INV ? c_ntt.w16i[P] : c_ntt.w16[P];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
+    for (int u = 0; u < SM::PER; ++u) {
+        if (!INV) dif<R>(&v[u * R], w16, m);
+        if constexpr (SM::LSS > 0) {
+            static_assert(R == 16, "twiddled steps are radix 16");
+            const double *tw = tab + (INV ? kOffStepInv : kOffStep) + ((1u << SM::LMS) - 32u) + sm.n2[u];
 #pragma unroll
-        for (int j = 0; j < R; ++j) lds[lds_index(d[u].base + j * d[u].stride)] = v[u * R + j];
+            for (int j = 1; j < 16; ++j) v[u * 16 + j] = mulp(v[u * 16 + j], tw[j << SM::LSS], m);
+        }
+        if (INV) dit<R>(&v[u * R], w16, m);
     }
 }
 
-// All steps of G = 4096 / 2^m DFTs of 2^m points held in the tile.
-template <bool INV>
-__device__ void tile_dft(double2 *lds, int m, const double2 *tw) {
-    const int steps = step_count(m);
-    for (int i = 0; i < steps; ++i) {
-        const int s = INV ? steps - 1 - i : i;
-        switch (step_log2radix(m, s)) {
-            case 4: tile_step<16, INV>(lds, m, s, tw); break;
-            case 3: tile_step<8, INV>(lds, m, s, tw); break;
-            case 2: tile_step<4, INV>(lds, m, s, tw); break;
-            default: tile_step<2, INV>(lds, m, s, tw); break;
-        }
+template <int M, int S, bool COL>
+__device__ __forceinline__ void lds_load(double *v, const double *lds, const StepMap<M, S, COL> &sm) {
+    using SM = StepMap<M, S, COL>;
+#pragma unroll
+    for (int u = 0; u < SM::PER; ++u) {
+        const uint32_t b = lds_index(sm.base[u]);
+#pragma unroll
+        for (int j = 0; j < SM::R; ++j) v[u * SM::R + j] = lds[b + lds_index(j * SM::kStride)];
+    }
+}
+
+template <int M, int S, bool COL>
+__device__ __forceinline__ void lds_store(double *lds, const double *v, const StepMap<M, S, COL> &sm) {
+    using SM = StepMap<M, S, COL>;
+#pragma unroll
+    for (int u = 0; u < SM::PER; ++u) {
+        const uint32_t b = lds_index(sm.base[u]);
+#pragma unroll
+        for (int j = 0; j < SM::R; ++j) lds[b + lds_index(j * SM::kStride)] = v[u * SM::R + j];
+    }
+}
+
+// Steps S..LAST (forward, increasing) or S..LAST (inverse, decreasing) as
+// LDS -> registers -> LDS passes, each followed by a barrier.
+template <int M, bool INV, bool COL, int S, int LAST>
+__device__ __forceinline__ void lds_steps(double *lds, const double *tab, int P, Mod m) {
+    if constexpr (INV ? S >= LAST : S <= LAST) {
+        const StepMap<M, S, COL> sm;
+        double v[16];
+        lds_load(v, lds, sm);
+        step_regs<M, S, INV, COL>(v, sm, tab, P, m);
+        lds_store(lds, v, sm);
         __syncthreads();
+        lds_steps<M, INV, COL, INV ? S - 1 : S + 1, LAST>(lds, tab, P, m);
     }
 }
 
-// w_{2^30}^E (or its inverse) from the two-level table.
-__device__ __forceinline__ double2 big_twiddle(const double2 *tables, uint64_t E, bool inv) {
-    const double2 lo = tables[(inv ? kOffLoInv : kOffLo) + (E & 32767u)];
-    const double2 hi = tables[(inv ? kOffHiInv : kOffHi) + (E >> 15)];
-    return mul2(lo, hi);
+// Forward transform of the tile.  On entry v holds this thread's step-0
+// elements; on exit its last-step elements (bit-reversed DFT order).  LDS is
+// used when M > 4: the caller must make sure no thread still reads `lds`.
+template <int M, bool COL>
+__device__ __forceinline__ void fwd_dft(double *v, double *lds, const double *tab, int P, Mod m) {
+    constexpr int kLast = step_count(M) - 1;
+    step_regs<M, 0, false, COL>(v, StepMap<M, 0, COL>(), tab, P, m);
+    if constexpr (kLast > 0) {
+        lds_store(lds, v, StepMap<M, 0, COL>());
+        __syncthreads();
+        lds_steps<M, false, COL, 1, kLast - 1>(lds, tab, P, m);
+        const StepMap<M, kLast, COL> sl;
+        lds_load(v, lds, sl);
+        step_regs<M, kLast, false, COL>(v, sl, tab, P, m);
+    }
+}
+
+// Inverse transform of the tile.  On entry v holds this thread's last-step
+// elements (as left by fwd_dft, so the first LDS store touches only slots
+// this thread read); on exit its step-0 elements.
+template <int M, bool COL>
+__device__ __forceinline__ void inv_dft(double *v, double *lds, const double *tab, int P, Mod m) {
+    constexpr int kLast = step_count(M) - 1;
+    const StepMap<M, kLast, COL> sl;
+    step_regs<M, kLast, true, COL>(v, sl, tab, P, m);
+    if constexpr (kLast > 0) {
+        lds_store(lds, v, sl);
+        __syncthreads();
+        lds_steps<M, true, COL, kLast - 1, 1>(lds, tab, P, m);
+        const StepMap<M, 0, COL> s0;
+        lds_load(v, lds, s0);
+        step_regs<M, 0, true, COL>(v, s0, tab, P, m);
+    }
+}
+
+// w_{2^30}^E (or its inverse) from the two-level table; E < 2^30.
+__device__ __forceinline__ double big_twiddle(const double *tab, uint32_t E, bool inv, Mod m) {
+    const double lo = tab[(inv ? kOffLoInv : kOffLo) + (E & 32767u)];
+    const double hi = tab[(inv ? kOffHiInv : kOffHi) + (E >> 15)];
+    return mulp(lo, hi, m);
+}
+
+// Multiplies x[u] (u < 16) by w^(cg * (b0 + bitrev(u, 4))): base = w^(cg b0),
+// step = w^cg, and two interleaved chains base * step^v.
+__device__ __forceinline__ void column_twiddles(double *x, const double *tab, uint32_t cg, uint32_t b0, bool inv,
+                                                Mod m) {
+    const double base = big_twiddle(tab, cg * b0, inv, m);
+    const double step = big_twiddle(tab, cg, inv, m);
+    const double step2 = mulp(step, step, m);
+    double c0 = base, c1 = mulp(base, step, m);
+#pragma unroll
+    for (int v = 0; v < 16; v += 2) {
+        x[brev4(v)] = mulp(x[brev4(v)], c0, m);
+        x[brev4(v + 1)] = mulp(x[brev4(v + 1)], c1, m);
+        if (v + 2 < 16) {
+            c0 = mulp(c0, step2, m);
+            c1 = mulp(c1, step2, m);
+        }
+    }
 }
 
 // Garner recombination of centred residues -> exact coefficient (< 2^98).
-__device__ __forceinline__ unsigned __int128 recombine(double2 v) {
-    const double r0 = v.x < 0.0 ? v.x + kP0 : v.x;
-    const double r1 = v.y < 0.0 ? v.y + kP1 : v.y;
-    double t = mulp(r1 - r0, c_ntt.garner, kP1, kPinv1);
+__device__ __forceinline__ unsigned __int128 recombine(double v0, double v1) {
+    const Mod m1 = mod_of(1);
+    const double r0 = v0 < 0.0 ? v0 + kP0 : v0;
+    const double r1 = v1 < 0.0 ? v1 + kP1 : v1;
+    double t = mulp(r1 - r0, c_ntt.garner, m1);
     t = t < 0.0 ? t + kP1 : t;
     return (unsigned __int128)kPrime0 * (uint64_t)t + (uint64_t)r0;
 }
@@ -195,196 +288,274 @@ struct PassArgs {
     const uint32_t *A;
     const uint32_t *B;
     uint32_t La, Lb, sA, sB;
-    double2 *TA;
-    double2 *TB;
+    double *T;        // planes: A p0, A p1, B p0, B p1; each `plane` elements
+    uint64_t plane;
     uint32_t *C;
     uint32_t Lc, sC;
     uint64_t *spill;
     uint32_t N;  // products in this launch
     Plan plan;
-    const double2 *tables;
+    const double *tables;
 };
 
-// Forward column level `level`.  Level 0 reads limbs; blocks cover
-// (product, operand, tile).  The inter-level twiddle is applied on store.
-template <bool FROM_LIMBS>
+__device__ __forceinline__ double *plane_of(const PassArgs &a, int op, int P) {
+    return a.T + (size_t)(2 * op + P) * a.plane;
+}
+
+// Per-thread geometry of a column tile (level `level`, DFT size 2^M): the
+// thread's elements tid + 256 u are exactly its step-0 elements.
+template <int M>
+struct ColThread {
+    uint64_t off0;     // element offset of u = 0 within the product
+    uint32_t ustep;    // log2 of the offset step between u and u + 1
+    uint32_t cg;       // (col0 + g) << (30 - lk): twiddle exponent per bitrev(r)
+    uint32_t b0;       // bitrev(r0, M)
+    uint32_t lds0;     // LDS slot of u = 0
+
+    __device__ __forceinline__ ColThread(const Plan &p, int level, uint32_t t) {
+        constexpr int LC = kTileLog2 - M;
+        const ColTile ct = col_tile(p, level, t);
+        const uint32_t g = threadIdx.x & ((1u << LC) - 1u);
+        const uint32_t r0 = threadIdx.x >> LC;
+        off0 = col_thread_offset(p, level, ct, threadIdx.x);
+        ustep = (uint32_t)col_row_step_log2(p, level);
+        cg = (ct.col0 + g) << (kMaxLog2 - slab_log2(p, level));
+        b0 = __brev(r0) >> (32 - M);
+        lds0 = lds_index(threadIdx.x);
+    }
+    __device__ __forceinline__ uint64_t off(int u) const { return off0 + ((uint64_t)u << ustep); }
+};
+
+// Forward column level: blocks cover (product, operand, tile); both primes
+// pass through the tile in turn; the inter-level twiddle is applied on
+// store.  Level 0 reads limbs.
+template <bool FROM_LIMBS, int M>
 __global__ void __launch_bounds__(256) ntt_col_fwd_kernel(PassArgs a, int level) {
-    __shared__ double2 lds[kLdsSlots];
-    const Plan &p = a.plan;
-    const uint32_t tiles = tiles_per_product(p);
+    __shared__ double lds[kLdsSlots];
+    const uint32_t tiles = tiles_per_product(a.plan);
     const uint32_t t = blockIdx.x % tiles;
     const uint32_t op = (blockIdx.x / tiles) & 1u;
     const uint32_t prod = blockIdx.x / (2u * tiles);
-    const int lm = p.log2M[level];
-    const int lc = kTileLog2 - lm;
-    const int lk = slab_log2(p, level);
-    const uint64_t K = 1ull << p.log2K;
-    const ColTile ct = col_tile(p, level, t);
-    double2 *T = (op ? a.TB : a.TA) + (size_t)prod * K;
-    const uint32_t *X = op ? a.B : a.A;
-    const uint32_t LX = op ? a.Lb : a.La;
-    const uint32_t sX = op ? a.sB : a.sA;
-#pragma unroll 4
-    for (int u = 0; u < 16; ++u) {
-        const uint32_t i = threadIdx.x + 256u * u;
-        const uint32_t g = i & ((1u << lc) - 1u), r = i >> lc;
-        const uint64_t off = col_offset(p, level, ct, g, r);
-        double2 v;
-        if (FROM_LIMBS) {
-            const double x = off < LX ? (double)X[(size_t)prod * sX + off] : 0.0;
-            v = make_double2(x, x);
-        } else {
-            v = T[off];
-        }
-        lds[lds_index((g << lm) + r)] = v;
-    }
-    __syncthreads();
-    tile_dft<false>(lds, lm, a.tables + kOffTw);
-#pragma unroll 4
-    for (int u = 0; u < 16; ++u) {
-        const uint32_t i = threadIdx.x + 256u * u;
-        const uint32_t g = i & ((1u << lc) - 1u), r = i >> lc;
-        const uint64_t off = col_offset(p, level, ct, g, r);
-        const uint64_t e = (uint64_t)(ct.col0 + g) * brev(r, lm);
-        const double2 w = big_twiddle(a.tables, e << (kMaxLog2 - lk), false);
-        T[off] = mul2(lds[lds_index((g << lm) + r)], w);
-    }
-}
-
-// Last level: forward on A and B tiles, pointwise product, inverse.
-__global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
-    extern __shared__ double2 dyn[];
-    double2 *la = dyn;
-    double2 *lb = dyn + kLdsSlots;
-    const Plan &p = a.plan;
-    const uint32_t tiles = tiles_per_product(p);
-    const uint32_t t = blockIdx.x % tiles;
-    const uint32_t prod = blockIdx.x / tiles;
-    const int lm = p.log2M[p.levels - 1];
-    const uint64_t K = 1ull << p.log2K;
-    double2 *TA = a.TA + (size_t)prod * K;
-    const double2 *TB = a.TB + (size_t)prod * K;
-#pragma unroll 4
-    for (int u = 0; u < 16; ++u) {
-        const uint32_t i = threadIdx.x + 256u * u;
-        const uint64_t off = row_offset(p, t, i >> lm, i & ((1u << lm) - 1u));
-        la[lds_index(i)] = TA[off];
-        lb[lds_index(i)] = TB[off];
-    }
-    __syncthreads();
-    tile_dft<false>(la, lm, a.tables + kOffTw);
-    tile_dft<false>(lb, lm, a.tables + kOffTw);
-#pragma unroll 4
-    for (int u = 0; u < 16; ++u) {
-        const uint32_t i = lds_index(threadIdx.x + 256u * u);
-        la[i] = mul2(la[i], lb[i]);
-    }
-    __syncthreads();
-    tile_dft<true>(la, lm, a.tables + kOffTwInv);
-#pragma unroll 4
-    for (int u = 0; u < 16; ++u) {
-        const uint32_t i = threadIdx.x + 256u * u;
-        const uint64_t off = row_offset(p, t, i >> lm, i & ((1u << lm) - 1u));
-        TA[off] = la[lds_index(i)];
-    }
-}
-
-// Inverse column level.  Level 0 finishes the product: 1/K, Garner, limbs
-// of each run of C adjacent coefficients, and the run's spill word.
-template <bool TO_LIMBS>
-__global__ void __launch_bounds__(256) ntt_col_inv_kernel(PassArgs a, int level) {
-    __shared__ double2 lds[kLdsSlots];
-    const Plan &p = a.plan;
-    const uint32_t tiles = tiles_per_product(p);
-    const uint32_t t = blockIdx.x % tiles;
-    const uint32_t prod = blockIdx.x / tiles;
-    const int lm = p.log2M[level];
-    const int lc = kTileLog2 - lm;
-    const int lk = slab_log2(p, level);
-    const uint64_t K = 1ull << p.log2K;
-    const ColTile ct = col_tile(p, level, t);
-    double2 *T = a.TA + (size_t)prod * K;
-#pragma unroll 4
-    for (int u = 0; u < 16; ++u) {
-        const uint32_t i = threadIdx.x + 256u * u;
-        const uint32_t g = i & ((1u << lc) - 1u), r = i >> lc;
-        const uint64_t off = col_offset(p, level, ct, g, r);
-        const uint64_t e = (uint64_t)(ct.col0 + g) * brev(r, lm);
-        const double2 w = big_twiddle(a.tables, e << (kMaxLog2 - lk), true);
-        lds[lds_index((g << lm) + r)] = mul2(T[off], w);
-    }
-    __syncthreads();
-    tile_dft<true>(lds, lm, a.tables + kOffTwInv);
-    if (!TO_LIMBS) {
-#pragma unroll 4
+    const uint64_t K = 1ull << a.plan.log2K;
+    const ColThread<M> ct(a.plan, level, t);
+    uint32_t xin[16];
+    if (FROM_LIMBS) {
+        const uint32_t *X = (op ? a.B : a.A) + (size_t)prod * (op ? a.sB : a.sA);
+        const uint32_t LX = op ? a.Lb : a.La;
+#pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const uint32_t i = threadIdx.x + 256u * u;
-            const uint32_t g = i & ((1u << lc) - 1u), r = i >> lc;
-            T[col_offset(p, level, ct, g, r)] = lds[lds_index((g << lm) + r)];
+            const uint64_t o = ct.off(u);
+            xin[u] = o < LX ? X[o] : 0u;
         }
-        return;
     }
-    // level 0: slab 0 covers the product; run r = coefficients r*S + col0 + [0, C)
-    const double2 kinv = c_ntt.kinv[p.log2K];
-    const uint32_t C = 1u << lc;
-    const int ls = lk - lm;  // log2 S
+#pragma clang loop unroll(disable)
+    for (int P = 0; P < 2; ++P) {
+        const Mod m = mod_of(P);
+        const double *tab = a.tables + P * kPrimeTable;
+        double *T = plane_of(a, (int)op, P) + (size_t)prod * K;
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = FROM_LIMBS ? (double)xin[u] : T[ct.off(u)];
+        fwd_dft<M, true>(v, lds, tab, P, m);
+        lds_store(lds, v, StepMap<M, step_count(M) - 1, true>());
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = lds[ct.lds0 + 272u * u];
+        column_twiddles(v, tab, ct.cg, ct.b0, false, m);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) T[ct.off(u)] = v[u];
+        __syncthreads();
+    }
+}
+
+// Step-0 elements of a row tile (DFT size 2^M): v[j] is tile index
+// row0 + j 2^(M-4).
+template <int M>
+__device__ __forceinline__ uint32_t row_step0_base() {
+    return ((threadIdx.x >> (M - 4)) << M) + (threadIdx.x & ((1u << (M - 4)) - 1u));
+}
+
+// Last level: per prime, forward on A's tile (kept in registers), forward on
+// B's tile, pointwise product, inverse; written back over A's plane.
+template <int M>
+__global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
+    __shared__ double lds[kLdsSlots];
+    constexpr uint32_t kJ = 1u << (M - 4);
+    const uint32_t tiles = tiles_per_product(a.plan);
+    const uint32_t t = blockIdx.x % tiles;
+    const uint32_t prod = blockIdx.x / tiles;
+    const uint64_t K = 1ull << a.plan.log2K;
+    const size_t g0 = (size_t)prod * K + (size_t)t * kTile + row_step0_base<M>();
+#pragma clang loop unroll(disable)
+    for (int P = 0; P < 2; ++P) {
+        const Mod m = mod_of(P);
+        const double *tab = a.tables + P * kPrimeTable;
+        double *TA = plane_of(a, 0, P) + g0;
+        const double *TB = plane_of(a, 1, P) + g0;
+        double ra[16], v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) ra[j] = TA[j * kJ];
+        fwd_dft<M, false>(ra, lds, tab, P, m);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = TB[j * kJ];
+        fwd_dft<M, false>(v, lds, tab, P, m);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = mulp(v[j], ra[j], m);
+        inv_dft<M, false>(v, lds, tab, P, m);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) TA[j * kJ] = v[j];
+        __syncthreads();
+    }
+}
+
+// Inverse column level of prime P: inverse twiddle on load, inverse DFT;
+// returns the thread's elements tid + 256 u in v.
+template <int M>
+__device__ __forceinline__ void col_inv_prime(double *v, const PassArgs &a, double *lds, const ColThread<M> &ct,
+                                              uint32_t prod, int P) {
+    constexpr int kLast = step_count(M) - 1;
+    const Mod m = mod_of(P);
+    const double *tab = a.tables + P * kPrimeTable;
+    const double *T = plane_of(a, 0, P) + (size_t)prod * (1ull << a.plan.log2K);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = T[ct.off(u)];
+    column_twiddles(v, tab, ct.cg, ct.b0, true, m);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lds[ct.lds0 + 272u * u] = v[u];
+    __syncthreads();
+    lds_steps<M, true, true, kLast, 1>(lds, tab, P, m);
+    const StepMap<M, 0, true> s0;
+    lds_load(v, lds, s0);
+    step_regs<M, 0, true, true>(v, s0, tab, P, m);
+}
+
+// Inverse column level above the first: both primes through one tile.
+template <int M>
+__global__ void __launch_bounds__(256) ntt_col_inv_kernel(PassArgs a, int level) {
+    __shared__ double lds[kLdsSlots];
+    const uint32_t tiles = tiles_per_product(a.plan);
+    const uint32_t t = blockIdx.x % tiles;
+    const uint32_t prod = blockIdx.x / tiles;
+    const ColThread<M> ct(a.plan, level, t);
+#pragma clang loop unroll(disable)
+    for (int P = 0; P < 2; ++P) {
+        double v[16];
+        col_inv_prime<M>(v, a, lds, ct, prod, P);
+        double *T = plane_of(a, 0, P) + (size_t)prod * (1ull << a.plan.log2K);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) T[ct.off(u)] = v[u];
+        __syncthreads();
+    }
+}
+
+// First inverse column level: both primes (two tiles), 1/K, Garner, limbs of
+// each run of <= 16 adjacent coefficients and the run's spill word.
+template <int M>
+__global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
+    __shared__ double l0[kLdsSlots];
+    __shared__ double l1[kLdsSlots];
+    constexpr int LC = kTileLog2 - M;
+    constexpr int RL = run_log2_for(M);
+    constexpr uint32_t kRunLen = 1u << RL;
+    const uint32_t tiles = tiles_per_product(a.plan);
+    const uint32_t t = blockIdx.x % tiles;
+    const uint32_t prod = blockIdx.x / tiles;
+    const int log2K = a.plan.log2K;
+    const ColThread<M> ct(a.plan, 0, t);
+    {
+        double v[16];
+        col_inv_prime<M>(v, a, l0, ct, prod, 0);
+        const double kinv0 = c_ntt.kinv[0][log2K];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) l0[ct.lds0 + 272u * u] = mulp(v[u], kinv0, mod_of(0));
+        col_inv_prime<M>(v, a, l1, ct, prod, 1);
+        const double kinv1 = c_ntt.kinv[1][log2K];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) l1[ct.lds0 + 272u * u] = mulp(v[u], kinv1, mod_of(1));
+    }
+    __syncthreads();
+    const uint32_t col0 = col_tile(a.plan, 0, t).col0;
+    const int ls = log2K - M;
     uint32_t *out = a.C + (size_t)prod * a.sC;
-    for (uint32_t r = threadIdx.x; r < (1u << lm); r += 256u) {
-        const uint64_t n0 = ((uint64_t)r << ls) + ct.col0;
+    const bool vec = (((uintptr_t)out) & 15u) == 0;
+#pragma unroll
+    for (uint32_t q = threadIdx.x; q < (kTile >> RL); q += 256u) {
+        const Run run = run_of(M, q);
+        const uint32_t e0 = lds_index((run.r << LC) + run.g0);
+        const uint64_t n0 = ((uint64_t)run.r << ls) + col0 + run.g0;
+        uint32_t limb[kRunLen];
         unsigned __int128 pending = 0;
-        for (uint32_t g = 0; g < C; ++g) {
-            pending += recombine(mul2(lds[lds_index((g << lm) + r)], kinv));
-            const uint64_t n = n0 + g;
-            if (n < a.Lc) out[n] = (uint32_t)pending;
+#pragma unroll
+        for (uint32_t j = 0; j < kRunLen; ++j) {
+            pending += recombine(l0[e0 + j], l1[e0 + j]);
+            limb[j] = (uint32_t)pending;
             pending >>= 32;
         }
-        a.spill[(size_t)prod * (K >> lc) + (n0 >> lc)] = (uint64_t)pending;
+        if (vec && n0 + kRunLen <= a.Lc) {
+#pragma unroll
+            for (uint32_t j = 0; j < kRunLen; j += 4)
+                *reinterpret_cast<uint4 *>(out + n0 + j) = make_uint4(limb[j], limb[j + 1], limb[j + 2], limb[j + 3]);
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kRunLen; ++j)
+                if (n0 + j < a.Lc) out[n0 + j] = limb[j];
+        }
+        a.spill[(size_t)prod * ((1ull << log2K) >> RL) + (n0 >> RL)] = (uint64_t)pending;
     }
 }
 
-// K <= 4096: whole products inside one tile (G = 4096 / K per block).
+// K = 2^M <= 4096, one prime: transforms of A (kept in registers) and B,
+// pointwise, inverse; the result (times 1/K) is left in `out` in tile order.
+template <int M>
+__device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, double *out, uint32_t prod0, int P) {
+    constexpr uint32_t kJ = 1u << (M - 4);
+    const Mod m = mod_of(P);
+    const double *tab = a.tables + P * kPrimeTable;
+    const uint32_t i0 = row_step0_base<M>();
+    double ra[16], v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t i = i0 + j * kJ;
+        const uint32_t prod = prod0 + (i >> M), r = i & ((1u << M) - 1u);
+        ra[j] = (prod < a.N && r < a.La) ? (double)a.A[(size_t)prod * a.sA + r] : 0.0;
+        v[j] = (prod < a.N && r < a.Lb) ? (double)a.B[(size_t)prod * a.sB + r] : 0.0;
+    }
+    fwd_dft<M, false>(ra, lds, tab, P, m);
+    if constexpr (M > 4) __syncthreads();
+    fwd_dft<M, false>(v, lds, tab, P, m);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = mulp(v[j], ra[j], m);
+    inv_dft<M, false>(v, lds, tab, P, m);
+    const double kinv = c_ntt.kinv[P][M];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) out[lds_index(i0 + j * kJ)] = mulp(v[j], kinv, m);
+}
+
+// K = 2^M <= 4096: whole products inside one tile (4096 / K per block).
+template <int M>
 __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
-    extern __shared__ double2 dyn[];
+    __shared__ double l0[kLdsSlots];
+    __shared__ double l1[kLdsSlots];
     __shared__ uint64_t spill_s[256];
     __shared__ uint32_t wave_flags[4];
-    double2 *la = dyn;
-    double2 *lb = dyn + kLdsSlots;
-    const int lm = a.plan.log2K;
-    const uint32_t G = kTile >> lm;
-    const uint32_t prod0 = blockIdx.x * G;
-#pragma unroll 4
-    for (int u = 0; u < 16; ++u) {
-        const uint32_t i = threadIdx.x + 256u * u;
-        const uint32_t prod = prod0 + (i >> lm), r = i & ((1u << lm) - 1u);
-        double xa = 0.0, xb = 0.0;
-        if (prod < a.N) {
-            if (r < a.La) xa = (double)a.A[(size_t)prod * a.sA + r];
-            if (r < a.Lb) xb = (double)a.B[(size_t)prod * a.sB + r];
-        }
-        la[lds_index(i)] = make_double2(xa, xa);
-        lb[lds_index(i)] = make_double2(xb, xb);
-    }
+    const uint32_t prod0 = blockIdx.x * (kTile >> M);
+    // prime 0 transforms in l1 and leaves its result in l0; prime 1 the reverse
+    local_prime<M>(a, l1, l0, prod0, 0);
     __syncthreads();
-    tile_dft<false>(la, lm, a.tables + kOffTw);
-    tile_dft<false>(lb, lm, a.tables + kOffTw);
-#pragma unroll 4
-    for (int u = 0; u < 16; ++u) {
-        const uint32_t i = lds_index(threadIdx.x + 256u * u);
-        la[i] = mul2(la[i], lb[i]);
-    }
+    local_prime<M>(a, l1, l1, prod0, 1);
     __syncthreads();
-    tile_dft<true>(la, lm, a.tables + kOffTwInv);
 
     // each thread: 16 consecutive coefficients of one product
-    const double2 kinv = c_ntt.kinv[lm];
     const uint32_t i0 = threadIdx.x * 16u;
-    const uint32_t prod = prod0 + (i0 >> lm);
-    const uint32_t r0 = i0 & ((1u << lm) - 1u);
+    const uint32_t prod = prod0 + (i0 >> M);
+    const uint32_t r0 = i0 & ((1u << M) - 1u);
+    const uint32_t e0 = lds_index(i0);
     uint32_t limb[16];
     unsigned __int128 pending = 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        pending += recombine(mul2(la[lds_index(i0 + j)], kinv));
+        pending += recombine(l0[e0 + j], l1[e0 + j]);
         limb[j] = (uint32_t)pending;
         pending >>= 32;
     }
@@ -405,7 +576,7 @@ __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
         all_ones = all_ones && limb[j] == 0xffffffffu;
     }
     // the last thread of a product neither passes nor forwards a carry
-    const bool last_of_product = (r0 + 16u) == (1u << lm);
+    const bool last_of_product = (r0 + 16u) == (1u << M);
     const bool gen = !last_of_product && c != 0;
     const bool prop = !last_of_product && c == 0 && all_ones;
     const uint64_t Gm = ballot64(gen), Pm = ballot64(prop);
@@ -432,6 +603,16 @@ __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
         }
     }
 }
+
+// Calls f(std::integral_constant<int, M>) for the M in [LO, HI] equal to m.
+template <int LO, int HI, typename F>
+void dispatch_m(int m, F &&f) {
+    if constexpr (LO <= HI) {
+        if (m == LO) f(std::integral_constant<int, LO>{});
+        else dispatch_m<LO + 1, HI>(m, f);
+    }
+}
+constexpr int kColLo = 5, kColHi = 9, kRowLo = 6, kRowHi = 12, kLocalLo = kMinLog2, kLocalHi = kTileLog2;
 
 // ---------------------------------------------------------------- host side
 
@@ -470,14 +651,12 @@ size_t chunk_products(uint32_t N, int log2K) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-bool g_attr_done = false;
-
 // Optional per-launch timing of the NTT kernels (HIP events on the launch
 // stream) with each launch's algorithmic bytes, for live roofline numbers.
 enum KernelKind { kColFwdLimbs, kColFwd, kRowFused, kColInv, kColInvLimbs, kLocal, kKindCount };
 const char *const kKindNames[kKindCount] = {"ntt_col_fwd_kernel<limbs>", "ntt_col_fwd_kernel",
                                             "ntt_row_fused_kernel", "ntt_col_inv_kernel",
-                                            "ntt_col_inv_kernel<limbs>", "ntt_local_kernel"};
+                                            "ntt_col_inv_limbs_kernel", "ntt_local_kernel"};
 struct StatRecord {
     int kind;
     hipEvent_t start, stop;
@@ -531,39 +710,41 @@ size_t ntt_workspace_bytes(uint32_t N, uint32_t L_a, uint32_t L_b) {
     const size_t nc = chunk_products(N, k);
     const size_t K = size_t(1) << k;
     const size_t runs = K >> final_run_log2(p);
-    return align256(nc * K * sizeof(double2)) * 2 + align256(nc * runs * sizeof(uint64_t)) +
+    return 4 * align256(nc * K * sizeof(double)) + align256(nc * runs * sizeof(uint64_t)) +
            align256(carry_flags_bytes((uint32_t)nc, L_a + L_b));
 }
 
 hipError_t ntt_tables_init(NTTPrecomputedTables *tables) {
     const uint64_t primes[2] = {kPrime0, kPrime1};
     const uint64_t gens[2] = {kGen0, kGen1};
-    std::vector<double2> host(kTableLen);
+    std::vector<double> host(kTableLen);
     NttConsts cst{};
     for (int q = 0; q < 2; ++q) {
         const uint64_t P = primes[q];
+        double *tab = host.data() + q * kPrimeTable;
         const uint64_t w30 = powmod_u64(gens[q], (P - 1) >> kMaxLog2, P);
         const uint64_t w30i = powmod_u64(w30, P - 2, P);
-        auto put = [&](size_t idx, uint64_t v) {
-            double *d = reinterpret_cast<double *>(&host[idx]);
-            d[q] = centred(v, P);
-        };
-        const uint64_t w4096 = powmod_u64(w30, 1ull << (kMaxLog2 - kTileLog2), P);
-        const uint64_t w4096i = powmod_u64(w4096, P - 2, P);
-        uint64_t x = 1, y = 1;
-        for (size_t e = 0; e < 4096; ++e) {
-            put(kOffTw + e, x);
-            put(kOffTwInv + e, y);
-            x = mulmod_u64(x, w4096, P);
-            y = mulmod_u64(y, w4096i, P);
+        // in-tile step tables, [j][n2] for Ms = 2^x
+        for (int x = 5; x <= kTileLog2; ++x) {
+            const uint64_t wx = powmod_u64(w30, 1ull << (kMaxLog2 - x), P);
+            const uint64_t wxi = powmod_u64(wx, P - 2, P);
+            const uint32_t ss = 1u << (x - 4);
+            for (uint32_t j = 0; j < 16; ++j) {
+                const uint32_t bj = bitrev(j, 4);
+                for (uint32_t n2 = 0; n2 < ss; ++n2) {
+                    const size_t slot = ((size_t)1 << x) - 32 + (size_t)j * ss + n2;
+                    tab[kOffStep + slot] = centred(powmod_u64(wx, (uint64_t)n2 * bj, P), P);
+                    tab[kOffStepInv + slot] = centred(powmod_u64(wxi, (uint64_t)n2 * bj, P), P);
+                }
+            }
         }
         const uint64_t step_hi = powmod_u64(w30, 32768, P), step_hi_i = powmod_u64(w30i, 32768, P);
         uint64_t lo = 1, loi = 1, hi = 1, hii = 1;
         for (size_t e = 0; e < 32768; ++e) {
-            put(kOffLo + e, lo);
-            put(kOffLoInv + e, loi);
-            put(kOffHi + e, hi);
-            put(kOffHiInv + e, hii);
+            tab[kOffLo + e] = centred(lo, P);
+            tab[kOffLoInv + e] = centred(loi, P);
+            tab[kOffHi + e] = centred(hi, P);
+            tab[kOffHiInv + e] = centred(hii, P);
             lo = mulmod_u64(lo, w30, P);
             loi = mulmod_u64(loi, w30i, P);
             hi = mulmod_u64(hi, step_hi, P);
@@ -572,30 +753,23 @@ hipError_t ntt_tables_init(NTTPrecomputedTables *tables) {
         const uint64_t w16 = powmod_u64(w30, 1ull << (kMaxLog2 - 4), P);
         const uint64_t w16i = powmod_u64(w16, P - 2, P);
         for (int e = 0; e < 8; ++e) {
-            reinterpret_cast<double *>(&cst.w16[e])[q] = centred(powmod_u64(w16, e, P), P);
-            reinterpret_cast<double *>(&cst.w16i[e])[q] = centred(powmod_u64(w16i, e, P), P);
+            cst.w16[q][e] = centred(powmod_u64(w16, e, P), P);
+            cst.w16i[q][e] = centred(powmod_u64(w16i, e, P), P);
         }
-        for (int k = 0; k <= kMaxLog2; ++k) {
-            reinterpret_cast<double *>(&cst.kinv[k])[q] = centred(powmod_u64(powmod_u64(2, k, P), P - 2, P), P);
-        }
+        for (int k = 0; k <= kMaxLog2; ++k)
+            cst.kinv[q][k] = centred(powmod_u64(powmod_u64(2, k, P), P - 2, P), P);
     }
     cst.garner = centred(powmod_u64(kPrime0 % kPrime1, kPrime1 - 2, kPrime1), kPrime1);
 
-    double2 *d = nullptr;
-    GPUPI_RETURN_IF(hipMalloc(&d, kTableLen * sizeof(double2)));
-    hipError_t err = hipMemcpy(d, host.data(), kTableLen * sizeof(double2), hipMemcpyHostToDevice);
+    double *d = nullptr;
+    GPUPI_RETURN_IF(hipMalloc(&d, kTableLen * sizeof(double)));
+    hipError_t err = hipMemcpy(d, host.data(), kTableLen * sizeof(double), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpyToSymbol(HIP_SYMBOL(c_ntt), &cst, sizeof(cst));
     if (err != hipSuccess) {
         (void)hipFree(d);
         return err;
     }
-    if (!g_attr_done) {
-        const int dyn = (int)(2 * kLdsSlots * sizeof(double2));
-        (void)hipFuncSetAttribute((const void *)ntt_row_fused_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-        (void)hipFuncSetAttribute((const void *)ntt_local_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-        g_attr_done = true;
-    }
-    tables->roots = reinterpret_cast<double *>(d);
+    tables->roots = d;
     tables->max_log2 = kMaxLog2;
     return hipSuccess;
 }
@@ -616,7 +790,6 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
     const Plan p = make_plan(k);
     const uint32_t Lc = L_a + L_b;
     const size_t K = size_t(1) << k;
-    const size_t dyn = 2 * kLdsSlots * sizeof(double2);
 
     PassArgs a{};
     a.La = L_a;
@@ -626,7 +799,7 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
     a.Lc = Lc;
     a.sC = stride_C;
     a.plan = p;
-    a.tables = reinterpret_cast<const double2 *>(tables.roots);
+    a.tables = tables.roots;
 
     if (p.levels == 1) {
         const uint32_t G = kTile >> k;
@@ -638,18 +811,21 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
         a.N = N;
         const double bytes = 4.0 * N * ((double)L_a + L_b + Lc);
         return timed(kLocal, bytes, stream, [&] {
-            hipLaunchKernelGGL(ntt_local_kernel, dim3((uint32_t)blocks), dim3(256), dyn, stream, a);
+            dispatch_m<kLocalLo, kLocalHi>(k, [&](auto mc) {
+                hipLaunchKernelGGL((ntt_local_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256), 0,
+                                   stream, a);
+            });
         });
     }
     if (workspace == nullptr || workspace_bytes < ntt_workspace_bytes(N, L_a, L_b)) return hipErrorInvalidValue;
     const size_t nc = chunk_products(N, k);
     const size_t runs = K >> final_run_log2(p);
+    const size_t plane_bytes = align256(nc * K * sizeof(double));
     char *ws = static_cast<char *>(workspace);
-    a.TA = reinterpret_cast<double2 *>(ws);
-    a.TB = reinterpret_cast<double2 *>(ws + align256(nc * K * sizeof(double2)));
-    a.spill = reinterpret_cast<uint64_t *>(ws + 2 * align256(nc * K * sizeof(double2)));
-    uint8_t *flags = reinterpret_cast<uint8_t *>(ws + 2 * align256(nc * K * sizeof(double2)) +
-                                                 align256(nc * runs * sizeof(uint64_t)));
+    a.T = reinterpret_cast<double *>(ws);
+    a.plane = plane_bytes / sizeof(double);
+    a.spill = reinterpret_cast<uint64_t *>(ws + 4 * plane_bytes);
+    uint8_t *flags = reinterpret_cast<uint8_t *>(ws + 4 * plane_bytes + align256(nc * runs * sizeof(uint64_t)));
     const uint32_t tiles = tiles_per_product(p);
     for (size_t r0 = 0; r0 < N; r0 += nc) {
         const uint32_t n = (uint32_t)((N - r0) < nc ? (N - r0) : nc);
@@ -659,26 +835,42 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
         a.N = n;
         const uint64_t fwd_blocks = (uint64_t)n * 2 * tiles, blocks = (uint64_t)n * tiles;
         if (fwd_blocks > 0x7fffffffull) return hipErrorInvalidValue;
-        // algorithmic bytes: residue pairs are 16 B, limbs 4 B
+        // algorithmic bytes: 16 B per residue pair (one binary64 per prime),
+        // 4 B per limb
         const double tb = 16.0 * (double)n * (double)K;
         GPUPI_RETURN_IF(timed(kColFwdLimbs, 4.0 * n * ((double)L_a + L_b) + 2 * tb, stream, [&] {
-            hipLaunchKernelGGL(ntt_col_fwd_kernel<true>, dim3((uint32_t)fwd_blocks), dim3(256), 0, stream, a, 0);
+            dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
+                hipLaunchKernelGGL((ntt_col_fwd_kernel<true, decltype(mc)::value>), dim3((uint32_t)fwd_blocks),
+                                   dim3(256), 0, stream, a, 0);
+            });
         }));
         if (p.levels == 3) {
             GPUPI_RETURN_IF(timed(kColFwd, 4 * tb, stream, [&] {
-                hipLaunchKernelGGL(ntt_col_fwd_kernel<false>, dim3((uint32_t)fwd_blocks), dim3(256), 0, stream, a, 1);
+                dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
+                    hipLaunchKernelGGL((ntt_col_fwd_kernel<false, decltype(mc)::value>), dim3((uint32_t)fwd_blocks),
+                                       dim3(256), 0, stream, a, 1);
+                });
             }));
         }
         GPUPI_RETURN_IF(timed(kRowFused, 3 * tb, stream, [&] {
-            hipLaunchKernelGGL(ntt_row_fused_kernel, dim3((uint32_t)blocks), dim3(256), dyn, stream, a);
+            dispatch_m<kRowLo, kRowHi>(p.log2M[p.levels - 1], [&](auto mc) {
+                hipLaunchKernelGGL((ntt_row_fused_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256),
+                                   0, stream, a);
+            });
         }));
         if (p.levels == 3) {
             GPUPI_RETURN_IF(timed(kColInv, 2 * tb, stream, [&] {
-                hipLaunchKernelGGL(ntt_col_inv_kernel<false>, dim3((uint32_t)blocks), dim3(256), 0, stream, a, 1);
+                dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
+                    hipLaunchKernelGGL((ntt_col_inv_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256),
+                                       0, stream, a, 1);
+                });
             }));
         }
         GPUPI_RETURN_IF(timed(kColInvLimbs, tb + 4.0 * n * Lc + 8.0 * n * runs, stream, [&] {
-            hipLaunchKernelGGL(ntt_col_inv_kernel<true>, dim3((uint32_t)blocks), dim3(256), 0, stream, a, 0);
+            dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
+                hipLaunchKernelGGL((ntt_col_inv_limbs_kernel<decltype(mc)::value>), dim3((uint32_t)blocks),
+                                   dim3(256), 0, stream, a);
+            });
         }));
         GPUPI_RETURN_IF(launch_add_spills(a.C, a.spill, n, Lc, stride_C, 1u << final_run_log2(p), (uint32_t)runs,
                                           flags, stream));
@@ -729,9 +921,7 @@ void gpupi_kernel_stats_reset(int enable) {
 int gpupi_kernel_stats_query(GpupiKernelStat *out, int max_entries) {
     using namespace gpupi;
     GpupiKernelStat agg[kKindCount] = {};
-    for (int k = 0; k < kKindCount; ++k) {
-        std::snprintf(agg[k].name, sizeof(agg[k].name), "%s", kKindNames[k]);
-    }
+    for (int k = 0; k < kKindCount; ++k) std::snprintf(agg[k].name, sizeof(agg[k].name), "%s", kKindNames[k]);
     for (auto &r : g_stats.records) {
         if (hipEventSynchronize(r.stop) != hipSuccess) return -1;
         float ms = 0.0f;

@@ -8,8 +8,8 @@
 // element (r, c) sits at r * S_l + c with S_l = K_l / M_l, and the level does
 // M_l-point DFTs down each column c.  A block holds a 4096-element tile: G
 // DFTs of M points each (G * M = 4096).  Column levels tile C = 4096 / M_l
-// adjacent columns (>= 8, i.e. >= 128-byte runs of double2); the last level
-// is contiguous (S = 1) and tiles G whole slabs.
+// adjacent columns (8..128 of them, runs of >= 64 bytes per prime plane);
+// the last level is contiguous (S = 1) and tiles G whole slabs.
 #pragma once
 
 #include <stdint.h>
@@ -32,9 +32,13 @@ constexpr int kMaxLevels = 3;
 constexpr int kMaxLog2 = 30;
 constexpr int kMinLog2 = 4;
 
-// LDS index of tile element i (one pad slot every 32 elements).
-GPUPI_HD uint32_t lds_index(uint32_t i) { return i + (i >> 5); }
-constexpr uint32_t kLdsSlots = kTile + (kTile >> 5);
+// LDS slot of tile element i: one pad slot every 16 elements, so lanes
+// reading 8-byte elements at a stride of 16 elements spread over all banks.
+// Inside an in-tile step lds_index(base + j * stride) equals
+// lds_index(base) + lds_index(j * stride), so the per-element LDS offsets of
+// a small DFT are compile-time constants.
+GPUPI_HD uint32_t lds_index(uint32_t i) { return i + (i >> 4); }
+constexpr uint32_t kLdsSlots = kTile + (kTile >> 4);
 
 struct Plan {
     int log2K;
@@ -100,6 +104,16 @@ GPUPI_HD uint64_t col_offset(const Plan &p, int level, ColTile ct, uint32_t g, u
     return ((uint64_t)ct.slab << lk) + ((uint64_t)r << (lk - lm)) + ct.col0 + g;
 }
 
+// Column tiles: thread `tid` of the 256 holds elements i = tid + 256 u
+// (u < 16) of the tile, i.e. column g = tid mod C and rows
+// r = (tid >> lc) + u 2^(lm-4); moving from u to u + 1 advances the element
+// offset by 2^(lk-4) and adds bitrev(u, 4) to bitrev(r, lm).
+GPUPI_HD uint64_t col_thread_offset(const Plan &p, int level, ColTile ct, uint32_t tid) {
+    const int lc = kTileLog2 - p.log2M[level];
+    return col_offset(p, level, ct, tid & ((1u << lc) - 1u), tid >> lc);
+}
+GPUPI_HD int col_row_step_log2(const Plan &p, int level) { return slab_log2(p, level) - 4; }
+
 GPUPI_HD uint32_t tiles_per_product(const Plan &p) { return 1u << (p.log2K - kTileLog2 > 0 ? p.log2K - kTileLog2 : 0); }
 
 // Row (last) level: tile t holds G = 4096 / M consecutive slabs.
@@ -120,40 +134,72 @@ GPUPI_HD uint32_t bitrev(uint32_t x, int bits) {
 }
 
 // Radix of step s for an M = 2^m point in-tile DFT: 16s, then the rest.
-GPUPI_HD int step_count(int m) { return (m + 3) / 4; }
-GPUPI_HD int step_log2radix(int m, int s) {
-    const int full = m / 4;
-    return s < full ? 4 : (m - 4 * full);
+// Only the last step can have a smaller radix, and the last step's
+// sub-problems have Ss = 1 (no twiddles), so every twiddled step is radix 16.
+GPUPI_HD constexpr int step_count(int m) { return (m + 3) / 4; }
+GPUPI_HD constexpr int step_log2radix(int m, int s) { return s < m / 4 ? 4 : (m - 4 * (m / 4)); }
+// log2 of the sub-problem size Ms that step s works on (Ms = R * Ss)
+GPUPI_HD constexpr int step_ms_log2(int m, int s) {
+    int x = m;
+    for (int i = 0; i < s; ++i) x -= step_log2radix(m, i);
+    return x;
 }
 
-// Small DFT q (of 4096 / R per tile) at step s: element j sits at tile index
-// g*M + b*Ms + j*Ss + n2.  n2 varies fastest with q.
+// Small DFT q (of 4096 / R per tile) at step s of G = 4096 / 2^m DFTs of
+// M = 2^m points.  Element j of the small DFT is point r = b*Ms + j*Ss + n2
+// of DFT g.  Two tile layouts: row tiles (contiguous levels) store point r of
+// DFT g at tile index g*M + r and let n2 vary fastest with q; column tiles
+// store it at r*G + g (the order of global memory) and let g vary fastest,
+// so consecutive lanes touch consecutive tile indices in both layouts.
 struct SmallDft {
     uint32_t base;   // tile index of element j = 0
-    uint32_t stride; // Ss
+    uint32_t stride; // tile-index step between elements j and j + 1
     uint32_t n2;     // position inside the current sub-problem (twiddle)
     int log2Ms;
 };
 
-GPUPI_HD SmallDft small_dft(int m, int s, uint32_t q) {
-    int log2Ms = m;
-    for (int i = 0; i < s; ++i) log2Ms -= step_log2radix(m, i);
+GPUPI_HD SmallDft small_dft(int m, int s, uint32_t q, bool col) {
+    const int log2Ms = step_ms_log2(m, s);
     const int lr = step_log2radix(m, s);
     const int log2Ss = log2Ms - lr;
+    const int lg = kTileLog2 - m;  // log2 G
     SmallDft d;
-    d.n2 = q & ((1u << log2Ss) - 1u);
-    const uint32_t rest = q >> log2Ss;
-    const uint32_t b = rest & ((1u << (m - log2Ms)) - 1u);
-    const uint32_t g = rest >> (m - log2Ms);
-    d.base = (g << m) + (b << log2Ms) + d.n2;
-    d.stride = 1u << log2Ss;
+    uint32_t g, b;
+    if (col) {
+        g = q & ((1u << lg) - 1u);
+        const uint32_t rest = q >> lg;
+        d.n2 = rest & ((1u << log2Ss) - 1u);
+        b = rest >> log2Ss;
+        d.base = (((b << log2Ms) + d.n2) << lg) + g;
+        d.stride = 1u << (log2Ss + lg);
+    } else {
+        d.n2 = q & ((1u << log2Ss) - 1u);
+        const uint32_t rest = q >> log2Ss;
+        b = rest & ((1u << (m - log2Ms)) - 1u);
+        g = rest >> (m - log2Ms);
+        d.base = (g << m) + (b << log2Ms) + d.n2;
+        d.stride = 1u << log2Ss;
+    }
     d.log2Ms = log2Ms;
     return d;
 }
 
-// Run length of the final inverse column level (limbs per run).
+// Run length of the final inverse column level (limbs per run): the C
+// adjacent coefficients of one tile row, cut into runs of at most 16 so
+// that every thread of the block owns at least one run.
+GPUPI_HD constexpr int run_log2_for(int lm0) { return (kTileLog2 - lm0) < 4 ? (kTileLog2 - lm0) : 4; }
 GPUPI_HD uint32_t final_run_log2(const Plan &p) {
-    return p.levels == 1 ? 4 : (uint32_t)(kTileLog2 - p.log2M[0]);
+    return p.levels == 1 ? 4u : (uint32_t)run_log2_for(p.log2M[0]);
+}
+
+// Run q (< 4096 >> run_log2) of a first-level tile: row r, first column g0.
+struct Run {
+    uint32_t r, g0;
+};
+GPUPI_HD Run run_of(int lm0, uint32_t q) {
+    const int lc = kTileLog2 - lm0;
+    const int rl = run_log2_for(lm0);
+    return Run{q >> (lc - rl), (q & ((1u << (lc - rl)) - 1u)) << rl};
 }
 
 }  // namespace ntt

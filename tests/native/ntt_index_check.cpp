@@ -15,6 +15,7 @@ static int fails = 0;
 #define EXPECT(c, ...) do { if (!(c)) { if (fails++ < 20) { printf("FAIL: " __VA_ARGS__); printf("\n"); } } } while (0)
 
 static void check_tile_steps() {
+    for (int col = 0; col < 2; ++col)
     for (int m = 1; m <= kTileLog2; ++m) {
         for (int s = 0; s < step_count(m); ++s) {
             const int lr = step_log2radix(m, s);
@@ -22,18 +23,52 @@ static void check_tile_steps() {
             const uint32_t R = 1u << lr;
             std::vector<int> seen(kTile, 0);
             for (uint32_t q = 0; q < kTile / R; ++q) {
-                const SmallDft d = small_dft(m, s, q);
+                const SmallDft d = small_dft(m, s, q, col != 0);
                 for (uint32_t j = 0; j < R; ++j) {
                     const uint32_t idx = d.base + j * d.stride;
                     EXPECT(idx < kTile, "step idx m=%d s=%d q=%u", m, s, q);
                     if (idx < kTile) seen[idx]++;
                     EXPECT(lds_index(idx) < kLdsSlots, "lds slot");
                 }
-                EXPECT((d.n2 << lr) < (1u << d.log2Ms) || d.log2Ms == lr, "twiddle exponent range");
+                EXPECT((d.n2 << lr) < (1u << d.log2Ms), "twiddle exponent range");
+                const uint32_t lss = d.log2Ms - lr;
+                if (lss > 0) {
+                    // twiddled steps are radix 16; table slot j*Ss + n2 < Ms
+                    EXPECT(lr == 4, "twiddled step radix m=%d s=%d", m, s);
+                    EXPECT(15u * (1u << lss) + d.n2 < (1u << d.log2Ms), "step table slot");
+                }
+                for (uint32_t j = 0; j < R; ++j)
+                    EXPECT(lds_index(d.base + j * d.stride) == lds_index(d.base) + lds_index(j * d.stride),
+                           "constant LDS offsets m=%d s=%d q=%u j=%u col=%d", m, s, q, j, col);
+                // byte offsets fit the ds_read immediate field
+                EXPECT(lds_index((R - 1) * d.stride) * 8u < 65536u, "ds offset");
             }
+            EXPECT(step_ms_log2(m, s) == small_dft(m, s, 0, col != 0).log2Ms, "ms");
             for (uint32_t i = 0; i < kTile; ++i) EXPECT(seen[i] == 1, "step coverage m=%d s=%d i=%u", m, s, i);
+            // each small DFT's points are r = b*Ms + j*Ss + n2 of one DFT g
+            for (uint32_t q = 0; q < kTile / R; q += 7) {
+                const SmallDft d = small_dft(m, s, q, col != 0);
+                const uint32_t lg = kTileLog2 - m;
+                for (uint32_t j = 0; j < R; ++j) {
+                    const uint32_t idx = d.base + j * d.stride;
+                    const uint32_t g = col ? (idx & ((1u << lg) - 1u)) : (idx >> m);
+                    const uint32_t r = col ? (idx >> lg) : (idx & ((1u << m) - 1u));
+                    const uint32_t g0 = col ? (d.base & ((1u << lg) - 1u)) : (d.base >> m);
+                    const uint32_t Ss = 1u << (d.log2Ms - lr);
+                    EXPECT(g == g0, "same DFT m=%d s=%d", m, s);
+                    EXPECT((r & ((1u << d.log2Ms) - 1u)) == j * Ss + d.n2, "point order m=%d s=%d", m, s);
+                }
+            }
         }
     }
+}
+
+static void check_bitrev_split() {
+    // bitrev(r0 + (u << (lm - 4)), lm) == bitrev(r0, lm) + bitrev(u, 4)
+    for (int lm = 5; lm <= kColMaxLog2; ++lm)
+        for (uint32_t r0 = 0; r0 < (1u << (lm - 4)); ++r0)
+            for (uint32_t u = 0; u < 16; ++u)
+                EXPECT(bitrev(r0 + (u << (lm - 4)), lm) == bitrev(r0, lm) + bitrev(u, 4), "bitrev split lm=%d", lm);
 }
 
 static void check_plan(int k, bool full) {
@@ -45,6 +80,9 @@ static void check_plan(int k, bool full) {
     if (p.levels == 1) return;
     for (int l = 0; l + 1 < p.levels; ++l) EXPECT(p.log2M[l] <= kColMaxLog2 && p.log2M[l] >= 3, "col size k=%d", k);
     EXPECT(p.log2M[p.levels - 1] <= kTileLog2, "row size k=%d", k);
+    // kernel instantiations: column levels 5..9, row level 6..12
+    for (int l = 0; l + 1 < p.levels; ++l) EXPECT(p.log2M[l] >= 5 && p.log2M[l] <= 9, "col inst k=%d", k);
+    EXPECT(p.log2M[p.levels - 1] >= 6 && p.log2M[p.levels - 1] <= 12, "row inst k=%d", k);
     const uint32_t tiles = tiles_per_product(p);
     EXPECT((uint64_t)tiles * kTile == K, "tiles k=%d", k);
     for (int l = 0; l < p.levels; ++l) {
@@ -64,23 +102,47 @@ static void check_plan(int k, bool full) {
                 }
                 EXPECT(off < K, "offset k=%d l=%d t=%u i=%u", k, l, t, i);
                 if (full && off < K) seen[off]++;
+                if (col) {
+                    // thread tid = i mod 256, u = i / 256: offset and twiddle exponent
+                    const ColTile ct = col_tile(p, l, t);
+                    const uint32_t tid = i & 255u, u = i >> 8;
+                    const uint64_t want = col_thread_offset(p, l, ct, tid) + ((uint64_t)u << col_row_step_log2(p, l));
+                    EXPECT(off == want, "col thread offset k=%d l=%d", k, l);
+                    const int lk = slab_log2(p, l);
+                    const uint32_t g = i & (C - 1), r = i >> (kTileLog2 - lm);
+                    const uint64_t cg = (uint64_t)(ct.col0 + g) << (kMaxLog2 - lk);
+                    EXPECT(cg < (1ull << kMaxLog2) && cg * bitrev(r, lm) < (1ull << kMaxLog2), "twiddle exp k=%d", k);
+                } else {
+                    EXPECT(off == (uint64_t)t * kTile + i, "row offset contiguous k=%d", k);
+                }
             }
         }
         if (full)
             for (uint64_t x = 0; x < K; ++x) EXPECT(seen[x] == 1, "coverage k=%d l=%d x=%llu", k, l, (unsigned long long)x);
         if (l == 0) {
-            // final inverse level: runs of C coefficients and their spill slots
-            const uint32_t lc = kTileLog2 - lm;
-            EXPECT(final_run_log2(p) == lc, "run length k=%d", k);
+            // final inverse level: runs of adjacent coefficients (<= 16, within
+            // one tile row) and their spill slots; runs cover every tile element
+            const uint32_t rl = final_run_log2(p);
+            EXPECT(rl <= 4 && (1u << rl) <= C && (kTile >> rl) >= 256, "run length k=%d", k);
             const int ls = slab_log2(p, 0) - lm;
             for (uint32_t t = 0; t < tiles; t += step) {
                 const ColTile ct = col_tile(p, 0, t);
                 EXPECT(ct.slab == 0, "slab 0 at level 0");
-                for (uint32_t r = 0; r < (1u << lm); ++r) {
-                    const uint64_t n0 = ((uint64_t)r << ls) + ct.col0;
-                    EXPECT(n0 % C == 0 && n0 + C <= K, "run start k=%d", k);
-                    EXPECT((n0 >> lc) < (K >> lc), "spill slot k=%d", k);
+                std::vector<int> hit(kTile, 0);
+                for (uint32_t q = 0; q < (kTile >> rl); ++q) {
+                    const Run run = run_of(lm, q);
+                    const uint64_t n0 = ((uint64_t)run.r << ls) + ct.col0 + run.g0;
+                    EXPECT(run.g0 + (1u << rl) <= C && run.r < (1u << lm), "run inside row k=%d", k);
+                    EXPECT(n0 % (1u << rl) == 0 && n0 + (1u << rl) <= K, "run start k=%d", k);
+                    EXPECT((n0 >> rl) < (K >> rl), "spill slot k=%d", k);
+                    for (uint32_t j = 0; j < (1u << rl); ++j) {
+                        const uint32_t e = run.r * C + run.g0 + j;
+                        if (e < kTile) hit[e]++;
+                        EXPECT(lds_index(e) == lds_index(run.r * C + run.g0) + j, "run lds offsets k=%d", k);
+                        EXPECT(col_offset(p, 0, ct, run.g0 + j, run.r) == n0 + j, "run coefficient k=%d", k);
+                    }
                 }
+                for (uint32_t e = 0; e < kTile; ++e) EXPECT(hit[e] == 1, "run coverage k=%d", k);
             }
         }
     }
@@ -88,6 +150,7 @@ static void check_plan(int k, bool full) {
 
 int main() {
     check_tile_steps();
+    check_bitrev_split();
     for (int k = kMinLog2; k <= kMaxLog2; ++k) check_plan(k, k <= 23);
     printf(fails ? "index check: %d failures\n" : "index check: ok\n", fails);
     return fails ? 1 : 0;

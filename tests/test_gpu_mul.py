@@ -108,6 +108,32 @@ def test_broadcast_operand(ctx):
         assert np.array_equal(device_mul(ctx, A, B, b_broadcast=True), oracle_mul(A, B))
 
 
+def every_size_case(k):
+    # La + Lb in (2^(k-1), 2^k]: transform size 2^k, unequal operands
+    total = (1 << k) - (k % 3)
+    la = total // 2 + k
+    return max(1, min(64, (1 << 22) >> k)), la, total - la
+
+
+EVERY_K = list(range(4, 25))
+SLOW_K = [25, 26, 27]
+
+
+@pytest.mark.parametrize("k", EVERY_K + [pytest.param(k, marks=pytest.mark.slow) for k in SLOW_K],
+                         ids=[f"K2^{k}" for k in EVERY_K + SLOW_K])
+def test_ntt_every_transform_size(ctx, k):
+    # one case per transform size: every in-tile DFT size and level split
+    g.check(ctx.set_ntt_threshold(2))
+    try:
+        n, la, lb = every_size_case(k)
+        rng = np.random.default_rng(k)
+        A = rand_rows(rng, n, la)
+        B = rand_rows(rng, n, lb)
+        assert np.array_equal(device_mul(ctx, A, B), oracle_mul(A, B)), (n, la, lb)
+    finally:
+        g.check(ctx.set_ntt_threshold(384))
+
+
 @pytest.mark.slow
 def test_ntt_three_level(ctx):
     # K = 2^23 and 2^25: two column levels + the contiguous level
