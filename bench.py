@@ -114,6 +114,52 @@ def bench_mul(args, g) -> int:
     return 0
 
 
+def bench_leaf(args, g) -> int:
+    """The 17-term group stage of a --digits run (N groups: the reference's
+    N*17*14.3 >= digits rule, pi_bs_gpu.cu:1539) in its three forms:
+    the reference's per-op form (a synchronizing compaction after every
+    product), the batched form (lengths fixed in advance), and the batched
+    form replayed from a hipGraph.  Checks that all three leave the same
+    rows and reports ms per run and schoolbook multiply-adds per second."""
+    import numpy as np
+    N = 1
+    while N * 17 * 14.3 < args.digits:
+        N *= 2
+    ctx = g.Context()
+    forms = {"plain": g.LEAF_PLAIN, "batched": g.LEAF_BATCHED, "graph": g.LEAF_GRAPH}
+    out, rows = {}, {}
+    for name, mode in forms.items():
+        st = g.LeafStage(ctx, N, mode)
+        for _ in range(args.warmup):
+            st.run()
+        g.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = st.run()
+        g.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+        rows[name] = [g.rows_of(a) for a in res]
+        # work: sum over the stage's products of batch * L_a * L_b (32x32->64
+        # multiply-adds), from one profiled run of this form
+        ctx.stage_profile(True)
+        st.run()
+        entries = ctx.stage_entries()
+        ctx.stage_profile(False)
+        macs = sum(e["batch"] * e["lhs"] * e["rhs"] for e in entries if e["kind"] == "mul")
+        out[name] = {"ms": round(ms, 4), "macs": int(macs), "gmac_per_s": round(macs / (ms / 1e3) / 1e9, 1),
+                     "stages": len(entries)}
+        st.close()
+    same = all(all(np.array_equal(a, b) for a, b in zip(rows["plain"], rows[k])) for k in ("batched", "graph"))
+    ctx.close()
+    print(json.dumps({"metric": f"ms per 17-term group stage, {N} groups ({args.digits} digits)",
+                      "value": out["graph"]["ms"], "unit": "ms", "steps": args.steps, "warmup": args.warmup,
+                      "groups": N, "forms": out, "speedup_graph_vs_plain": round(out["plain"]["ms"] / out["graph"]["ms"], 2),
+                      "speedup_batched_vs_plain": round(out["plain"]["ms"] / out["batched"]["ms"], 2),
+                      "identical_rows": same,
+                      "lengths": {k: int(v.shape[1]) for k, v in zip("PQR", rows["plain"])}}))
+    return 0 if same else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,8 +168,9 @@ def main() -> int:
     ap.add_argument("--digits", type=int, default=1_000_000_000)
     ap.add_argument("--prec", choices=["rounded", "requested"], default="rounded")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["pi", "mul"], default="pi",
-                    help="pi: the headline run; mul: one batched product shape (kernel study)")
+    ap.add_argument("--workload", choices=["pi", "mul", "leaf"], default="pi",
+                    help="pi: the headline run; mul: one batched product shape (kernel study); "
+                         "leaf: the 17-term group stage of --digits in plain, batched and graph form")
     ap.add_argument("--la", type=int, default=1 << 27)
     ap.add_argument("--lb", type=int, default=1 << 27)
     ap.add_argument("--n", type=int, default=1)
@@ -139,6 +186,8 @@ def main() -> int:
 
     if args.workload == "mul":
         return bench_mul(args, g)
+    if args.workload == "leaf":
+        return bench_leaf(args, g)
 
     conv = g.PREC_ROUNDED if args.prec == "rounded" else g.PREC_REQUESTED
     lib = g.lib()

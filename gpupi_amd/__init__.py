@@ -72,6 +72,19 @@ class GpupiKernelStat(ctypes.Structure):
     ]
 
 
+class GpupiStageEntry(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_char * 16),
+        ("label", ctypes.c_char * 24),
+        ("n1", ctypes.c_uint32),
+        ("batch_size", ctypes.c_uint32),
+        ("lhs_length", ctypes.c_uint32),
+        ("rhs_length", ctypes.c_uint32),
+        ("elapsed_ms", ctypes.c_float),
+        ("uses_ntt", ctypes.c_int),
+    ]
+
+
 class ConvertDecimalPowerTable(ctypes.Structure):
     _fields_ = [
         ("powers", ctypes.POINTER(BatchMPArray)),
@@ -131,6 +144,11 @@ def _declare(lib):
         "batch_mul_ntt_workspace_size": (ctypes.c_size_t, [u32, u32, u32]),
         "gpupi_kernel_stats_reset": (None, [ctypes.c_int]),
         "gpupi_kernel_stats_query": (ctypes.c_int, [ctypes.POINTER(GpupiKernelStat), ctypes.c_int]),
+        "gpupi_stage_profile_enable": (err, [c_ctx, ctypes.c_int]),
+        "gpupi_leaf_stage_create": (err, [c_ctx, u32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+        "gpupi_leaf_stage_run": (err, [ctypes.c_void_p, ctypes.POINTER(A), ctypes.POINTER(A), ctypes.POINTER(A)]),
+        "gpupi_leaf_stage_destroy": (None, [ctypes.c_void_p]),
+        "gpupi_stage_profile_query": (ctypes.c_int, [c_ctx, ctypes.POINTER(GpupiStageEntry), ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -278,6 +296,56 @@ class Context:
         finally:
             self._lib.gpupi_pi_free(ctypes.byref(res))
         return ret, dec, res
+
+
+    def stage_profile(self, enable: bool) -> None:
+        """Turn per-stage timing of the pipeline on (clearing it) or off."""
+        check(self._lib.gpupi_stage_profile_enable(self.ptr, 1 if enable else 0), "gpupi_stage_profile_enable")
+
+    def stage_entries(self) -> list:
+        """[dict(kind, label, n1, batch, lhs, rhs, ms, ntt)] recorded since stage_profile(True)."""
+        n = self._lib.gpupi_stage_profile_query(self.ptr, None, 0)
+        if n < 0:
+            raise RuntimeError("gpupi_stage_profile_query failed")
+        arr = (GpupiStageEntry * max(n, 1))()
+        self._lib.gpupi_stage_profile_query(self.ptr, arr, n)
+        return [dict(kind=e.kind.decode(), label=e.label.decode(), n1=e.n1, batch=e.batch_size, lhs=e.lhs_length,
+                     rhs=e.rhs_length, ms=e.elapsed_ms, ntt=bool(e.uses_ntt)) for e in arr[:n]]
+
+
+LEAF_PLAIN, LEAF_BATCHED, LEAF_GRAPH = 0, 1, 2
+
+
+class LeafStage:
+    """The 17-term group stage for `groups` groups in one of three forms
+    (LEAF_PLAIN, LEAF_BATCHED, LEAF_GRAPH); run() returns (P, Q, R) views."""
+
+    def __init__(self, ctx: "Context", groups: int, mode: int):
+        self._lib = lib()
+        self.ptr = ctypes.c_void_p()
+        check(self._lib.gpupi_leaf_stage_create(ctx.ptr, groups, mode, ctypes.byref(self.ptr)),
+              "gpupi_leaf_stage_create")
+
+    def run(self):
+        P, Q, R = BatchMPArray(), BatchMPArray(), BatchMPArray()
+        check(self._lib.gpupi_leaf_stage_run(self.ptr, ctypes.byref(P), ctypes.byref(Q), ctypes.byref(R)),
+              "gpupi_leaf_stage_run")
+        return P, Q, R
+
+    def close(self) -> None:
+        if self.ptr:
+            self._lib.gpupi_leaf_stage_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def rows_of(arr: BatchMPArray) -> np.ndarray:
+    """Download the (batch, length) limbs of any device array view."""
+    stride = arr.stride if arr.stride else arr.length
+    total = (arr.batch_size - 1) * stride + arr.length if arr.batch_size else 0
+    full = np.zeros(arr.batch_size * max(stride, 1), dtype=np.uint32)
+    if total:
+        full[:total] = download_words(arr.data, total)
+    return full.reshape(arr.batch_size, max(stride, 1))[:, :arr.length].copy()
 
 
 def upload_rows(rows: np.ndarray, stride: Optional[int] = None, pad_value: int = 0):

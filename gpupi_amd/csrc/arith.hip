@@ -134,6 +134,7 @@ void batch_mp_destroy(BatchMPContext *ctx) {
     if (ctx->d_scalar) (void)hipFree(ctx->d_scalar);
     if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
     gpupi::ntt_tables_release(&ctx->ntt_tables);
+    stage_profiler_free(ctx->profiler);
     delete ctx;
 }
 

@@ -101,6 +101,10 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStrea
 }  // namespace gpupi
 
 // Context internals shared between arith.hip and the pipeline sources.
+// Per-stage event timings of the pi pipeline (defined with the pipeline).
+struct StageProfiler;
+void stage_profiler_free(StageProfiler *p);
+
 struct BatchMPContext {
     NTTPrecomputedTables ntt_tables;
     uint32_t *workspace;
@@ -109,4 +113,5 @@ struct BatchMPContext {
     uint32_t *h_scalar;       // pinned host mirror
     hipStream_t stream;
     int ntt_min_limbs;        // schoolbook -> NTT crossover on L_a + L_b
+    StageProfiler *profiler;  // null unless stage profiling is on
 };

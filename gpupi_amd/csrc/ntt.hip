@@ -36,6 +36,7 @@
 
 #include "common.hpp"
 #include "gpupi_pi.h"
+#include "ntt_arith.hpp"
 #include "ntt_plan.hpp"
 
 namespace gpupi {
@@ -46,8 +47,6 @@ using namespace ntt;
 constexpr uint64_t kPrime0 = 0x1ffffc0000001ull;  // 562948879679489 = 524287 * 2^30 + 1
 constexpr uint64_t kPrime1 = 0x1fffe00000001ull;  // 562941363486721 = 524280 * 2^30 + 1 (2^33 | p - 1)
 constexpr uint64_t kGen0 = 3, kGen1 = 13;          // primitive roots
-constexpr double kRoundMagic = 6755399441055744.0;  // 1.5 * 2^52
-constexpr double kP0 = 562948879679489.0, kP1 = 562941363486721.0;
 
 // Table layout (binary64) inside tables.roots, one block per prime.
 //   step tables: for Ms = 2^x (x = 5..12), at (2^x - 32): [j * Ms/16 + n2] =
@@ -70,65 +69,7 @@ struct NttConsts {
 };
 __constant__ NttConsts c_ntt;
 
-// ------------------------------------------------------- modular arithmetic
-struct Mod {
-    double p, pinv;
-};
-__device__ __forceinline__ Mod mod_of(int P) { return P ? Mod{kP1, 1.0 / kP1} : Mod{kP0, 1.0 / kP0}; }
-
-__device__ __forceinline__ double qround(double x, Mod m) {
-    return __builtin_fma(x, m.pinv, kRoundMagic) - kRoundMagic;
-}
-__device__ __forceinline__ double redp(double x, Mod m) { return __builtin_fma(-qround(x, m), m.p, x); }
-__device__ __forceinline__ double mulp(double a, double b, Mod m) {
-    const double h = a * b;
-    const double l = __builtin_fma(a, b, -h);
-    return __builtin_fma(-qround(h, m), m.p, h) + l;
-}
-
-constexpr int log2c(int x) { return x <= 1 ? 0 : 1 + log2c(x / 2); }
 constexpr uint32_t brev4(uint32_t x) { return ((x & 1u) << 3) | ((x & 2u) << 1) | ((x & 4u) >> 1) | ((x & 8u) >> 3); }
-
-// In-register radix-R DIF (natural in, bit-reversed out) and its inverse DIT.
-// Sums are re-centred only every other radix-2 stage (always in the last):
-// with inputs |v| <= 0.75p no intermediate exceeds 3p, far inside the
-// exactness limits of the modular product (|a| < 2^52) and of the sums.
-template <int R>
-__device__ __forceinline__ void dif(double *v, const double *w16, Mod m) {
-    constexpr int S = log2c(R);
-#pragma unroll
-    for (int st = 0; st < S; ++st) {
-        const int h = R >> (st + 1);
-        const bool reduce = ((S - 1 - st) & 1) == 0;  // log2(h) even
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            if (j & h) continue;
-            const int e = (j & (h - 1)) * (8 / h);
-            const double a = v[j], b = v[j + h];
-            v[j] = reduce ? redp(a + b, m) : a + b;
-            v[j + h] = (e != 0) ? mulp(a - b, w16[e], m) : (reduce ? redp(a - b, m) : a - b);
-        }
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void dit(double *v, const double *w16i, Mod m) {
-    constexpr int S = log2c(R);
-#pragma unroll
-    for (int st = 0; st < S; ++st) {
-        const int h = 1 << st;
-        const bool reduce = ((S - 1 - st) & 1) == 0;
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            if (j & h) continue;
-            const int e = (j & (h - 1)) * (8 / h);
-            const double t = (e == 0) ? v[j + h] : mulp(v[j + h], w16i[e], m);
-            const double a = v[j];
-            v[j] = reduce ? redp(a + t, m) : a + t;
-            v[j + h] = reduce ? redp(a - t, m) : a - t;
-        }
-    }
-}
 
 // In-tile steps.  Step S of the 2^M-point DFTs of a tile is 4096 / R small
 // DFTs of radix R; thread q = threadIdx.x handles 16 / R of them (q, q + 256,
@@ -157,22 +98,32 @@ struct StepMap {
     __device__ __forceinline__ uint32_t index(int k) const { return base[k / R] + (k % R) * kStride; }
 };
 
-// Butterflies (and the step's twiddles) of step S on registers.
-template <int M, int S, bool INV, bool COL>
+// Butterfly kind of forward step S (FINAL: the kind of the last step) and
+// of inverse step S.
+template <int M, int S, int FINAL>
+constexpr int fwd_kind() { return S < step_count(M) - 1 ? kDifMid : FINAL; }
+template <int M, int S>
+constexpr int inv_kind() {
+    return S == step_count(M) - 1 ? (S == 0 ? kDitOnly : kDitFirst) : (S == 0 ? kDitLast : kDitMid);
+}
+
+// Butterflies (and the step's twiddles) of step S on registers; KIND picks
+// the re-centring plan.
+template <int M, int S, bool INV, bool COL, int KIND>
 __device__ __forceinline__ void step_regs(double *v, const StepMap<M, S, COL> &sm, const double *tab, int P, Mod m) {
     using SM = StepMap<M, S, COL>;
     constexpr int R = SM::R;
     const double *w16 = INV ? c_ntt.w16i[P] : c_ntt.w16[P];
 #pragma unroll
     for (int u = 0; u < SM::PER; ++u) {
-        if (!INV) dif<R>(&v[u * R], w16, m);
+        if (!INV) dif<KIND, R>(&v[u * R], w16, m);
         if constexpr (SM::LSS > 0) {
             static_assert(R == 16, "twiddled steps are radix 16");
             const double *tw = tab + (INV ? kOffStepInv : kOffStep) + ((1u << SM::LMS) - 32u) + sm.n2[u];
 #pragma unroll
             for (int j = 1; j < 16; ++j) v[u * 16 + j] = mulp(v[u * 16 + j], tw[j << SM::LSS], m);
         }
-        if (INV) dit<R>(&v[u * R], w16, m);
+        if (INV) dit<KIND, R>(&v[u * R], w16, m);
     }
 }
 
@@ -206,7 +157,7 @@ __device__ __forceinline__ void lds_steps(double *lds, const double *tab, int P,
         const StepMap<M, S, COL> sm;
         double v[16];
         lds_load(v, lds, sm);
-        step_regs<M, S, INV, COL>(v, sm, tab, P, m);
+        step_regs<M, S, INV, COL, INV ? inv_kind<M, S>() : fwd_kind<M, S, kDifMid>()>(v, sm, tab, P, m);
         lds_store(lds, v, sm);
         __syncthreads();
         lds_steps<M, INV, COL, INV ? S - 1 : S + 1, LAST>(lds, tab, P, m);
@@ -216,17 +167,17 @@ __device__ __forceinline__ void lds_steps(double *lds, const double *tab, int P,
 // Forward transform of the tile.  On entry v holds this thread's step-0
 // elements; on exit its last-step elements (bit-reversed DFT order).  LDS is
 // used when M > 4: the caller must make sure no thread still reads `lds`.
-template <int M, bool COL>
+template <int M, bool COL, int FINAL>
 __device__ __forceinline__ void fwd_dft(double *v, double *lds, const double *tab, int P, Mod m) {
     constexpr int kLast = step_count(M) - 1;
-    step_regs<M, 0, false, COL>(v, StepMap<M, 0, COL>(), tab, P, m);
+    step_regs<M, 0, false, COL, fwd_kind<M, 0, FINAL>()>(v, StepMap<M, 0, COL>(), tab, P, m);
     if constexpr (kLast > 0) {
         lds_store(lds, v, StepMap<M, 0, COL>());
         __syncthreads();
         lds_steps<M, false, COL, 1, kLast - 1>(lds, tab, P, m);
         const StepMap<M, kLast, COL> sl;
         lds_load(v, lds, sl);
-        step_regs<M, kLast, false, COL>(v, sl, tab, P, m);
+        step_regs<M, kLast, false, COL, FINAL>(v, sl, tab, P, m);
     }
 }
 
@@ -237,14 +188,14 @@ template <int M, bool COL>
 __device__ __forceinline__ void inv_dft(double *v, double *lds, const double *tab, int P, Mod m) {
     constexpr int kLast = step_count(M) - 1;
     const StepMap<M, kLast, COL> sl;
-    step_regs<M, kLast, true, COL>(v, sl, tab, P, m);
+    step_regs<M, kLast, true, COL, inv_kind<M, kLast>()>(v, sl, tab, P, m);
     if constexpr (kLast > 0) {
         lds_store(lds, v, sl);
         __syncthreads();
         lds_steps<M, true, COL, kLast - 1, 1>(lds, tab, P, m);
         const StepMap<M, 0, COL> s0;
         lds_load(v, lds, s0);
-        step_regs<M, 0, true, COL>(v, s0, tab, P, m);
+        step_regs<M, 0, true, COL, kDitLast>(v, s0, tab, P, m);
     }
 }
 
@@ -333,9 +284,10 @@ template <bool FROM_LIMBS, int M>
 __global__ void __launch_bounds__(256) ntt_col_fwd_kernel(PassArgs a, int level) {
     __shared__ double lds[kLdsSlots];
     const uint32_t tiles = tiles_per_product(a.plan);
-    const uint32_t t = blockIdx.x % tiles;
-    const uint32_t op = (blockIdx.x / tiles) & 1u;
-    const uint32_t prod = blockIdx.x / (2u * tiles);
+    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t t = bid % tiles;
+    const uint32_t op = (bid / tiles) & 1u;
+    const uint32_t prod = bid / (2u * tiles);
     const uint64_t K = 1ull << a.plan.log2K;
     const ColThread<M> ct(a.plan, level, t);
     uint32_t xin[16];
@@ -356,7 +308,7 @@ __global__ void __launch_bounds__(256) ntt_col_fwd_kernel(PassArgs a, int level)
         double v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) v[u] = FROM_LIMBS ? (double)xin[u] : T[ct.off(u)];
-        fwd_dft<M, true>(v, lds, tab, P, m);
+        fwd_dft<M, true, kDifToTwiddle>(v, lds, tab, P, m);
         lds_store(lds, v, StepMap<M, step_count(M) - 1, true>());
         __syncthreads();
 #pragma unroll
@@ -382,8 +334,9 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     __shared__ double lds[kLdsSlots];
     constexpr uint32_t kJ = 1u << (M - 4);
     const uint32_t tiles = tiles_per_product(a.plan);
-    const uint32_t t = blockIdx.x % tiles;
-    const uint32_t prod = blockIdx.x / tiles;
+    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t t = bid % tiles;
+    const uint32_t prod = bid / tiles;
     const uint64_t K = 1ull << a.plan.log2K;
     const size_t g0 = (size_t)prod * K + (size_t)t * kTile + row_step0_base<M>();
 #pragma clang loop unroll(disable)
@@ -395,11 +348,11 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
         double ra[16], v[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) ra[j] = TA[j * kJ];
-        fwd_dft<M, false>(ra, lds, tab, P, m);
+        fwd_dft<M, false, kDifToPointA>(ra, lds, tab, P, m);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < 16; ++j) v[j] = TB[j * kJ];
-        fwd_dft<M, false>(v, lds, tab, P, m);
+        fwd_dft<M, false, kDifToPointB>(v, lds, tab, P, m);
 #pragma unroll
         for (int j = 0; j < 16; ++j) v[j] = mulp(v[j], ra[j], m);
         inv_dft<M, false>(v, lds, tab, P, m);
@@ -427,7 +380,7 @@ __device__ __forceinline__ void col_inv_prime(double *v, const PassArgs &a, doub
     lds_steps<M, true, true, kLast, 1>(lds, tab, P, m);
     const StepMap<M, 0, true> s0;
     lds_load(v, lds, s0);
-    step_regs<M, 0, true, true>(v, s0, tab, P, m);
+    step_regs<M, 0, true, true, kDitLast>(v, s0, tab, P, m);
 }
 
 // Inverse column level above the first: both primes through one tile.
@@ -435,8 +388,9 @@ template <int M>
 __global__ void __launch_bounds__(256) ntt_col_inv_kernel(PassArgs a, int level) {
     __shared__ double lds[kLdsSlots];
     const uint32_t tiles = tiles_per_product(a.plan);
-    const uint32_t t = blockIdx.x % tiles;
-    const uint32_t prod = blockIdx.x / tiles;
+    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t t = bid % tiles;
+    const uint32_t prod = bid / tiles;
     const ColThread<M> ct(a.plan, level, t);
 #pragma clang loop unroll(disable)
     for (int P = 0; P < 2; ++P) {
@@ -459,8 +413,9 @@ __global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
     constexpr int RL = run_log2_for(M);
     constexpr uint32_t kRunLen = 1u << RL;
     const uint32_t tiles = tiles_per_product(a.plan);
-    const uint32_t t = blockIdx.x % tiles;
-    const uint32_t prod = blockIdx.x / tiles;
+    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t t = bid % tiles;
+    const uint32_t prod = bid / tiles;
     const int log2K = a.plan.log2K;
     const ColThread<M> ct(a.plan, 0, t);
     {
@@ -521,9 +476,9 @@ __device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, doub
         ra[j] = (prod < a.N && r < a.La) ? (double)a.A[(size_t)prod * a.sA + r] : 0.0;
         v[j] = (prod < a.N && r < a.Lb) ? (double)a.B[(size_t)prod * a.sB + r] : 0.0;
     }
-    fwd_dft<M, false>(ra, lds, tab, P, m);
+    fwd_dft<M, false, kDifToPointA>(ra, lds, tab, P, m);
     if constexpr (M > 4) __syncthreads();
-    fwd_dft<M, false>(v, lds, tab, P, m);
+    fwd_dft<M, false, kDifToPointB>(v, lds, tab, P, m);
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = mulp(v[j], ra[j], m);
     inv_dft<M, false>(v, lds, tab, P, m);
@@ -539,7 +494,7 @@ __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
     __shared__ double l1[kLdsSlots];
     __shared__ uint64_t spill_s[256];
     __shared__ uint32_t wave_flags[4];
-    const uint32_t prod0 = blockIdx.x * (kTile >> M);
+    const uint32_t prod0 = xcd_tile(blockIdx.x, gridDim.x) * (kTile >> M);
     // prime 0 transforms in l1 and leaves its result in l0; prime 1 the reverse
     local_prime<M>(a, l1, l0, prod0, 0);
     __syncthreads();

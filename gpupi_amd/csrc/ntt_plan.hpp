@@ -184,6 +184,17 @@ GPUPI_HD SmallDft small_dft(int m, int s, uint32_t q, bool col) {
     return d;
 }
 
+// Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin:
+// block b runs on XCD b mod 8.  xcd_tile(b, n) renumbers blocks so that
+// each XCD gets one contiguous range of tiles, and neighbouring tiles (which
+// share 128-byte lines of limbs) meet in one L2.  A bijection on [0, n).
+constexpr uint32_t kXcds = 8;
+GPUPI_HD uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    const uint32_t x = b % kXcds, i = b / kXcds;
+    const uint32_t q = n / kXcds, r = n % kXcds;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // Run length of the final inverse column level (limbs per run): the C
 // adjacent coefficients of one tile row, cut into runs of at most 16 so
 // that every thread of the block owns at least one run.

@@ -11,7 +11,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "common.hpp"
@@ -107,7 +109,8 @@ __global__ void __launch_bounds__(256) leaf17_kernel(int is_head, uint32_t *P, u
 // A grow-only device buffer holding one batch array.
 struct Buf {
     uint32_t *ptr = nullptr;
-    size_t cap = 0;  // words
+    size_t cap = 0;       // words
+    bool frozen = false;  // set while a graph is being captured: no reallocation
     Buf() = default;
     Buf(const Buf &) = delete;
     Buf &operator=(const Buf &) = delete;
@@ -116,6 +119,7 @@ struct Buf {
     }
     hipError_t reserve(size_t words) {
         if (words <= cap) return hipSuccess;
+        if (frozen) return hipErrorInvalidValue;
         if (ptr) {
             GPUPI_RETURN_IF(hipFree(ptr));
             ptr = nullptr;
@@ -163,7 +167,108 @@ BatchMPArray make_subview(const BatchMPArray &a, uint32_t start, uint32_t length
     return view_of(a.data + start, length, a.batch_size, a.stride >= start ? a.stride - start : 0u);
 }
 
-hipError_t compact(BatchMPContext *ctx, BatchMPArray &a) { return batch_mp_array_compact(ctx, &a); }
+}  // namespace
+
+// ------------------------------------------------------- stage profiling
+// pi_bs_gpu.cu:133-242: with profiling on, each stage is bracketed by events
+// on the context stream and synchronized; one entry per stage.
+struct StageProfiler {
+    bool enabled = false;
+    std::vector<GpupiStageEntry> entries;
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+void stage_profiler_free(StageProfiler *p) {
+    if (p == nullptr) return;
+    if (p->start) (void)hipEventDestroy(p->start);
+    if (p->stop) (void)hipEventDestroy(p->stop);
+    delete p;
+}
+
+namespace {
+
+template <typename F>
+hipError_t stage(BatchMPContext *ctx, const char *kind, const char *label, uint32_t n1, uint32_t batch, uint32_t lhs,
+                 uint32_t rhs, bool uses_ntt, F &&fn) {
+    StageProfiler *p = ctx->profiler;
+    if (p == nullptr || !p->enabled) return fn();
+    GPUPI_RETURN_IF(hipEventRecord(p->start, ctx->stream));
+    GPUPI_RETURN_IF(fn());
+    GPUPI_RETURN_IF(hipEventRecord(p->stop, ctx->stream));
+    GPUPI_RETURN_IF(hipEventSynchronize(p->stop));
+    GpupiStageEntry e{};
+    std::snprintf(e.kind, sizeof(e.kind), "%s", kind);
+    std::snprintf(e.label, sizeof(e.label), "%s", label);
+    e.n1 = n1;
+    e.batch_size = batch;
+    e.lhs_length = lhs;
+    e.rhs_length = rhs;
+    e.uses_ntt = uses_ntt ? 1 : 0;
+    GPUPI_RETURN_IF(hipEventElapsedTime(&e.elapsed_ms, p->start, p->stop));
+    p->entries.push_back(e);
+    return hipSuccess;
+}
+
+// whether batch_mp_mul takes the NTT path for these lengths
+bool mul_uses_ntt(const BatchMPContext *ctx, const BatchMPArray &A, const BatchMPArray &B) {
+    if (A.length == 0 || B.length == 0) return false;
+    const bool small = (int64_t)A.length + B.length < ctx->ntt_min_limbs && gpupi::mul_naive_supports(A.length, B.length);
+    return !small && gpupi::ntt_supports(A.length, B.length);
+}
+
+hipError_t p_mul(BatchMPContext *ctx, const char *label, uint32_t n1, const BatchMPArray &A, const BatchMPArray &B,
+                 const BatchMPArray &C) {
+    return stage(ctx, "mul", label, n1, A.batch_size, A.length, B.length, mul_uses_ntt(ctx, A, B),
+                 [&] { return batch_mp_mul(ctx, A, B, C); });
+}
+
+hipError_t compact(BatchMPContext *ctx, BatchMPArray &a, const char *label = "", uint32_t n1 = 0) {
+    return stage(ctx, "compact", label, n1, a.batch_size, a.length, 0, false,
+                 [&] { return batch_mp_array_compact(ctx, &a); });
+}
+
+hipError_t p_shift_addsub(BatchMPContext *ctx, const BatchMPArray &A, const BatchMPArray &B, uint32_t shift,
+                          bool subtract, uint32_t n1, const BatchMPArray &C) {
+    return stage(ctx, subtract ? "sub" : "add", subtract ? "R*Q<<15n-P*R" : "R*Q<<15n+P*R", n1, A.batch_size,
+                 A.length, B.length, false, [&] {
+                     return subtract ? batch_mp_shift_sub(ctx, A, B, shift, C) : batch_mp_shift_add(ctx, A, B, shift, C);
+                 });
+}
+
+hipError_t p_exactdiv(BatchMPContext *ctx, const char *label, uint32_t n1, const BatchMPArray &A, uint32_t d) {
+    return stage(ctx, "exactdiv", label, n1, A.batch_size, A.length, 0, false,
+                 [&] { return batch_mp_exactdiv_small(ctx, A, d); });
+}
+
+hipError_t p_mul_small(BatchMPContext *ctx, const char *label, uint32_t n1, const BatchMPArray &A, uint32_t b,
+                       const BatchMPArray &C) {
+    return stage(ctx, "mul_small", label, n1, A.batch_size, A.length, 1, false,
+                 [&] { return batch_mp_mul_small(ctx, A, b, C); });
+}
+
+hipError_t p_add(BatchMPContext *ctx, const char *label, uint32_t n1, const BatchMPArray &A, const BatchMPArray &B,
+                 const BatchMPArray &C) {
+    return stage(ctx, "add", label, n1, A.batch_size, A.length, B.length, false,
+                 [&] { return batch_mp_add(ctx, A, B, C); });
+}
+
+hipError_t p_sub(BatchMPContext *ctx, const char *label, uint32_t n1, const BatchMPArray &A, const BatchMPArray &B,
+                 const BatchMPArray &C) {
+    return stage(ctx, "sub", label, n1, A.batch_size, A.length, B.length, false,
+                 [&] { return batch_mp_sub(ctx, A, B, C); });
+}
+
+hipError_t p_add_small(BatchMPContext *ctx, const char *label, uint32_t n1, const BatchMPArray &A, uint32_t b,
+                       const BatchMPArray &C) {
+    return stage(ctx, "add", label, n1, A.batch_size, A.length, 1, false,
+                 [&] { return batch_mp_add_small(ctx, A, b, C); });
+}
+
+hipError_t p_shift_bits(BatchMPContext *ctx, const char *label, uint32_t n1, const BatchMPArray &A,
+                        const BatchMPArray &B, int32_t shift) {
+    return stage(ctx, "shift", label, n1, A.batch_size, A.length, B.length, false,
+                 [&] { return batch_mp_shift_bits(ctx, A, B, shift); });
+}
 
 // Host-side guard: a batch-1 view must lie inside [base, base + words).
 bool inside(const BatchMPArray &v, const uint32_t *base, size_t words) {
@@ -172,13 +277,14 @@ bool inside(const BatchMPArray &v, const uint32_t *base, size_t words) {
 }
 
 // pi_bs_gpu.cu:1068-1093
-hipError_t shift_right_large(BatchMPContext *ctx, const BatchMPArray &A, uint64_t shift_bits, BatchMPArray B) {
+hipError_t shift_right_large(BatchMPContext *ctx, const BatchMPArray &A, uint64_t shift_bits, BatchMPArray B,
+                             const char *label, uint32_t n1) {
     const uint64_t limb_shift = shift_bits / 32u;
     const uint32_t bit_shift = (uint32_t)(shift_bits % 32u);
     if (limb_shift >= A.length) return hipErrorInvalidValue;
     const BatchMPArray pre = view_of(A.data + limb_shift, A.length - (uint32_t)limb_shift, A.batch_size,
                                      A.stride - (uint32_t)limb_shift);
-    return batch_mp_shift_bits(ctx, pre, B, -(int32_t)bit_shift);
+    return p_shift_bits(ctx, label, n1, pre, B, -(int32_t)bit_shift);
 }
 
 hipError_t launch_leaf(BatchMPContext *ctx, int is_head, uint32_t *P, uint32_t *Q, uint32_t *R, uint32_t count) {
@@ -297,6 +403,196 @@ hipError_t upload(BatchMPContext *ctx, const Limbs &v, BatchMPArray &a) {
 }  // namespace
 
 // ====================================================================== API
+namespace {
+
+// ------------------------------------------------- the binary-split tree
+struct TreeBufs {
+    Buf bP0, bQ0, bR0, bP, bQ, bR, bPn, bQn, bRn, bR1, bR2;
+    void freeze(bool f) {
+        for (Buf *b : {&bP0, &bQ0, &bR0, &bP, &bQ, &bR, &bPn, &bQn, &bRn, &bR1, &bR2}) b->frozen = f;
+    }
+};
+
+// One pairwise level n1 of the tree (pi_bs_gpu.cu:625-723) over the rows of
+// P, Q, R; with compacting = false the output lengths are the allocation
+// bounds and nothing synchronizes (the values are the same integers).
+hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1, bool compacting, BatchMPArray &P,
+                      BatchMPArray &Q, BatchMPArray &R) {
+    const uint32_t n = N * 16u;
+    const bool subtract = (n1 == n) || (n1 == (n >> 4));
+    const uint32_t ignore_last = (n1 <= N) ? 1u : 0u;
+    const uint32_t half = n1 / 2;
+    BatchMPArray Pn, Qn, Rn, R1, R2;
+    // P: the last group's P is never used once rows are whole groups
+    GPUPI_RETURN_IF(t.bPn.view(half - ignore_last, P.length * 2, Pn));
+    BatchMPArray Pe = view_of(P.data, P.length, half - ignore_last, P.stride * 2);
+    BatchMPArray Po = view_of(P.data + P.stride, P.length, half - ignore_last, P.stride * 2);
+    if (Po.batch_size > 0) {
+        GPUPI_RETURN_IF(p_mul(ctx, "P", n1, Pe, Po, Pn));
+        if (compacting) GPUPI_RETURN_IF(compact(ctx, Pn, "P", n1));
+    }
+    GPUPI_RETURN_IF(t.bQn.view(half, Q.length * 2, Qn));
+    const BatchMPArray Qe = view_of(Q.data, Q.length, half, Q.stride * 2);
+    const BatchMPArray Qo = view_of(Q.data + Q.stride, Q.length, half, Q.stride * 2);
+    GPUPI_RETURN_IF(p_mul(ctx, "Q", n1, Qe, Qo, Qn));
+    if (compacting) GPUPI_RETURN_IF(compact(ctx, Qn, "Q", n1));
+    const BatchMPArray Re = view_of(R.data, R.length, half, R.stride * 2);
+    const BatchMPArray Ro = view_of(R.data + R.stride, R.length, half, R.stride * 2);
+    GPUPI_RETURN_IF(t.bR1.view(half, R.length + Q.length, R1));
+    GPUPI_RETURN_IF(p_mul(ctx, "R*Q", n1, Re, Qo, R1));
+    // R(a,c) = R(a,b) Q(b,c) 2^(15 terms(b,c)) +- P(a,b) R(b,c)
+    const uint32_t terms_right = (n / n1) + ((n / n1) >> 4);
+    const uint32_t shift = 15u * terms_right;
+    Pe.batch_size = half;
+    GPUPI_RETURN_IF(t.bR2.view(half, R.length + P.length, R2));
+    GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, Pe, Ro, R2));
+    const uint32_t shifted_len = R1.length + (shift >> 5) + ((shift & 31u) != 0);
+    const uint32_t rlen = std::max(shifted_len, R2.length) + (subtract ? 0u : 1u);
+    GPUPI_RETURN_IF(t.bRn.view(half, rlen, Rn));
+    GPUPI_RETURN_IF(p_shift_addsub(ctx, R1, R2, shift, subtract, n1, Rn));
+    if (compacting) GPUPI_RETURN_IF(compact(ctx, Rn, "R", n1));
+    t.bP.swap(t.bPn);
+    t.bQ.swap(t.bQn);
+    t.bR.swap(t.bRn);
+    P = Pn;
+    Q = Qn;
+    R = Rn;
+    return hipSuccess;
+}
+
+// The head term 17g+16 of each group appended to its 16-term body, then the
+// block's common factor divided out (pi_bs_gpu.cu:725-779).
+hipError_t head_merge(BatchMPContext *ctx, TreeBufs &t, uint32_t n1, bool compacting, const BatchMPArray &P0,
+                      const BatchMPArray &Q0, const BatchMPArray &R0, BatchMPArray &P, BatchMPArray &Q,
+                      BatchMPArray &R) {
+    const uint32_t div_lo = 830297u, div_hi = 156590819u;  // 13^2 17^3, 7^6 11^3 (pi_bs_gpu.cu:151-152)
+    const uint32_t half = P.batch_size;
+    BatchMPArray Ph, Qh, Rh, Rp1, Rp2;
+    GPUPI_RETURN_IF(t.bPn.view(half, P.length + P0.length, Ph));
+    GPUPI_RETURN_IF(p_mul(ctx, "P", n1, P, P0, Ph));
+    if (compacting) GPUPI_RETURN_IF(compact(ctx, Ph, "P", n1));
+    GPUPI_RETURN_IF(t.bQn.view(half, Q.length + Q0.length, Qh));
+    GPUPI_RETURN_IF(p_mul(ctx, "Q", n1, Q, Q0, Qh));
+    if (compacting) GPUPI_RETURN_IF(compact(ctx, Qh, "Q", n1));
+    GPUPI_RETURN_IF(t.bR1.view(half, R.length + Q0.length, Rp1));
+    GPUPI_RETURN_IF(p_mul(ctx, "R*Q", n1, R, Q0, Rp1));
+    GPUPI_RETURN_IF(t.bR2.view(half, P.length + R0.length, Rp2));
+    GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, P, R0, Rp2));
+    const uint32_t hl = std::max(Rp1.length + 1, Rp2.length) + 1;
+    GPUPI_RETURN_IF(t.bRn.view(half, hl, Rh));
+    GPUPI_RETURN_IF(p_shift_addsub(ctx, Rp1, Rp2, 15u, false, n1, Rh));
+    if (compacting) GPUPI_RETURN_IF(compact(ctx, Rh, "R", n1));
+    GPUPI_RETURN_IF(p_exactdiv(ctx, "P", n1, Ph, div_lo));
+    GPUPI_RETURN_IF(p_exactdiv(ctx, "P", n1, Ph, div_hi));
+    if (compacting) GPUPI_RETURN_IF(compact(ctx, Ph, "P", n1));
+    GPUPI_RETURN_IF(p_exactdiv(ctx, "Q", n1, Qh, div_lo));
+    GPUPI_RETURN_IF(p_exactdiv(ctx, "Q", n1, Qh, div_hi));
+    if (compacting) GPUPI_RETURN_IF(compact(ctx, Qh, "Q", n1));
+    GPUPI_RETURN_IF(p_exactdiv(ctx, "R", n1, Rh, div_lo));
+    GPUPI_RETURN_IF(p_exactdiv(ctx, "R", n1, Rh, div_hi));
+    if (compacting) GPUPI_RETURN_IF(compact(ctx, Rh, "R", n1));
+    t.bP.swap(t.bPn);
+    t.bQ.swap(t.bQn);
+    t.bR.swap(t.bRn);
+    P = Ph;
+    Q = Qh;
+    R = Rh;
+    return hipSuccess;
+}
+
+// The 17-term group stage (pi_bs_gpu.cu:555-779): leaf terms, the four
+// pairwise levels of each 16-term body and the head merge; one row of P, Q,
+// R per group.  compacting = true is the reference's per-op form (a
+// synchronizing compaction after each product); false keeps the allocation
+// bounds as lengths (at most 51, 68 and 83 limbs at the end) and launches
+// everything without a host round trip.
+hipError_t leaf_stage(BatchMPContext *ctx, TreeBufs &t, uint32_t N, bool compacting, BatchMPArray &P,
+                      BatchMPArray &Q, BatchMPArray &R) {
+    const uint32_t n = N * 16u;
+    BatchMPArray P0, Q0, R0;
+    GPUPI_RETURN_IF(t.bP0.view(N, 3, P0));
+    GPUPI_RETURN_IF(t.bQ0.view(N, 4, Q0));
+    GPUPI_RETURN_IF(t.bR0.view(N, 6, R0));
+    GPUPI_RETURN_IF(t.bP.view(n, 3, P));
+    GPUPI_RETURN_IF(t.bQ.view(n, 4, Q));
+    GPUPI_RETURN_IF(t.bR.view(n, 6, R));
+    // capacities as the reference pre-allocates (pi_bs_gpu.cu:611-617)
+    GPUPI_RETURN_IF(t.bPn.reserve((size_t)(n / 2) * 6));
+    GPUPI_RETURN_IF(t.bQn.reserve((size_t)(n / 2) * 10));
+    GPUPI_RETURN_IF(t.bRn.reserve((size_t)(n / 2) * 12));
+    GPUPI_RETURN_IF(t.bR1.reserve((size_t)(n / 2) * 11));
+    GPUPI_RETURN_IF(t.bR2.reserve((size_t)(n / 2) * 9));
+    GPUPI_RETURN_IF(stage(ctx, "leaf", "construct_leaf", N * 17u, N, P0.length, 0, false,
+                          [&] { return launch_leaf(ctx, 1, P0.data, Q0.data, R0.data, N); }));
+    GPUPI_RETURN_IF(stage(ctx, "leaf", "construct_leaf", N * 17u, n, P.length, 0, false,
+                          [&] { return launch_leaf(ctx, 0, P.data, Q.data, R.data, n); }));
+    for (uint32_t n1 = n; n1 >= 2 * N; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, compacting, P, Q, R));
+    return head_merge(ctx, t, 2 * N, compacting, P0, Q0, R0, P, Q, R);
+}
+
+}  // namespace
+
+// ------------------------------------------------ 17-term group stage
+// Stage object for the three forms of the group stage.
+struct GpupiLeafStage {
+    BatchMPContext *ctx = nullptr;
+    uint32_t N = 0;
+    int mode = GPUPI_LEAF_PLAIN;
+    TreeBufs t;
+    BatchMPArray P{}, Q{}, R{};  // GRAPH: the captured outputs (bound lengths)
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t *ws = nullptr;  // GRAPH: scratch the captured carry passes use
+    size_t ws_bytes = 0;
+};
+
+namespace {
+
+// Runs `fn` with the context's stream and workspace replaced by the stage's
+// own, so that what a graph captures never points at memory the context may
+// later reallocate.
+template <typename F>
+hipError_t with_stage_context(GpupiLeafStage *s, F &&fn) {
+    BatchMPContext *ctx = s->ctx;
+    const hipStream_t old_stream = ctx->stream;
+    uint32_t *const old_ws = ctx->workspace;
+    const size_t old_bytes = ctx->workspace_size_bytes;
+    ctx->stream = s->stream;
+    ctx->workspace = s->ws;
+    ctx->workspace_size_bytes = s->ws_bytes;
+    const hipError_t err = fn();
+    s->ws = ctx->workspace;
+    s->ws_bytes = ctx->workspace_size_bytes;
+    ctx->stream = old_stream;
+    ctx->workspace = old_ws;
+    ctx->workspace_size_bytes = old_bytes;
+    return err;
+}
+
+hipError_t leaf_stage_capture(GpupiLeafStage *s) {
+    BatchMPArray P, Q, R;
+    // two eager runs: every ping-pong buffer reaches the size of both roles
+    for (int i = 0; i < 2; ++i) GPUPI_RETURN_IF(leaf_stage(s->ctx, s->t, s->N, false, P, Q, R));
+    GPUPI_RETURN_IF(hipStreamSynchronize(s->stream));
+    s->t.freeze(true);
+    GPUPI_RETURN_IF(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t body = leaf_stage(s->ctx, s->t, s->N, false, P, Q, R);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(s->stream, &graph);
+    s->t.freeze(false);
+    s->graph = graph;
+    GPUPI_RETURN_IF(body);
+    GPUPI_RETURN_IF(end);
+    GPUPI_RETURN_IF(hipGraphInstantiate(&s->exec, graph, nullptr, nullptr, 0));
+    s->P = P;
+    s->Q = Q;
+    s->R = R;
+    return hipStreamSynchronize(s->stream);
+}
+
+}  // namespace
+
 extern "C" {
 
 hipError_t gpupi_construct_leaf_17(BatchMPContext *ctx, int is_head, uint32_t *P, uint32_t *Q, uint32_t *R,
@@ -308,103 +604,18 @@ hipError_t gpupi_construct_leaf_17(BatchMPContext *ctx, int is_head, uint32_t *P
 // pi_bs_gpu.cu:503-792
 hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t N, BatchMPArray *Q_out, BatchMPArray *R_out) {
     if (ctx == nullptr || Q_out == nullptr || R_out == nullptr || N == 0 || (N & (N - 1)) != 0) return hipErrorInvalidValue;
-    const uint32_t n = N * 16u;
-    const uint32_t div_lo = 830297u, div_hi = 156590819u;  // 13^2 17^3, 7^6 11^3 (pi_bs_gpu.cu:151-152)
-    Buf bP0, bQ0, bR0, bP, bQ, bR, bPn, bQn, bRn, bR1, bR2;
-    BatchMPArray P0, Q0, R0, P, Q, R;
-    GPUPI_RETURN_IF(bP0.view(N, 3, P0));
-    GPUPI_RETURN_IF(bQ0.view(N, 4, Q0));
-    GPUPI_RETURN_IF(bR0.view(N, 6, R0));
-    GPUPI_RETURN_IF(bP.view(n, 3, P));
-    GPUPI_RETURN_IF(bQ.view(n, 4, Q));
-    GPUPI_RETURN_IF(bR.view(n, 6, R));
-    // capacities as the reference pre-allocates (pi_bs_gpu.cu:611-617)
-    GPUPI_RETURN_IF(bPn.reserve((size_t)(n / 2) * 6));
-    GPUPI_RETURN_IF(bQn.reserve((size_t)(n / 2) * 10));
-    GPUPI_RETURN_IF(bRn.reserve((size_t)(n / 2) * 12));
-    GPUPI_RETURN_IF(bR1.reserve((size_t)(n / 2) * 11));
-    GPUPI_RETURN_IF(bR2.reserve((size_t)(n / 2) * 9));
-    GPUPI_RETURN_IF(launch_leaf(ctx, 1, P0.data, Q0.data, R0.data, N));
-    GPUPI_RETURN_IF(launch_leaf(ctx, 0, P.data, Q.data, R.data, n));
-
-    for (uint32_t n1 = n; n1 > 1; n1 >>= 1) {
-        const bool subtract = (n1 == n) || (n1 == (n >> 4));
-        const uint32_t ignore_last = (n1 <= N) ? 1u : 0u;
-        const uint32_t half = n1 / 2;
-        BatchMPArray Pn, Qn, Rn, R1, R2;
-        // P: the last group's P is never used once rows are whole groups
-        GPUPI_RETURN_IF(bPn.view(half - ignore_last, P.length * 2, Pn));
-        BatchMPArray Pe = view_of(P.data, P.length, half - ignore_last, P.stride * 2);
-        BatchMPArray Po = view_of(P.data + P.stride, P.length, half - ignore_last, P.stride * 2);
-        if (Po.batch_size > 0) {
-            GPUPI_RETURN_IF(batch_mp_mul(ctx, Pe, Po, Pn));
-            GPUPI_RETURN_IF(compact(ctx, Pn));
-        }
-        GPUPI_RETURN_IF(bQn.view(half, Q.length * 2, Qn));
-        const BatchMPArray Qe = view_of(Q.data, Q.length, half, Q.stride * 2);
-        const BatchMPArray Qo = view_of(Q.data + Q.stride, Q.length, half, Q.stride * 2);
-        GPUPI_RETURN_IF(batch_mp_mul(ctx, Qe, Qo, Qn));
-        GPUPI_RETURN_IF(compact(ctx, Qn));
-        const BatchMPArray Re = view_of(R.data, R.length, half, R.stride * 2);
-        const BatchMPArray Ro = view_of(R.data + R.stride, R.length, half, R.stride * 2);
-        GPUPI_RETURN_IF(bR1.view(half, R.length + Q.length, R1));
-        GPUPI_RETURN_IF(batch_mp_mul(ctx, Re, Qo, R1));
-        // R(a,c) = R(a,b) Q(b,c) 2^(15 terms(b,c)) +- P(a,b) R(b,c)
-        const uint32_t terms_right = (n / n1) + ((n / n1) >> 4);
-        const uint32_t shift = 15u * terms_right;
-        Pe.batch_size = half;
-        GPUPI_RETURN_IF(bR2.view(half, R.length + P.length, R2));
-        GPUPI_RETURN_IF(batch_mp_mul(ctx, Pe, Ro, R2));
-        const uint32_t shifted_len = R1.length + (shift >> 5) + ((shift & 31u) != 0);
-        const uint32_t rlen = std::max(shifted_len, R2.length) + (subtract ? 0u : 1u);
-        GPUPI_RETURN_IF(bRn.view(half, rlen, Rn));
-        GPUPI_RETURN_IF(subtract ? batch_mp_shift_sub(ctx, R1, R2, shift, Rn) : batch_mp_shift_add(ctx, R1, R2, shift, Rn));
-        GPUPI_RETURN_IF(compact(ctx, Rn));
-        bP.swap(bPn);
-        bQ.swap(bQn);
-        bR.swap(bRn);
-        P = Pn;
-        Q = Qn;
-        R = Rn;
-
-        if (n1 == (n >> 3)) {
-            // append the head term 17g+16 of each group, then divide out the
-            // block's common factor (pi_bs_gpu.cu:725-779)
-            BatchMPArray Ph, Qh, Rh, Rp1, Rp2;
-            GPUPI_RETURN_IF(bPn.view(half, P.length + P0.length, Ph));
-            GPUPI_RETURN_IF(batch_mp_mul(ctx, P, P0, Ph));
-            GPUPI_RETURN_IF(compact(ctx, Ph));
-            GPUPI_RETURN_IF(bQn.view(half, Q.length + Q0.length, Qh));
-            GPUPI_RETURN_IF(batch_mp_mul(ctx, Q, Q0, Qh));
-            GPUPI_RETURN_IF(compact(ctx, Qh));
-            GPUPI_RETURN_IF(bR1.view(half, R.length + Q0.length, Rp1));
-            GPUPI_RETURN_IF(batch_mp_mul(ctx, R, Q0, Rp1));
-            GPUPI_RETURN_IF(bR2.view(half, P.length + R0.length, Rp2));
-            GPUPI_RETURN_IF(batch_mp_mul(ctx, P, R0, Rp2));
-            const uint32_t hl = std::max(Rp1.length + 1, Rp2.length) + 1;
-            GPUPI_RETURN_IF(bRn.view(half, hl, Rh));
-            GPUPI_RETURN_IF(batch_mp_shift_add(ctx, Rp1, Rp2, 15u, Rh));
-            GPUPI_RETURN_IF(compact(ctx, Rh));
-            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Ph, div_lo));
-            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Ph, div_hi));
-            GPUPI_RETURN_IF(compact(ctx, Ph));
-            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Qh, div_lo));
-            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Qh, div_hi));
-            GPUPI_RETURN_IF(compact(ctx, Qh));
-            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Rh, div_lo));
-            GPUPI_RETURN_IF(batch_mp_exactdiv_small(ctx, Rh, div_hi));
-            GPUPI_RETURN_IF(compact(ctx, Rh));
-            bP.swap(bPn);
-            bQ.swap(bQn);
-            bR.swap(bRn);
-            P = Ph;
-            Q = Qh;
-            R = Rh;
-        }
-    }
+    TreeBufs t;
+    BatchMPArray P, Q, R;
+    // the 17-term group stage with bound lengths (no synchronization inside),
+    // then one compaction of each array
+    GPUPI_RETURN_IF(leaf_stage(ctx, t, N, false, P, Q, R));
+    GPUPI_RETURN_IF(compact(ctx, P, "P", 2 * N));
+    GPUPI_RETURN_IF(compact(ctx, Q, "Q", 2 * N));
+    GPUPI_RETURN_IF(compact(ctx, R, "R", 2 * N));
+    for (uint32_t n1 = N; n1 > 1; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, true, P, Q, R));
     // Q and R now hold one row each; hand their buffers to the caller.
-    *Q_out = view_of(bQ.release(), Q.length, 1, Q.stride);
-    *R_out = view_of(bR.release(), R.length, 1, R.stride);
+    *Q_out = view_of(t.bQ.release(), Q.length, 1, Q.stride);
+    *R_out = view_of(t.bR.release(), R.length, 1, R.stride);
     return hipSuccess;
 }
 
@@ -422,25 +633,26 @@ hipError_t gpupi_sqrt_10005(BatchMPContext *ctx, int target_digits, BatchMPArray
     double valid_digits = 7.5;
     while (valid_digits < target_digits) {
         valid_digits *= 2;
+        const uint32_t sd = (uint32_t)valid_digits;
         BatchMPArray PQ, Qn, PP, QQ, QQ5, Pn;
         GPUPI_RETURN_IF(bPQ.view(1, P.length + Q.length, PQ));
-        GPUPI_RETURN_IF(batch_mp_mul(ctx, P, Q, PQ));
-        GPUPI_RETURN_IF(compact(ctx, PQ));
+        GPUPI_RETURN_IF(p_mul(ctx, "P*Q", sd, P, Q, PQ));
+        GPUPI_RETURN_IF(compact(ctx, PQ, "P*Q", sd));
         GPUPI_RETURN_IF(bQn.view(1, PQ.length + 1, Qn));
-        GPUPI_RETURN_IF(batch_mp_shift_bits(ctx, PQ, Qn, 1));
-        GPUPI_RETURN_IF(compact(ctx, Qn));
+        GPUPI_RETURN_IF(p_shift_bits(ctx, "2*P*Q", sd, PQ, Qn, 1));
+        GPUPI_RETURN_IF(compact(ctx, Qn, "Q", sd));
         GPUPI_RETURN_IF(bPP.view(1, P.length * 2, PP));
-        GPUPI_RETURN_IF(batch_mp_mul(ctx, P, P, PP));
-        GPUPI_RETURN_IF(compact(ctx, PP));
+        GPUPI_RETURN_IF(p_mul(ctx, "P*P", sd, P, P, PP));
+        GPUPI_RETURN_IF(compact(ctx, PP, "P*P", sd));
         GPUPI_RETURN_IF(bQQ.view(1, Q.length * 2, QQ));
-        GPUPI_RETURN_IF(batch_mp_mul(ctx, Q, Q, QQ));
-        GPUPI_RETURN_IF(compact(ctx, QQ));
+        GPUPI_RETURN_IF(p_mul(ctx, "Q*Q", sd, Q, Q, QQ));
+        GPUPI_RETURN_IF(compact(ctx, QQ, "Q*Q", sd));
         GPUPI_RETURN_IF(bQQ5.view(1, QQ.length + 1, QQ5));
-        GPUPI_RETURN_IF(batch_mp_mul_small(ctx, QQ, 10005u, QQ5));
-        GPUPI_RETURN_IF(compact(ctx, QQ5));
+        GPUPI_RETURN_IF(p_mul_small(ctx, "10005*Q*Q", sd, QQ, 10005u, QQ5));
+        GPUPI_RETURN_IF(compact(ctx, QQ5, "10005*Q*Q", sd));
         GPUPI_RETURN_IF(bPn.view(1, std::max(PP.length, QQ5.length) + 1, Pn));
-        GPUPI_RETURN_IF(batch_mp_add(ctx, PP, QQ5, Pn));
-        GPUPI_RETURN_IF(compact(ctx, Pn));
+        GPUPI_RETURN_IF(p_add(ctx, "P*P+10005*Q*Q", sd, PP, QQ5, Pn));
+        GPUPI_RETURN_IF(compact(ctx, Pn, "P", sd));
         bP.swap(bPn);
         bQ.swap(bQn);
         P = Pn;
@@ -490,7 +702,7 @@ hipError_t gpupi_recursive_inverse(BatchMPContext *ctx, BatchMPArray x, uint64_t
         GPUPI_RETURN_IF(upload(ctx, qv, y0));
         GPUPI_RETURN_IF(upload(ctx, rv, res));
     }
-    GPUPI_RETURN_IF(compact(ctx, res));
+    GPUPI_RETURN_IF(compact(ctx, res, "res", (uint32_t)(levels.back() >> 5)));
 
     for (size_t i = levels.size() - 1; i > 0; --i) {
         const uint64_t L1 = levels[i], L2 = levels[i - 1];
@@ -501,8 +713,9 @@ hipError_t gpupi_recursive_inverse(BatchMPContext *ctx, BatchMPArray x, uint64_t
         const BatchMPArray x2y1 = view_of(res2_base, x2.length + y1.length, 1, x2.length + y1.length);
         BatchMPArray res1_m = view_of(res.data - mw, res.length + mw + 1, 1, res.length + mw + 1);
         if (!inside(x2y1, bws.ptr, bws.cap) || !inside(res1_m, bws.ptr, bws.cap)) return hipErrorInvalidValue;
-        GPUPI_RETURN_IF(batch_mp_mul(ctx, x2, y1, x2y1));
-        GPUPI_RETURN_IF(batch_mp_sub(ctx, res1_m, x2y1, res1_m));
+        const uint32_t d5 = (uint32_t)(L2 >> 5);
+        GPUPI_RETURN_IF(p_mul(ctx, "x2*y", d5, x2, y1, x2y1));
+        GPUPI_RETURN_IF(p_sub(ctx, "res1_m", d5, res1_m, x2y1, res1_m));
         uint32_t top = 0;
         GPUPI_RETURN_IF(hipMemcpyAsync(ctx->h_scalar, res1_m.data + res1_m.length - 1, 4, hipMemcpyDeviceToHost,
                                        ctx->stream));
@@ -510,27 +723,27 @@ hipError_t gpupi_recursive_inverse(BatchMPContext *ctx, BatchMPArray x, uint64_t
         top = *ctx->h_scalar;
         const bool negative = top >= 0x80000000u;
         if (negative) {
-            GPUPI_RETURN_IF(batch_mp_sub(ctx, view_of(res1_m.data, 0, 1, 0), res1_m, res1_m));
+            GPUPI_RETURN_IF(p_sub(ctx, "res1_m_neg", d5, view_of(res1_m.data, 0, 1, 0), res1_m, res1_m));
         }
         const BatchMPArray res2 = view_of(res2_base, res1_m.length + y1.length, 1, res1_m.length + y1.length);
-        GPUPI_RETURN_IF(batch_mp_mul(ctx, res1_m, y1, res2));
+        GPUPI_RETURN_IF(p_mul(ctx, "res2*y", d5, res1_m, y1, res2));
         const uint32_t drop = (uint32_t)((L2 - m * 2) >> 5);
         const BatchMPArray y2 = view_of(y2_base, res2.length - drop, 1, res2.length - drop);
         if (res2.length < drop || !inside(res2, bws.ptr, bws.cap) || !inside(y2, bws.ptr, bws.cap)) return hipErrorInvalidValue;
-        GPUPI_RETURN_IF(shift_right_large(ctx, res2, L2 - m * 2u, y2));
-        if (negative) GPUPI_RETURN_IF(batch_mp_add_small(ctx, y2, 1u, y2));
+        GPUPI_RETURN_IF(shift_right_large(ctx, res2, L2 - m * 2u, y2, "y2", d5));
+        if (negative) GPUPI_RETURN_IF(p_add_small(ctx, "y2_inc", d5, y2, 1u, y2));
         const BatchMPArray y_new = make_subview(y, (uint32_t)((L - L2) >> 6), y.length - (uint32_t)((L - L2) >> 6));
-        if (!negative) GPUPI_RETURN_IF(batch_mp_add(ctx, y_new, y2, y_new));
-        else GPUPI_RETURN_IF(batch_mp_sub(ctx, y_new, y2, y_new));
+        if (!negative) GPUPI_RETURN_IF(p_add(ctx, "(y<<m)+y2", d5, y_new, y2, y_new));
+        else GPUPI_RETURN_IF(p_sub(ctx, "(y<<m)-y2", d5, y_new, y2, y_new));
         if (L2 != L) {
             const BatchMPArray x_next = make_subview(x, (uint32_t)((L - L2) >> 6), x.length - (uint32_t)((L - L2) >> 6));
             res = view_of(res1_m.data - mw, res1_m.length + mw, 1, res1_m.length + mw);
             const BatchMPArray y2x = view_of(res2_base, y2.length + x_next.length, 1, y2.length + x_next.length);
             if (!inside(res, bws.ptr, bws.cap) || !inside(y2x, bws.ptr, bws.cap)) return hipErrorInvalidValue;
-            GPUPI_RETURN_IF(batch_mp_mul(ctx, y2, x_next, y2x));
-            if (!negative) GPUPI_RETURN_IF(batch_mp_sub(ctx, res, y2x, res));
-            else GPUPI_RETURN_IF(batch_mp_sub(ctx, y2x, res, res));
-            GPUPI_RETURN_IF(compact(ctx, res));
+            GPUPI_RETURN_IF(p_mul(ctx, "y2*x", d5, y2, x_next, y2x));
+            if (!negative) GPUPI_RETURN_IF(p_sub(ctx, "(res<<m)-y2x", d5, res, y2x, res));
+            else GPUPI_RETURN_IF(p_sub(ctx, "y2x-(res<<m)", d5, y2x, res, res));
+            GPUPI_RETURN_IF(compact(ctx, res, "res", d5));
         }
     }
     GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
@@ -561,8 +774,9 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
     Buf bRsh;
     BatchMPArray R_sh;
     GPUPI_RETURN_IF(bRsh.view(1, R_pre.length + 1, R_sh));
-    GPUPI_RETURN_IF(batch_mp_shift_bits(ctx, R_pre, R_sh, -rs_bits));
-    GPUPI_RETURN_IF(compact(ctx, R_sh));
+    const uint32_t td = (uint32_t)target_digits;
+    GPUPI_RETURN_IF(p_shift_bits(ctx, "R_shift", td, R_pre, R_sh, -rs_bits));
+    GPUPI_RETURN_IF(compact(ctx, R_sh, "R", td));
 
     long extra = std::max(0l, (long)std::min(Q_bs.length, R_sh.length) - prec_limbs);
     const BatchMPArray Q_tr = view_of(Q_bs.data + extra, Q_bs.length - (uint32_t)extra, 1, Q_bs.length - (uint32_t)extra);
@@ -579,15 +793,15 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
     BatchMPArray P_final, Q_final;
     GPUPI_RETURN_IF(bPf.view(1, P15.length + Q_tr.length, P_final));
     GPUPI_RETURN_IF(bQf.view(1, Q15.length + R_tr.length, Q_final));
-    GPUPI_RETURN_IF(batch_mp_mul(ctx, P15, Q_tr, P_final));
-    GPUPI_RETURN_IF(batch_mp_mul(ctx, R_tr, Q15, Q_final));
-    GPUPI_RETURN_IF(compact(ctx, P_final));
-    GPUPI_RETURN_IF(compact(ctx, Q_final));
+    GPUPI_RETURN_IF(p_mul(ctx, "P15*Q", td, P15, Q_tr, P_final));
+    GPUPI_RETURN_IF(p_mul(ctx, "R*Q15", td, R_tr, Q15, Q_final));
+    GPUPI_RETURN_IF(compact(ctx, P_final, "P", td));
+    GPUPI_RETURN_IF(compact(ctx, Q_final, "Q", td));
 
     extra = std::max(0l, (long)std::min(P_final.length, Q_final.length) - prec_limbs);
     const BatchMPArray P_tf = make_subview(P_final, (uint32_t)extra, P_final.length - (uint32_t)extra);
     BatchMPArray Q_tf = make_subview(Q_final, (uint32_t)extra, Q_final.length - (uint32_t)extra);
-    GPUPI_RETURN_IF(compact(ctx, Q_tf));
+    GPUPI_RETURN_IF(compact(ctx, Q_tf, "Q", td));
     uint32_t q_bits = 0;
     if (Q_tf.length > 0) {
         GPUPI_RETURN_IF(hipMemcpyAsync(ctx->h_scalar, Q_tf.data + Q_tf.length - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -604,15 +818,15 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
     Buf bprod;
     BatchMPArray prod;
     GPUPI_RETURN_IF(bprod.view(1, P_tf.length + inv.length, prod));
-    GPUPI_RETURN_IF(batch_mp_mul(ctx, P_tf, inv, prod));
-    GPUPI_RETURN_IF(compact(ctx, prod));
+    GPUPI_RETURN_IF(p_mul(ctx, "P*invQ", td, P_tf, inv, prod));
+    GPUPI_RETURN_IF(compact(ctx, prod, "P*invQ", td));
 
     Buf bret;
     BatchMPArray ret;
     GPUPI_RETURN_IF(bret.view(1, prod.length - (uint32_t)(inverse_bits / 32u) + (uint32_t)prec_limbs, ret));
     const uint64_t final_shift = inverse_bits - (uint64_t)prec_limbs * 32u;
-    GPUPI_RETURN_IF(shift_right_large(ctx, prod, final_shift, ret));
-    GPUPI_RETURN_IF(compact(ctx, ret));
+    GPUPI_RETURN_IF(shift_right_large(ctx, prod, final_shift, ret, "RET_shift", td));
+    GPUPI_RETURN_IF(compact(ctx, ret, "RET", td));
     ret.data = bret.release();
     *ret_out = ret;
     if (groups_out) *groups_out = (uint32_t)N;
@@ -804,12 +1018,15 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
     uint64_t inverse_bits = 0;
     GPUPI_RETURN_IF(gpupi_pi_fractional(ctx, (int)prec_digits, &ret, &groups, &inverse_bits));
     ConvertDecimalPowerTable powers{};
-    hipError_t err = convert_decimal_precompute_powers(ctx, &powers, leaf, levels);
+    hipError_t err = stage(ctx, "decimal_powers", "precompute_5pow", convert_digits, levels, leaf, 0, false,
+                           [&] { return convert_decimal_precompute_powers(ctx, &powers, leaf, levels); });
     char *digits = nullptr;
     if (err == hipSuccess) err = hipMalloc(&digits, std::max<uint32_t>(convert_digits, 1));
     if (err == hipSuccess && requested_digits > 0) {
         const BatchMPArray frac = make_subview(ret, 0, std::min(prec_limbs, ret.length));
-        err = convert_decimal_equal_split(ctx, frac, prec_limbs * 32u, convert_digits, leaf, &powers, digits);
+        err = stage(ctx, "decimal_convert", "equal_split", convert_digits, 1, frac.length, 0, false, [&] {
+            return convert_decimal_equal_split(ctx, frac, prec_limbs * 32u, convert_digits, leaf, &powers, digits);
+        });
     }
     convert_decimal_release_powers(&powers);
     if (err == hipSuccess) err = hipEventRecord(stop, ctx->stream);
@@ -832,6 +1049,93 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
     result->elapsed_ms = ms;
     result->inverse_bits = inverse_bits;
     return hipSuccess;
+}
+
+hipError_t gpupi_stage_profile_enable(BatchMPContext *ctx, int enable) {
+    if (ctx == nullptr) return hipErrorInvalidValue;
+    if (ctx->profiler == nullptr) {
+        if (!enable) return hipSuccess;
+        StageProfiler *p = new (std::nothrow) StageProfiler();
+        if (p == nullptr) return hipErrorOutOfMemory;
+        if (hipEventCreate(&p->start) != hipSuccess || hipEventCreate(&p->stop) != hipSuccess) {
+            stage_profiler_free(p);
+            return hipErrorOutOfMemory;
+        }
+        ctx->profiler = p;
+    }
+    ctx->profiler->enabled = enable != 0;
+    if (enable) ctx->profiler->entries.clear();
+    return hipSuccess;
+}
+
+int gpupi_stage_profile_query(const BatchMPContext *ctx, GpupiStageEntry *out, int max_entries) {
+    if (ctx == nullptr) return -1;
+    if (ctx->profiler == nullptr) return 0;
+    const int n = (int)ctx->profiler->entries.size();
+    for (int i = 0; i < n && i < max_entries && out != nullptr; ++i) out[i] = ctx->profiler->entries[i];
+    return n;
+}
+
+hipError_t gpupi_leaf_stage_create(BatchMPContext *ctx, uint32_t groups, int mode, GpupiLeafStage **out) {
+    if (ctx == nullptr || out == nullptr || groups == 0 || (groups & (groups - 1)) != 0 ||
+        groups > (1u << 26) || mode < GPUPI_LEAF_PLAIN || mode > GPUPI_LEAF_GRAPH)
+        return hipErrorInvalidValue;
+    *out = nullptr;
+    GpupiLeafStage *s = new (std::nothrow) GpupiLeafStage();
+    if (s == nullptr) return hipErrorOutOfMemory;
+    s->ctx = ctx;
+    s->N = groups;
+    s->mode = mode;
+    if (mode == GPUPI_LEAF_GRAPH) {
+        hipError_t err = hipStreamCreate(&s->stream);
+        if (err == hipSuccess) err = with_stage_context(s, [&] { return leaf_stage_capture(s); });
+        if (err != hipSuccess) {
+            gpupi_leaf_stage_destroy(s);
+            return err;
+        }
+    }
+    *out = s;
+    return hipSuccess;
+}
+
+hipError_t gpupi_leaf_stage_run(GpupiLeafStage *s, BatchMPArray *P, BatchMPArray *Q, BatchMPArray *R) {
+    if (s == nullptr || P == nullptr || Q == nullptr || R == nullptr) return hipErrorInvalidValue;
+    BatchMPContext *ctx = s->ctx;
+    BatchMPArray p, q, r;
+    if (s->mode == GPUPI_LEAF_GRAPH) {
+        p = s->P;
+        q = s->Q;
+        r = s->R;
+        GPUPI_RETURN_IF(with_stage_context(s, [&]() -> hipError_t {
+            GPUPI_RETURN_IF(hipGraphLaunch(s->exec, s->stream));
+            GPUPI_RETURN_IF(compact(ctx, p, "P", 2 * s->N));
+            GPUPI_RETURN_IF(compact(ctx, q, "Q", 2 * s->N));
+            return compact(ctx, r, "R", 2 * s->N);
+        }));
+    } else {
+        const bool plain = s->mode == GPUPI_LEAF_PLAIN;
+        GPUPI_RETURN_IF(leaf_stage(ctx, s->t, s->N, plain, p, q, r));
+        if (!plain) {
+            GPUPI_RETURN_IF(compact(ctx, p, "P", 2 * s->N));
+            GPUPI_RETURN_IF(compact(ctx, q, "Q", 2 * s->N));
+            GPUPI_RETURN_IF(compact(ctx, r, "R", 2 * s->N));
+        }
+        GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+    }
+    *P = p;
+    *Q = q;
+    *R = r;
+    return hipSuccess;
+}
+
+void gpupi_leaf_stage_destroy(GpupiLeafStage *s) {
+    if (s == nullptr) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->ws) (void)hipFree(s->ws);
+    delete s;
 }
 
 void gpupi_pi_free(GpupiPiResult *result) {

@@ -67,6 +67,35 @@ bool check(hipError_t e, const char *what) {
     return false;
 }
 
+// pi_bs_gpu.cu:2258-2325: one line per stage, totals per kind, then the
+// stats line with the accounted (sum of stage) time.
+void print_stage_profile(BatchMPContext *ctx, const GpupiPiResult &res, int ws_mb) {
+    const int n = gpupi_stage_profile_query(ctx, nullptr, 0);
+    std::vector<GpupiStageEntry> e(n > 0 ? n : 0);
+    if (n > 0) gpupi_stage_profile_query(ctx, e.data(), n);
+    static const char *const kinds[] = {"leaf", "mul", "mul_small", "shift", "add", "sub", "compact",
+                                        "exactdiv", "decimal_powers", "decimal_convert"};
+    double total[10] = {}, ntt_ms = 0.0;
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 10; ++k)
+            if (!strcmp(e[i].kind, kinds[k])) total[k] += e[i].elapsed_ms;
+        if (!strcmp(e[i].kind, "mul") && e[i].uses_ntt) ntt_ms += e[i].elapsed_ms;
+        printf("stage[%03d] kind=%9s label=%18s digits=%8u batch=%8u lhs_len=%9u rhs_len=%9u kernel_ms=%6.3f "
+               "uses_ntt=%d\n",
+               i, e[i].kind, e[i].label, e[i].n1, e[i].batch_size, e[i].lhs_length, e[i].rhs_length,
+               e[i].elapsed_ms, e[i].uses_ntt);
+    }
+    printf("leaf_total_ms=%.3f mul_total_ms=%.3f mul_small_total_ms=%.3f shift_total_ms=%.3f add_total_ms=%.3f "
+           "sub_total_ms=%.3f compact_total_ms=%.3f exactdiv_total_ms=%.3f decimal_powers_total_ms=%.3f "
+           "decimal_convert_total_ms=%.3f ntt_total_ms=%.3f\n",
+           total[0], total[1], total[2], total[3], total[4], total[5], total[6], total[7], total[8], total[9],
+           ntt_ms);
+    double accounted = 0.0;
+    for (double t : total) accounted += t;
+    printf("target_digits=%u convert_digits=%u RET_len=%u elapsed_ms=%.3f accounted_ms=%.3f workspace_max=%dMB\n",
+           res.requested_digits, res.convert_digits, res.ret_length, res.elapsed_ms, accounted, ws_mb);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -103,6 +132,10 @@ int main(int argc, char **argv) {
         fprintf(stderr, "batch_mp_init failed\n");
         return 1;
     }
+    if (profile && !check(gpupi_stage_profile_enable(ctx, 1), "gpupi_stage_profile_enable")) {
+        batch_mp_destroy(ctx);
+        return 1;
+    }
     GpupiPiResult res{};
     if (!check(gpupi_pi_run(ctx, requested, convention, &res), "gpupi_pi_run")) {
         batch_mp_destroy(ctx);
@@ -119,7 +152,11 @@ int main(int argc, char **argv) {
         return 1;
     }
     const int ws_mb = (int)(batch_mp_workspace_size(ctx) / (1024 * 1024));
-    if (benchmark || profile) {
+    if (profile) {
+        print_stage_profile(ctx, res, ws_mb);
+        print_hex_preview("RET", ret);
+        if (requested > 0) printf("DEC_preview_hash = %u\n", decimal_hash(digits.data(), std::min(requested, 4096u)));
+    } else if (benchmark) {
         printf("target_digits=%u convert_digits=%u RET_len=%u elapsed_ms=%.3f workspace_max=%dMB\n", requested,
                res.convert_digits, res.ret_length, res.elapsed_ms, ws_mb);
         print_hex_preview("RET", ret);
@@ -128,10 +165,6 @@ int main(int argc, char **argv) {
             printf("DEC = 3%.*s...%.*s\n", (int)pre, digits.data(), (int)suf, digits.data() + (requested - suf));
         } else {
             printf("DEC = 3\n");
-        }
-        if (profile) {
-            const uint32_t n = std::min(requested, 4096u);
-            printf("DEC_preview_hash = %u\n", decimal_hash(digits.data(), n));
         }
     } else {
         fprintf(stderr, "target_digits=%u convert_digits=%u RET_len=%u elapsed_ms=%.3f\n", requested,

@@ -72,6 +72,48 @@ typedef struct GpupiKernelStat {
 void gpupi_kernel_stats_reset(int enable);
 int gpupi_kernel_stats_query(GpupiKernelStat *out, int max_entries);
 
+/* The 17-term group stage on its own (pi_bs_gpu.cu:555-779) for `groups`
+ * groups (a power of two), in three forms that leave the same P, Q, R rows
+ * (one per group, compacted):
+ *   GPUPI_LEAF_PLAIN    the reference's per-op form: every product is
+ *                       followed by a synchronizing compaction;
+ *   GPUPI_LEAF_BATCHED  lengths fixed in advance (allocation bounds), all
+ *                       launches queued without a host round trip, one
+ *                       compaction per array at the end;
+ *   GPUPI_LEAF_GRAPH    the batched form captured once as a hipGraph on the
+ *                       stage's own stream (with its own scratch) and
+ *                       replayed by every run.
+ * No counterpart in the reference (its tree always runs the plain form).
+ * run returns views of P, Q, R that stay valid until the next run or
+ * destroy.  A GRAPH stage must not outlive its context. */
+enum { GPUPI_LEAF_PLAIN = 0, GPUPI_LEAF_BATCHED = 1, GPUPI_LEAF_GRAPH = 2 };
+typedef struct GpupiLeafStage GpupiLeafStage;
+hipError_t gpupi_leaf_stage_create(BatchMPContext *ctx, uint32_t groups, int mode, GpupiLeafStage **out);
+hipError_t gpupi_leaf_stage_run(GpupiLeafStage *stage, BatchMPArray *P, BatchMPArray *Q, BatchMPArray *R);
+void gpupi_leaf_stage_destroy(GpupiLeafStage *stage);
+
+/* Per-stage profile of the pipeline (the reference's StageProfiler,
+ * pi_bs_gpu.cu:133-489, behind --profile-stages).  When enabled, every
+ * multiply, add/sub, shift, compaction, exact division, leaf construction
+ * and decimal stage run through the context is bracketed by HIP events on
+ * the context stream and synchronized, and one entry is kept per stage;
+ * `kind` is one of leaf, mul, mul_small, shift, add, sub, compact,
+ * exactdiv, decimal_powers, decimal_convert.  enable(ctx, 1) clears the
+ * entries; query copies up to max_entries of them and returns how many are
+ * held (or -1 for a null context). */
+typedef struct GpupiStageEntry {
+    char kind[16];
+    char label[24];
+    uint32_t n1;
+    uint32_t batch_size;
+    uint32_t lhs_length;
+    uint32_t rhs_length;
+    float elapsed_ms;
+    int uses_ntt;
+} GpupiStageEntry;
+hipError_t gpupi_stage_profile_enable(BatchMPContext *ctx, int enable);
+int gpupi_stage_profile_query(const BatchMPContext *ctx, GpupiStageEntry *out, int max_entries);
+
 /* Sizing helpers (pi_bs_gpu.cu:924-944) */
 uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits);
 uint32_t gpupi_prec_limbs_for_digits(uint32_t digits);

@@ -395,7 +395,10 @@ static int mul_into(OArray A, OArray B, OArray *C, uint32_t batch) {
 #define CHECK(x) do { int _s = (x); if (_s != 0) return _s; } while (0)
 
 /* Merge of the whole 17N-term series.  Level loop: pi_bs_gpu.cu:625-785. */
-int o_binary_split(uint32_t N, OArray *Q_out, OArray *R_out) {
+/* The tree; with leaf_only the levels stop after the head merge and the
+ * block's exact division (the 17-term group stage), leaving N rows each of
+ * P, Q and R. */
+static int binary_split_impl(uint32_t N, int leaf_only, OArray *P_out, OArray *Q_out, OArray *R_out) {
     if (N == 0 || (N & (N - 1)) != 0) return -1;
     const uint32_t n = N * 16u;
     const uint32_t divisor_lo = 830297u, divisor_hi = 156590819u; /* pi_bs_gpu.cu:151-152 */
@@ -471,13 +474,27 @@ int o_binary_split(uint32_t N, OArray *Q_out, OArray *R_out) {
             free(Rp1.data); free(Rp2.data);
             free(P.data); free(Q.data); free(R.data);
             P = Ph; Q = Qh; R = Rh;
+            if (leaf_only) break;
         }
     }
-    free(P.data);
     free(P0.data); free(Q0.data); free(R0.data);
+    if (P_out) *P_out = P;
+    else free(P.data);
     *Q_out = Q;
     *R_out = R;
     return 0;
+}
+
+int o_binary_split(uint32_t N, OArray *Q_out, OArray *R_out) {
+    return binary_split_impl(N, 0, NULL, Q_out, R_out);
+}
+
+/* pi_bs_gpu.cu:555-779: leaf terms, the four pairwise levels of each
+ * 16-term body, the head-term merge and the exact division by
+ * 13^2 17^3 7^6 11^3: one row of P, Q, R per 17-term group. */
+int o_leaf_stage(uint32_t N, OArray *P_out, OArray *Q_out, OArray *R_out) {
+    if (P_out == NULL) return -1;
+    return binary_split_impl(N, 1, P_out, Q_out, R_out);
 }
 
 /* ------------------------------------------------------------------ */

@@ -51,6 +51,7 @@ void o_decimal_naive(const uint32_t *A, char *B, uint32_t N, int L_limb,
 void o_leaf17(int is_head, uint32_t *P, uint32_t *Q, uint32_t *R, uint32_t count);
 /* Returns malloc'd Q and R rows (batch 1) of the whole 17*N-term series. */
 int o_binary_split(uint32_t N, OArray *Q_out, OArray *R_out);
+int o_leaf_stage(uint32_t N, OArray *P_out, OArray *Q_out, OArray *R_out);
 int o_sqrt_10005(int target_digits, OArray *P_out, OArray *Q_out);
 /* y = approximate 2^L / x, length x.length (pi_bs_gpu.cu:1179-1344). */
 int o_recursive_inverse(OArray x, uint64_t L, OArray *y_out);

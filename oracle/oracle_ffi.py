@@ -65,6 +65,7 @@ def lib():
         L.o_decimal_naive.argtypes = [ctypes.c_void_p, ctypes.c_void_p, u32, ctypes.c_int, u32, ctypes.c_int, u32]
         L.o_leaf17.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, u32]
         L.o_binary_split.argtypes = [u32, ctypes.POINTER(A), ctypes.POINTER(A)]
+        L.o_leaf_stage.argtypes = [u32, ctypes.POINTER(A), ctypes.POINTER(A), ctypes.POINTER(A)]
         L.o_sqrt_10005.argtypes = [ctypes.c_int, ctypes.POINTER(A), ctypes.POINTER(A)]
         L.o_recursive_inverse.argtypes = [A, ctypes.c_uint64, ctypes.POINTER(A)]
         L.o_pi_fractional.argtypes = [ctypes.c_int, ctypes.POINTER(A), ctypes.POINTER(OPiInfo)]
@@ -125,6 +126,18 @@ def take_array(a: OArray) -> np.ndarray:
     """Copy a malloc'd batch-1 oracle array and free it."""
     n = a.length
     out = np.ctypeslib.as_array(ctypes.cast(a.data, ctypes.POINTER(ctypes.c_uint32)), shape=(max(n, 1),))[:n].copy()
+    lib().o_free(a.data)
+    return out
+
+
+def take_rows(a: OArray) -> np.ndarray:
+    """Copy a malloc'd oracle array as (batch_size, length) rows and free it."""
+    n, length, stride = a.batch_size, a.length, a.stride if a.stride else a.length
+    total = (n - 1) * stride + length if n else 0
+    flat = np.ctypeslib.as_array(ctypes.cast(a.data, ctypes.POINTER(ctypes.c_uint32)), shape=(max(total, 1),))
+    out = np.zeros((n, length), dtype=np.uint32)
+    for i in range(n):
+        out[i] = flat[i * stride: i * stride + length]
     lib().o_free(a.data)
     return out
 

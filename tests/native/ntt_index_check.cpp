@@ -148,8 +148,21 @@ static void check_plan(int k, bool full) {
     }
 }
 
+static void check_xcd_tile() {
+    for (uint32_t n = 1; n < 3000; n += (n < 100 ? 1 : 37)) {
+        std::vector<int> seen(n, 0);
+        for (uint32_t b = 0; b < n; ++b) {
+            const uint32_t t = xcd_tile(b, n);
+            EXPECT(t < n, "xcd_tile range n=%u", n);
+            if (t < n) seen[t]++;
+        }
+        for (uint32_t t = 0; t < n; ++t) EXPECT(seen[t] == 1, "xcd_tile bijection n=%u", n);
+    }
+}
+
 int main() {
     check_tile_steps();
+    check_xcd_tile();
     check_bitrev_split();
     for (int k = kMinLog2; k <= kMaxLog2; ++k) check_plan(k, k <= 23);
     printf(fails ? "index check: %d failures\n" : "index check: ok\n", fails);

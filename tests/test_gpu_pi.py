@@ -100,6 +100,58 @@ def test_pi_readme_convention(ctx, digits):
     assert dec == odec
 
 
+STAGE_KINDS = {"leaf", "mul", "mul_small", "shift", "add", "sub", "compact", "exactdiv", "decimal_powers",
+               "decimal_convert"}
+
+
+def test_stage_profile(ctx):
+    # the reference's --profile-stages accounting (pi_bs_gpu.cu:133-489,
+    # :2258-2325): profiling leaves the result unchanged and every stage kind
+    # of the pipeline shows up
+    ret0, dec0, _ = ctx.pi(100000)
+    ctx.stage_profile(True)
+    try:
+        ret1, dec1, res1 = ctx.pi(100000)
+        entries = ctx.stage_entries()
+    finally:
+        ctx.stage_profile(False)
+    assert np.array_equal(ret0, ret1) and dec0 == dec1
+    assert {e["kind"] for e in entries} == STAGE_KINDS
+    assert sum(1 for e in entries if e["kind"] == "leaf") == 2
+    assert any(e["ntt"] for e in entries if e["kind"] == "mul")
+    assert not any(e["ntt"] for e in entries if e["kind"] != "mul")
+    assert 0 < sum(e["ms"] for e in entries) <= res1.elapsed_ms * 1.05 + 1.0
+    ctx.pi(1000)  # disabled: nothing more is recorded
+    assert len(ctx.stage_entries()) == len(entries)
+
+
+def decimal_hash(digits: str) -> int:
+    v = 3  # pi_bs_gpu.cu:860-866
+    for ch in digits:
+        v = (v * 10 + ord(ch) - 48) % 1000000007
+    return v
+
+
+def test_cli_profile_stages():
+    import os
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(g.LIB_PATH), "pi_bs_gpu")
+    out = subprocess.run([exe, "100000", "--profile-stages"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    stages = [ln for ln in lines if ln.startswith("stage[")]
+    assert len(stages) > 50
+    assert all(re.match(r"stage\[\d{3,}\] kind=\s*\S+ label=\s*\S+ digits=\s*\d+ batch=\s*\d+ lhs_len=\s*\d+ "
+                        r"rhs_len=\s*\d+ kernel_ms=\s*[\d.]+ uses_ntt=[01]$", ln) for ln in stages)
+    totals = next(ln for ln in lines if ln.startswith("leaf_total_ms="))
+    assert "ntt_total_ms=" in totals and "decimal_convert_total_ms=" in totals
+    stats = next(ln for ln in lines if ln.startswith("target_digits=100000 "))
+    assert "accounted_ms=" in stats and f"RET_len={of.lib().o_prec_limbs_for_digits(100000) + 1} " in stats
+    _, odec, _ = of.pi_digits(100000)
+    assert f"DEC_preview_hash = {decimal_hash(odec[:4096])}" in lines
+
+
 @pytest.mark.slow
 def test_pi_1e7_matches_oracle(ctx):
     ret, dec, res = ctx.pi(10_000_000)

@@ -15,3 +15,16 @@ def test_ntt_index_math(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "index check: ok" in out.stdout
+
+
+def test_ntt_arith_bounds(tmp_path):
+    """The butterflies' re-centring plans (gpupi_amd/csrc/ntt_arith.hpp) keep
+    every value exact and within its bound: each butterfly kind, radix and
+    prime on all sign patterns at its input bounds and on random inputs,
+    against 128-bit modular arithmetic."""
+    exe = tmp_path / "ntt_bounds_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "gpupi_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "native", "ntt_bounds_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "bounds check: ok" in out.stdout

@@ -69,6 +69,36 @@ def test_binary_split_is_the_chudnovsky_sum(groups):
     assert Fraction(R, Q << (15 * 17 * groups)) == chudnovsky_partial(17 * groups)
 
 
+def group_sum(k0):
+    """(P/Q, R/(Q 2^255)) of the 17 terms k0..k0+16, each group's series
+    starting with a + sign."""
+    s, ratio = Fraction(0), Fraction(1)
+    for j in range(17):
+        k = k0 + j
+        s += (-1) ** j * ratio * Fraction(A_CONST + B_CONST * k, 3335 << 15)
+        ratio *= Fraction((6 * k + 1) * (2 * k + 1) * (6 * k + 5), (k + 1) ** 3 * C3_OVER_24)
+    return ratio * (1 << (15 * 17)), s
+
+
+@pytest.mark.parametrize("groups", [1, 4])
+def test_leaf_stage_is_each_groups_sum(groups):
+    # the 17-term group stage (leaf terms, 4 pairwise levels, head merge,
+    # exact division): row g holds group g's P/Q and series value
+    op, oq, orr = of.OArray(), of.OArray(), of.OArray()
+    assert of.lib().o_leaf_stage(groups, ctypes.byref(op), ctypes.byref(oq), ctypes.byref(orr)) == 0
+    P, Q, R = of.take_rows(op), of.take_rows(oq), of.take_rows(orr)
+    assert P.shape[0] == Q.shape[0] == R.shape[0] == groups
+    for g in range(groups):
+        pq, s = group_sum(17 * g)
+        Pg, Qg, Rg = to_int(P[g]), to_int(Q[g]), to_int(R[g])
+        assert Fraction(Pg, Qg) == pq
+        assert Fraction(Rg, Qg << (15 * 17)) == s
+    if groups == 1:  # one group: the whole tree
+        oq2, or2 = of.OArray(), of.OArray()
+        assert of.lib().o_binary_split(1, ctypes.byref(oq2), ctypes.byref(or2)) == 0
+        assert to_int(of.take_array(oq2)) == to_int(Q[0]) and to_int(of.take_array(or2)) == to_int(R[0])
+
+
 @pytest.mark.parametrize("digits", [5, 8, 16, 100, 5000, 100000])
 def test_sqrt_10005_pell(digits):
     op, oq = of.OArray(), of.OArray()
