@@ -160,6 +160,213 @@ def bench_leaf(args, g) -> int:
     return 0 if same else 1
 
 
+def ragged_pairs(seed: int, count: int, lo: float, hi: float):
+    """count pairs with lengths floor(2^U), U uniform in [lo, hi) (drawn L_a
+    then L_b per pair), from numpy's default generator seeded with `seed`."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(lo, hi, size=(count, 2))
+    return rng, [(int(2 ** a), int(2 ** b)) for a, b in u]
+
+
+def bench_ragged(args, g) -> int:
+    """256 pairs, lengths log-uniform in [2^10, 2^24] limbs, uniform random
+    limbs (seed 1234): one ragged call (gpupi_mul_ragged) against one
+    batch_mp_mul call per pair, with inputs resident in HBM.  Reports ms for
+    both forms, per-pass NTT GB/s of the ragged form, and the reference's
+    compulsory figure 4*sum(L_a+L_b+L_c)/t (batch_mul_ntt_test.cu:491-492)."""
+    import numpy as np
+    rng, lengths = ragged_pairs(1234, args.pairs, 10.0, 24.0)
+    ctx = g.Context()
+    bufs, items, uni = [], [], []
+    for la, lb in lengths:
+        A = rng.integers(0, 1 << 32, size=la, dtype=np.uint64).astype(np.uint32)
+        B = rng.integers(0, 1 << 32, size=lb, dtype=np.uint64).astype(np.uint32)
+        ba, bb = g.DeviceBuffer.from_array(A), g.DeviceBuffer.from_array(B)
+        bc, bu = g.DeviceBuffer(4 * (la + lb)), g.DeviceBuffer(4 * (la + lb))
+        bufs.append((ba, bb, bc, bu))
+        items.append((ba.ptr, bb.ptr, bc.ptr, la, lb))
+        uni.append((g.BatchMPArray(ba.ptr, la, 1, la), g.BatchMPArray(bb.ptr, lb, 1, lb),
+                    g.BatchMPArray(bu.ptr, la + lb, 1, la + lb)))
+    del A, B
+
+    def plain():
+        for a_, b_, c_ in uni:
+            g.check(ctx.mul(a_, b_, c_), "mul")
+
+    def ragged():
+        g.check(ctx.mul_ragged(items), "gpupi_mul_ragged")
+
+    res = {}
+    for name, fn in (("plain", plain), ("ragged", ragged)):
+        for _ in range(args.warmup):
+            fn()
+        g.synchronize()
+        if name == "ragged":
+            g.kernel_stats_reset(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        g.synchronize()
+        res[name] = (time.perf_counter() - t0) * 1000.0 / args.steps
+    stats = g.kernel_stats()
+    g.kernel_stats_reset(False)
+    same = all(np.array_equal(bc.download(np.uint32), bu.download(np.uint32)) for _, _, bc, bu in bufs)
+    limbs = sum(2 * (la + lb) for la, lb in lengths)
+    passes = {s[0]: {"launches": int(s[1]), "ms_per_call": round(s[2] / args.steps, 3),
+                     "GBps": round(s[3] / (s[2] / 1e3) / 1e9, 1),
+                     "frac_of_hbm_peak": round(s[3] / (s[2] / 1e3) / 1e9 / HBM_PEAK_GBPS, 3)} for s in stats}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        # GMP mpn_mul on a bounded sample of the pairs (the smallest ones up
+        # to ~20 s of CPU work), each checked against the GPU product
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_ffi as of
+        order = sorted(range(len(lengths)), key=lambda i: lengths[i][0] + lengths[i][1])
+        done, t_cpu, limbs_cpu, exact = 0, 0.0, 0, True
+        for i in order:
+            la, lb = lengths[i]
+            ba, bb, bc, _ = bufs[i]
+            A = ba.download(np.uint32).reshape(1, la)
+            B = bb.download(np.uint32).reshape(1, lb)
+            hc = of.HostArray.empty(1, la + lb)
+            t0 = time.perf_counter()
+            assert of.lib().o_mul(of.HostArray(A).oarray(), of.HostArray(B).oarray(), hc.oarray()) == 0
+            t_cpu += time.perf_counter() - t0
+            exact = exact and np.array_equal(hc.rows()[0], bc.download(np.uint32))
+            done += 1
+            limbs_cpu += 2 * (la + lb)
+            if t_cpu > 20.0:
+                break
+        cpu = {"value": round(4.0 * limbs_cpu / t_cpu / 1e9, 3), "unit": "GB/s (A+B+ret)", "cores": 1,
+               "kind": "port", "sample": f"GMP mpn_mul on the {done} smallest of {len(lengths)} pairs "
+                                         f"({limbs_cpu} limbs in+out) in {t_cpu:.1f} s, one thread",
+               "gpu_products_checked_bit_exact": exact}
+    ctx.close()
+    print(json.dumps({"metric": f"ms per batched product of {len(lengths)} ragged pairs (one ragged call)",
+                      "value": round(res["ragged"], 3), "unit": "ms", "steps": args.steps, "warmup": args.warmup,
+                      "plain_ms": round(res["plain"], 3), "speedup_ragged_vs_plain": round(res["plain"] / res["ragged"], 3),
+                      "a_b_ret_GBps": round(4.0 * limbs / (res["ragged"] / 1e3) / 1e9, 1),
+                      "total_limbs_in_out": limbs, "identical_to_plain": same, "ntt_passes": passes,
+                      "cpu_baseline": cpu}))
+    return 0 if same else 1
+
+
+def bench_serve(args, g) -> int:
+    """The serving loop on a Poisson request stream (seed 7): 40% pi jobs of
+    floor(10^U(5,8)) digits, 60% multiply jobs of floor(2^U(12,26)) limbs per
+    operand with uniform random limbs.  Every job first runs alone (service
+    time, reference result), then the stream arrives at 70% of the measured
+    sequential capacity; reports p50/p99 latency and throughput, and checks
+    that every result in the mix equals its result alone."""
+    import hashlib
+    import threading
+    import numpy as np
+    rng = np.random.default_rng(7)
+    n = args.jobs
+    jobs = []
+    for _ in range(n):
+        if rng.random() < 0.4:
+            jobs.append(("pi", int(10 ** rng.uniform(5, 8))))
+        else:
+            la, lb = int(2 ** rng.uniform(12, 26)), int(2 ** rng.uniform(12, 26))
+            A = rng.integers(0, 1 << 32, size=la, dtype=np.uint64).astype(np.uint32)
+            B = rng.integers(0, 1 << 32, size=lb, dtype=np.uint64).astype(np.uint32)
+            jobs.append(("mul", (A, B)))
+    gaps = rng.exponential(1.0, size=n)  # scaled by the rate below
+
+    def digest(kind, r, data):
+        if kind == "pi":
+            return (r.ret_length, r.ret_hash, r.dec_hash)
+        return hashlib.sha256(data.tobytes()).hexdigest()
+
+    srv = g.Server(budget_bytes=200 * 10 ** 9)
+    alone, service = {}, {}
+    for i, (kind, x) in enumerate(jobs):
+        jid = srv.submit_pi(x) if kind == "pi" else srv.submit_mul(*x)
+        assert jid >= 0
+        (r, data), = srv.results(wait=True)
+        assert r.status == 0, (kind, r.status)
+        alone[i] = digest(kind, r, data)
+        service[i] = r.finish_ms - r.start_ms
+    mean_service = sum(service.values()) / n
+    rate = 0.7 / (mean_service / 1e3)  # jobs per second
+    arrivals = np.cumsum(gaps / rate)
+
+    results, id2job = {}, {}
+
+    def collect():
+        while len(results) < n:
+            got = srv.results(wait=True)
+            for r, data in got:
+                results[r.id] = (r, data)
+            if not got:  # nothing in flight yet: the next arrival is pending
+                time.sleep(0.0005)
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for i, (kind, x) in enumerate(jobs):
+        delay = arrivals[i] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        jid = srv.submit_pi(x) if kind == "pi" else srv.submit_mul(*x)
+        assert jid >= 0
+        id2job[jid] = i
+    th.join()
+    srv.close()
+    lat = {"pi": [], "mul": []}
+    same = True
+    first_submit = min(r.submit_ms for r, _ in results.values())
+    last_finish = max(r.finish_ms for r, _ in results.values())
+    batch_max = 0
+    for jid, (r, data) in results.items():
+        i = id2job[jid]
+        kind = jobs[i][0]
+        lat[kind].append(r.finish_ms - r.submit_ms)
+        same = same and digest(kind, r, data) == alone[i]
+        if kind == "mul":
+            batch_max = max(batch_max, r.batch_size)
+    allv = np.array(lat["pi"] + lat["mul"])
+    pct = lambda v, q: round(float(np.percentile(np.array(v), q)), 2) if v else None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        # GMP mpn_mul on the smallest multiply jobs (~10 s of CPU work), each
+        # checked against the GPU result of the mix
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_ffi as of
+        muls = sorted((i for i, j in enumerate(jobs) if j[0] == "mul"), key=lambda i: jobs[i][1][0].size + jobs[i][1][1].size)
+        by_job = {id2job[jid]: data for jid, (r, data) in results.items()}
+        t_cpu, done, exact, limbs = 0.0, 0, True, 0
+        for i in muls:
+            A, B = jobs[i][1]
+            hc = of.HostArray.empty(1, A.size + B.size)
+            t1 = time.perf_counter()
+            assert of.lib().o_mul(of.HostArray(A.reshape(1, -1)).oarray(), of.HostArray(B.reshape(1, -1)).oarray(),
+                                  hc.oarray()) == 0
+            t_cpu += time.perf_counter() - t1
+            exact = exact and np.array_equal(hc.rows()[0], by_job[i])
+            done += 1
+            limbs += 2 * (A.size + B.size)
+            if t_cpu > 10.0:
+                break
+        cpu = {"value": round(done / t_cpu, 2), "unit": "multiply jobs/s", "cores": 1, "kind": "port",
+               "sample": f"GMP mpn_mul on the {done} smallest multiply jobs ({limbs} limbs in+out), one thread",
+               "gpu_results_checked_bit_exact": exact}
+    print(json.dumps({"metric": f"serving p99 latency, {n} jobs, Poisson arrivals at 70% of sequential capacity",
+                      "value": pct(allv.tolist(), 99), "unit": "ms", "higher_is_better": False,
+                      "p50_ms": pct(allv.tolist(), 50), "p99_ms": pct(allv.tolist(), 99),
+                      "pi": {"jobs": len(lat["pi"]), "p50_ms": pct(lat["pi"], 50), "p99_ms": pct(lat["pi"], 99)},
+                      "mul": {"jobs": len(lat["mul"]), "p50_ms": pct(lat["mul"], 50), "p99_ms": pct(lat["mul"], 99),
+                              "max_jobs_per_shared_launch": batch_max},
+                      "throughput_jobs_per_s": round(n / ((last_finish - first_submit) / 1e3), 2),
+                      "arrival_rate_jobs_per_s": round(rate, 2), "mean_service_alone_ms": round(mean_service, 2),
+                      "identical_alone_and_in_mix": same, "hbm_budget_gb": 200, "cpu_baseline": cpu}))
+    return 0 if same else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,9 +375,13 @@ def main() -> int:
     ap.add_argument("--digits", type=int, default=1_000_000_000)
     ap.add_argument("--prec", choices=["rounded", "requested"], default="rounded")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["pi", "mul", "leaf"], default="pi",
+    ap.add_argument("--pairs", type=int, default=256)
+    ap.add_argument("--jobs", type=int, default=120)
+    ap.add_argument("--workload", choices=["pi", "mul", "leaf", "ragged", "serve"], default="pi",
                     help="pi: the headline run; mul: one batched product shape (kernel study); "
-                         "leaf: the 17-term group stage of --digits in plain, batched and graph form")
+                         "leaf: the 17-term group stage of --digits in plain, batched and graph form; "
+                         "ragged: --pairs log-uniform pairs in one ragged call vs one call per pair; "
+                         "serve: the serving loop on a Poisson stream of --jobs pi and multiply jobs")
     ap.add_argument("--la", type=int, default=1 << 27)
     ap.add_argument("--lb", type=int, default=1 << 27)
     ap.add_argument("--n", type=int, default=1)
@@ -188,6 +399,10 @@ def main() -> int:
         return bench_mul(args, g)
     if args.workload == "leaf":
         return bench_leaf(args, g)
+    if args.workload == "ragged":
+        return bench_ragged(args, g)
+    if args.workload == "serve":
+        return bench_serve(args, g)
 
     conv = g.PREC_ROUNDED if args.prec == "rounded" else g.PREC_REQUESTED
     lib = g.lib()

@@ -72,6 +72,30 @@ class GpupiKernelStat(ctypes.Structure):
     ]
 
 
+class GpupiMulItem(ctypes.Structure):
+    _fields_ = [
+        ("A", ctypes.c_void_p),
+        ("B", ctypes.c_void_p),
+        ("C", ctypes.c_void_p),
+        ("L_a", ctypes.c_uint32),
+        ("L_b", ctypes.c_uint32),
+    ]
+
+
+class GpupiServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("pi_workers", ctypes.c_uint32),
+                ("mul_batch_max", ctypes.c_uint32)]
+
+
+class GpupiJobResult(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_int64), ("kind", ctypes.c_int32), ("status", ctypes.c_int32),
+        ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double), ("finish_ms", ctypes.c_double),
+        ("reserved_bytes", ctypes.c_uint64), ("batch_size", ctypes.c_uint32), ("ret_length", ctypes.c_uint32),
+        ("ret_hash", ctypes.c_uint32), ("dec_hash", ctypes.c_uint32),
+    ]
+
+
 class GpupiStageEntry(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_char * 16),
@@ -145,6 +169,18 @@ def _declare(lib):
         "gpupi_kernel_stats_reset": (None, [ctypes.c_int]),
         "gpupi_kernel_stats_query": (ctypes.c_int, [ctypes.POINTER(GpupiKernelStat), ctypes.c_int]),
         "gpupi_stage_profile_enable": (err, [c_ctx, ctypes.c_int]),
+        "gpupi_mul_ragged": (err, [c_ctx, ctypes.POINTER(GpupiMulItem), u32]),
+        "gpupi_server_create": (ctypes.c_void_p, [ctypes.POINTER(GpupiServeConfig)]),
+        "gpupi_server_submit_pi": (ctypes.c_int64, [ctypes.c_void_p, u32, ctypes.c_void_p]),
+        "gpupi_server_submit_mul": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_void_p, u32, ctypes.c_void_p, u32,
+                                                     ctypes.c_void_p]),
+        "gpupi_server_results": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GpupiJobResult), ctypes.c_int,
+                                                ctypes.c_int]),
+        "gpupi_server_drain": (None, [ctypes.c_void_p]),
+        "gpupi_server_destroy": (None, [ctypes.c_void_p]),
+        "gpupi_serve_job_bytes": (u64, [ctypes.c_int, u32, u32]),
+        "gpupi_serve_admit": (ctypes.c_int, [ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int32), ctypes.c_int, u64,
+                                             u64, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "gpupi_leaf_stage_create": (err, [c_ctx, u32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
         "gpupi_leaf_stage_run": (err, [ctypes.c_void_p, ctypes.POINTER(A), ctypes.POINTER(A), ctypes.POINTER(A)]),
         "gpupi_leaf_stage_destroy": (None, [ctypes.c_void_p]),
@@ -298,6 +334,11 @@ class Context:
         return ret, dec, res
 
 
+    def mul_ragged(self, items) -> int:
+        """items: sequence of (A_ptr, B_ptr, C_ptr, L_a, L_b); returns the hipError_t."""
+        arr = (GpupiMulItem * max(len(items), 1))(*[GpupiMulItem(*it) for it in items])
+        return self._lib.gpupi_mul_ragged(self.ptr, arr, len(items))
+
     def stage_profile(self, enable: bool) -> None:
         """Turn per-stage timing of the pipeline on (clearing it) or off."""
         check(self._lib.gpupi_stage_profile_enable(self.ptr, 1 if enable else 0), "gpupi_stage_profile_enable")
@@ -314,6 +355,69 @@ class Context:
 
 
 LEAF_PLAIN, LEAF_BATCHED, LEAF_GRAPH = 0, 1, 2
+JOB_PI, JOB_MUL = 0, 1
+
+
+def serve_admit(bytes_, kinds, budget: int, in_use: int, free_pi_workers: int) -> list:
+    """The server's admission rule on a queue (host only)."""
+    n = len(bytes_)
+    b = (ctypes.c_uint64 * max(n, 1))(*bytes_)
+    k = (ctypes.c_int32 * max(n, 1))(*kinds)
+    out = (ctypes.c_int * max(n, 1))()
+    m = lib().gpupi_serve_admit(b, k, n, budget, in_use, free_pi_workers, out)
+    return list(out[:m])
+
+
+class Server:
+    """The serving loop (gpupi_serve.h).  Host buffers passed to submit_*
+    are kept alive here until their results are collected."""
+
+    def __init__(self, budget_bytes: int = 0, pi_workers: int = 0, mul_batch_max: int = 0):
+        self._lib = lib()
+        cfg = GpupiServeConfig(budget_bytes, pi_workers, mul_batch_max)
+        self.ptr = self._lib.gpupi_server_create(ctypes.byref(cfg))
+        if not self.ptr:
+            raise RuntimeError("gpupi_server_create failed")
+        self.keep = {}
+
+    def submit_pi(self, digits: int, keep_digits: bool = False) -> int:
+        out = ctypes.create_string_buffer(max(digits, 1)) if keep_digits else None
+        jid = self._lib.gpupi_server_submit_pi(self.ptr, digits, out)
+        if jid >= 0:
+            self.keep[jid] = out
+        return jid
+
+    def submit_mul(self, A: np.ndarray, B: np.ndarray) -> int:
+        A = np.ascontiguousarray(A, dtype=np.uint32)
+        B = np.ascontiguousarray(B, dtype=np.uint32)
+        C = np.zeros(A.size + B.size, dtype=np.uint32)
+        jid = self._lib.gpupi_server_submit_mul(self.ptr, A.ctypes.data if A.size else None, A.size,
+                                                B.ctypes.data if B.size else None, B.size, C.ctypes.data)
+        if jid >= 0:
+            self.keep[jid] = (A, B, C)
+        return jid
+
+    def results(self, wait: bool = True, max_results: int = 256) -> list:
+        """[(GpupiJobResult, output)] of finished jobs; output is the digit
+        bytes (pi, if kept) or the product limbs (multiply)."""
+        arr = (GpupiJobResult * max_results)()
+        n = self._lib.gpupi_server_results(self.ptr, arr, max_results, 1 if wait else 0)
+        out = []
+        for r in arr[:n]:
+            k = self.keep.pop(r.id, None)
+            if r.kind == JOB_MUL:
+                out.append((r, k[2] if k is not None else None))
+            else:
+                out.append((r, k.raw if k is not None else None))
+        return out
+
+    def drain(self) -> None:
+        self._lib.gpupi_server_drain(self.ptr)
+
+    def close(self) -> None:
+        if self.ptr:
+            self._lib.gpupi_server_destroy(self.ptr)
+            self.ptr = None
 
 
 class LeafStage:

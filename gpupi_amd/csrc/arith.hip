@@ -2,6 +2,9 @@
 // (reference batch_arith.cu:18-424).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include <new>
 
 #include "common.hpp"
@@ -178,6 +181,66 @@ hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, Bat
     return gpupi::launch_mul_ntt(A.data, B.data, C.data, ctx->workspace, ctx->workspace_size_bytes,
                                  ctx->ntt_tables, C.batch_size, A.length, B.length, A.stride, B.stride,
                                  C.stride, ctx->stream);
+}
+
+hipError_t gpupi_mul_ragged(BatchMPContext *ctx, const GpupiMulItem *items, uint32_t count) {
+    if (ctx == nullptr || (count > 0 && items == nullptr)) return hipErrorInvalidValue;
+    // validate everything before launching anything
+    std::vector<std::pair<int, uint32_t>> big;  // (log2 K, item) of the NTT products
+    for (uint32_t i = 0; i < count; ++i) {
+        const GpupiMulItem &it = items[i];
+        const uint64_t total = (uint64_t)it.L_a + it.L_b;
+        if (total == 0) continue;
+        if (it.C == nullptr || total > 0xffffffffull || (it.L_a && it.A == nullptr) || (it.L_b && it.B == nullptr))
+            return hipErrorInvalidValue;
+        if (it.C == it.A || it.C == it.B) return hipErrorInvalidValue;
+        const bool small = it.L_a == 0 || it.L_b == 0 ||
+                           ((int64_t)total < ctx->ntt_min_limbs && gpupi::mul_naive_supports(it.L_a, it.L_b));
+        if (small) {
+            if (!gpupi::mul_naive_supports(it.L_a, it.L_b) && it.L_a && it.L_b) return hipErrorInvalidValue;
+        } else {
+            if (!gpupi::ntt_supports(it.L_a, it.L_b)) return hipErrorInvalidValue;
+            big.emplace_back(gpupi::ntt_transform_log2(it.L_a, it.L_b), i);
+        }
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        const GpupiMulItem &it = items[i];
+        const uint64_t total = (uint64_t)it.L_a + it.L_b;
+        if (total == 0) continue;
+        const bool small = it.L_a == 0 || it.L_b == 0 ||
+                           ((int64_t)total < ctx->ntt_min_limbs && gpupi::mul_naive_supports(it.L_a, it.L_b));
+        if (small)
+            GPUPI_RETURN_IF(gpupi::launch_mul_naive(it.A, it.B, it.C, 1, it.L_a, it.L_b, it.L_a, it.L_b,
+                                                    (uint32_t)total, ctx->stream));
+    }
+    if (big.empty()) return hipSuccess;
+    std::stable_sort(big.begin(), big.end(),
+                     [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) { return x.first < y.first; });
+    std::vector<GpupiMulItem> sorted(big.size());
+    for (size_t i = 0; i < big.size(); ++i) sorted[i] = items[big[i].second];
+    // workspace: the sorted descriptors, then the largest group's NTT scratch
+    const size_t desc_bytes = (sorted.size() * sizeof(GpupiMulItem) + 255) & ~(size_t)255;
+    size_t ntt_bytes = 0;
+    for (size_t g0 = 0; g0 < big.size();) {
+        size_t g1 = g0;
+        while (g1 < big.size() && big[g1].first == big[g0].first) ++g1;
+        ntt_bytes = std::max(ntt_bytes, gpupi::ntt_ragged_workspace_bytes((uint32_t)(g1 - g0), big[g0].first));
+        g0 = g1;
+    }
+    GPUPI_RETURN_IF(ensure(ctx, desc_bytes + ntt_bytes));
+    GpupiMulItem *desc = reinterpret_cast<GpupiMulItem *>(ctx->workspace);
+    char *ntt_ws = reinterpret_cast<char *>(ctx->workspace) + desc_bytes;
+    GPUPI_RETURN_IF(hipMemcpyAsync(desc, sorted.data(), sorted.size() * sizeof(GpupiMulItem), hipMemcpyHostToDevice,
+                                   ctx->stream));
+    GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+    for (size_t g0 = 0; g0 < big.size();) {
+        size_t g1 = g0;
+        while (g1 < big.size() && big[g1].first == big[g0].first) ++g1;
+        GPUPI_RETURN_IF(gpupi::launch_mul_ntt_ragged(desc + g0, sorted.data() + g0, (uint32_t)(g1 - g0), big[g0].first,
+                                                     ntt_ws, ntt_bytes, ctx->ntt_tables, ctx->stream));
+        g0 = g1;
+    }
+    return hipSuccess;
 }
 
 hipError_t batch_mp_mul_small(BatchMPContext *ctx, BatchMPArray A, uint32_t B, BatchMPArray C) {

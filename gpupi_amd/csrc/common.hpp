@@ -88,6 +88,11 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
                           uint32_t L_a, uint32_t L_b, uint32_t stride_A, uint32_t stride_B,
                           uint32_t stride_C, hipStream_t stream);
 size_t ntt_workspace_bytes(uint32_t N, uint32_t L_a, uint32_t L_b);
+int ntt_transform_log2(uint32_t L_a, uint32_t L_b);
+size_t ntt_ragged_workspace_bytes(uint32_t N, int log2K);
+hipError_t launch_mul_ntt_ragged(const GpupiMulItem *items, const GpupiMulItem *host_items, uint32_t N, int log2K,
+                                 void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
+                                 hipStream_t stream);
 bool ntt_supports(uint32_t L_a, uint32_t L_b);
 hipError_t ntt_tables_init(NTTPrecomputedTables *tables);
 void ntt_tables_release(NTTPrecomputedTables *tables);

@@ -247,6 +247,9 @@ struct PassArgs {
     uint32_t N;  // products in this launch
     Plan plan;
     const double *tables;
+    // ragged launches: per-product operands and lengths (device array), or
+    // null when every product uses A, B, C with the strides above
+    const GpupiMulItem *items;
 };
 
 __device__ __forceinline__ double *plane_of(const PassArgs &a, int op, int P) {
@@ -292,8 +295,16 @@ __global__ void __launch_bounds__(256) ntt_col_fwd_kernel(PassArgs a, int level)
     const ColThread<M> ct(a.plan, level, t);
     uint32_t xin[16];
     if (FROM_LIMBS) {
-        const uint32_t *X = (op ? a.B : a.A) + (size_t)prod * (op ? a.sB : a.sA);
-        const uint32_t LX = op ? a.Lb : a.La;
+        const uint32_t *X;
+        uint32_t LX;
+        if (a.items) {
+            const GpupiMulItem it = a.items[prod];
+            X = op ? it.B : it.A;
+            LX = op ? it.L_b : it.L_a;
+        } else {
+            X = (op ? a.B : a.A) + (size_t)prod * (op ? a.sB : a.sA);
+            LX = op ? a.Lb : a.La;
+        }
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const uint64_t o = ct.off(u);
@@ -473,8 +484,18 @@ __device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, doub
     for (int j = 0; j < 16; ++j) {
         const uint32_t i = i0 + j * kJ;
         const uint32_t prod = prod0 + (i >> M), r = i & ((1u << M) - 1u);
-        ra[j] = (prod < a.N && r < a.La) ? (double)a.A[(size_t)prod * a.sA + r] : 0.0;
-        v[j] = (prod < a.N && r < a.Lb) ? (double)a.B[(size_t)prod * a.sB + r] : 0.0;
+        ra[j] = 0.0;
+        v[j] = 0.0;
+        if (prod < a.N) {
+            if (a.items) {
+                const GpupiMulItem it = a.items[prod];
+                if (r < it.L_a) ra[j] = (double)it.A[r];
+                if (r < it.L_b) v[j] = (double)it.B[r];
+            } else {
+                if (r < a.La) ra[j] = (double)a.A[(size_t)prod * a.sA + r];
+                if (r < a.Lb) v[j] = (double)a.B[(size_t)prod * a.sB + r];
+            }
+        }
     }
     fwd_dft<M, false, kDifToPointA>(ra, lds, tab, P, m);
     if constexpr (M > 4) __syncthreads();
@@ -550,11 +571,17 @@ __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
     uint32_t ci = (uint32_t)(cm >> lane) & 1u;
     if (prod < a.N) {
         uint32_t *out = a.C + (size_t)prod * a.sC;
+        uint32_t Lc = a.Lc;
+        if (a.items) {
+            const GpupiMulItem it = a.items[prod];
+            out = it.C;
+            Lc = it.L_a + it.L_b;
+        }
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const uint64_t x = (uint64_t)limb[j] + ci;
             ci = (uint32_t)(x >> 32);
-            if (r0 + j < a.Lc) out[r0 + j] = (uint32_t)x;
+            if (r0 + j < Lc) out[r0 + j] = (uint32_t)x;
         }
     }
 }
@@ -735,36 +762,78 @@ void ntt_tables_release(NTTPrecomputedTables *tables) {
     tables->max_log2 = 0;
 }
 
-hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, void *workspace,
-                          size_t workspace_bytes, const NTTPrecomputedTables &tables, uint32_t N,
-                          uint32_t L_a, uint32_t L_b, uint32_t stride_A, uint32_t stride_B,
-                          uint32_t stride_C, hipStream_t stream) {
-    if (N == 0) return hipSuccess;
-    if (!ntt_supports(L_a, L_b) || tables.roots == nullptr) return hipErrorInvalidValue;
-    const int k = transform_log2(L_a, L_b);
+namespace {
+
+// Copies limbs [0, L_a + L_b) of each staged product row (stride K) to its
+// item's C.  Blocks: (item, 1024-limb slice).
+__global__ void __launch_bounds__(256) ragged_copy_kernel(const uint32_t *staging, uint64_t K,
+                                                          const GpupiMulItem *items, uint32_t slices) {
+    const uint32_t item = blockIdx.x / slices, slice = blockIdx.x % slices;
+    const GpupiMulItem it = items[item];
+    const uint32_t Lc = it.L_a + it.L_b;
+    const uint32_t *src = staging + (size_t)item * K;
+    for (uint32_t i = slice * 1024u + threadIdx.x; i < Lc && i < (slice + 1) * 1024u; i += 256u) it.C[i] = src[i];
+}
+
+// One batched NTT product launch sequence: N products of one transform size
+// 2^k, either uniform (A, B, C with strides) or ragged (per-product items,
+// `host_items` mirroring `items` for the byte accounting).
+struct NttJob {
+    const uint32_t *A = nullptr, *B = nullptr;
+    uint32_t *C = nullptr;
+    uint32_t La = 0, Lb = 0, sA = 0, sB = 0, sC = 0;
+    const GpupiMulItem *items = nullptr;
+    const GpupiMulItem *host_items = nullptr;
+    uint32_t N = 0;
+    int k = 0;
+};
+
+size_t job_workspace_bytes(uint32_t N, int k, bool ragged, uint32_t Lc) {
     const Plan p = make_plan(k);
-    const uint32_t Lc = L_a + L_b;
+    if (p.levels == 1) return 0;
+    const size_t nc = chunk_products(N, k);
     const size_t K = size_t(1) << k;
+    const size_t runs = K >> final_run_log2(p);
+    return 4 * align256(nc * K * sizeof(double)) + align256(nc * runs * sizeof(uint64_t)) +
+           align256(carry_flags_bytes((uint32_t)nc, ragged ? (uint32_t)K : Lc)) +
+           (ragged ? align256(nc * K * sizeof(uint32_t)) : 0);
+}
+
+hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
+                   hipStream_t stream) {
+    if (j.N == 0) return hipSuccess;
+    const bool ragged = j.items != nullptr;
+    const int k = j.k;
+    const Plan p = make_plan(k);
+    const size_t K = size_t(1) << k;
+    const uint32_t Lc = ragged ? (uint32_t)K : j.La + j.Lb;
+    auto limbs_in = [&](size_t r0, uint32_t n) {  // sum of L_a + L_b over products [r0, r0 + n)
+        if (!ragged) return (double)n * ((double)j.La + j.Lb);
+        double t = 0;
+        for (uint32_t i = 0; i < n; ++i) t += (double)j.host_items[r0 + i].L_a + j.host_items[r0 + i].L_b;
+        return t;
+    };
 
     PassArgs a{};
-    a.La = L_a;
-    a.Lb = L_b;
-    a.sA = stride_A;
-    a.sB = stride_B;
-    a.Lc = Lc;
-    a.sC = stride_C;
+    a.La = j.La;
+    a.Lb = j.Lb;
+    a.sA = j.sA;
+    a.sB = j.sB;
+    a.Lc = j.La + j.Lb;
+    a.sC = j.sC;
     a.plan = p;
     a.tables = tables.roots;
 
     if (p.levels == 1) {
         const uint32_t G = kTile >> k;
-        const uint64_t blocks = ceil_div64(N, G);
+        const uint64_t blocks = ceil_div64(j.N, G);
         if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-        a.A = A;
-        a.B = B;
-        a.C = C;
-        a.N = N;
-        const double bytes = 4.0 * N * ((double)L_a + L_b + Lc);
+        a.A = j.A;
+        a.B = j.B;
+        a.C = j.C;
+        a.N = j.N;
+        a.items = j.items;
+        const double bytes = 8.0 * limbs_in(0, j.N);
         return timed(kLocal, bytes, stream, [&] {
             dispatch_m<kLocalLo, kLocalHi>(k, [&](auto mc) {
                 hipLaunchKernelGGL((ntt_local_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256), 0,
@@ -772,8 +841,8 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
             });
         });
     }
-    if (workspace == nullptr || workspace_bytes < ntt_workspace_bytes(N, L_a, L_b)) return hipErrorInvalidValue;
-    const size_t nc = chunk_products(N, k);
+    if (workspace == nullptr || workspace_bytes < job_workspace_bytes(j.N, k, ragged, Lc)) return hipErrorInvalidValue;
+    const size_t nc = chunk_products(j.N, k);
     const size_t runs = K >> final_run_log2(p);
     const size_t plane_bytes = align256(nc * K * sizeof(double));
     char *ws = static_cast<char *>(workspace);
@@ -781,19 +850,31 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
     a.plane = plane_bytes / sizeof(double);
     a.spill = reinterpret_cast<uint64_t *>(ws + 4 * plane_bytes);
     uint8_t *flags = reinterpret_cast<uint8_t *>(ws + 4 * plane_bytes + align256(nc * runs * sizeof(uint64_t)));
+    uint32_t *staging = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(flags) +
+                                                     align256(carry_flags_bytes((uint32_t)nc, Lc)));
     const uint32_t tiles = tiles_per_product(p);
-    for (size_t r0 = 0; r0 < N; r0 += nc) {
-        const uint32_t n = (uint32_t)((N - r0) < nc ? (N - r0) : nc);
-        a.A = A + r0 * stride_A;
-        a.B = B + r0 * stride_B;
-        a.C = C + r0 * stride_C;
+    for (size_t r0 = 0; r0 < j.N; r0 += nc) {
+        const uint32_t n = (uint32_t)((j.N - r0) < nc ? (j.N - r0) : nc);
+        if (ragged) {
+            // limbs of the last level go to a staging row per product (stride
+            // K), the spill pass runs there, then each row moves to its C
+            a.items = j.items + r0;
+            a.C = staging;
+            a.sC = (uint32_t)K;
+            a.Lc = (uint32_t)K;
+        } else {
+            a.A = j.A + r0 * j.sA;
+            a.B = j.B + r0 * j.sB;
+            a.C = j.C + r0 * j.sC;
+        }
         a.N = n;
         const uint64_t fwd_blocks = (uint64_t)n * 2 * tiles, blocks = (uint64_t)n * tiles;
         if (fwd_blocks > 0x7fffffffull) return hipErrorInvalidValue;
         // algorithmic bytes: 16 B per residue pair (one binary64 per prime),
         // 4 B per limb
         const double tb = 16.0 * (double)n * (double)K;
-        GPUPI_RETURN_IF(timed(kColFwdLimbs, 4.0 * n * ((double)L_a + L_b) + 2 * tb, stream, [&] {
+        const double lin = limbs_in(r0, n);
+        GPUPI_RETURN_IF(timed(kColFwdLimbs, 4.0 * lin + 2 * tb, stream, [&] {
             dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
                 hipLaunchKernelGGL((ntt_col_fwd_kernel<true, decltype(mc)::value>), dim3((uint32_t)fwd_blocks),
                                    dim3(256), 0, stream, a, 0);
@@ -821,16 +902,62 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
                 });
             }));
         }
-        GPUPI_RETURN_IF(timed(kColInvLimbs, tb + 4.0 * n * Lc + 8.0 * n * runs, stream, [&] {
+        GPUPI_RETURN_IF(timed(kColInvLimbs, tb + 4.0 * n * (double)a.Lc + 8.0 * n * runs, stream, [&] {
             dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
                 hipLaunchKernelGGL((ntt_col_inv_limbs_kernel<decltype(mc)::value>), dim3((uint32_t)blocks),
                                    dim3(256), 0, stream, a);
             });
         }));
-        GPUPI_RETURN_IF(launch_add_spills(a.C, a.spill, n, Lc, stride_C, 1u << final_run_log2(p), (uint32_t)runs,
+        GPUPI_RETURN_IF(launch_add_spills(a.C, a.spill, n, a.Lc, a.sC, 1u << final_run_log2(p), (uint32_t)runs,
                                           flags, stream));
+        if (ragged) {
+            const uint32_t slices = (uint32_t)ceil_div64(K, 1024);
+            hipLaunchKernelGGL(ragged_copy_kernel, dim3(n * slices), dim3(256), 0, stream, staging, (uint64_t)K,
+                               a.items, slices);
+            GPUPI_RETURN_IF(hipGetLastError());
+        }
     }
     return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, void *workspace,
+                          size_t workspace_bytes, const NTTPrecomputedTables &tables, uint32_t N,
+                          uint32_t L_a, uint32_t L_b, uint32_t stride_A, uint32_t stride_B,
+                          uint32_t stride_C, hipStream_t stream) {
+    if (N == 0) return hipSuccess;
+    if (!ntt_supports(L_a, L_b) || tables.roots == nullptr) return hipErrorInvalidValue;
+    NttJob j;
+    j.A = A;
+    j.B = B;
+    j.C = C;
+    j.La = L_a;
+    j.Lb = L_b;
+    j.sA = stride_A;
+    j.sB = stride_B;
+    j.sC = stride_C;
+    j.N = N;
+    j.k = transform_log2(L_a, L_b);
+    return run_ntt(j, workspace, workspace_bytes, tables, stream);
+}
+
+int ntt_transform_log2(uint32_t L_a, uint32_t L_b) { return transform_log2(L_a, L_b); }
+
+size_t ntt_ragged_workspace_bytes(uint32_t N, int log2K) { return job_workspace_bytes(N, log2K, true, 0); }
+
+hipError_t launch_mul_ntt_ragged(const GpupiMulItem *items, const GpupiMulItem *host_items, uint32_t N, int log2K,
+                                 void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
+                                 hipStream_t stream) {
+    if (N == 0) return hipSuccess;
+    if (items == nullptr || host_items == nullptr || tables.roots == nullptr || log2K < kMinLog2 || log2K > kMaxLog2)
+        return hipErrorInvalidValue;
+    NttJob j;
+    j.items = items;
+    j.host_items = host_items;
+    j.N = N;
+    j.k = log2K;
+    return run_ntt(j, workspace, workspace_bytes, tables, stream);
 }
 
 }  // namespace gpupi

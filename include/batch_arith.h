@@ -63,6 +63,20 @@ hipError_t batch_mp_ensure_workspace(BatchMPContext *ctx, size_t required_bytes)
 /* reference batch_arith.h:45-53: C = A * B; C.length must equal
  * A.length + B.length.  Schoolbook or NTT is chosen internally. */
 hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, BatchMPArray C);
+/* Ragged batched product (no counterpart in the reference, whose batch API
+ * has one L_a/L_b per call): items[i].C = items[i].A * items[i].B with
+ * L_a + L_b limbs written, for pairs of any lengths.  Items are grouped by
+ * transform size and each group runs as one launch sequence; descriptors
+ * are read from host memory before the call returns.  C must not alias its
+ * own item's operands; distinct items' C must not overlap. */
+typedef struct GpupiMulItem {
+    const uint32_t *A;
+    const uint32_t *B;
+    uint32_t *C;
+    uint32_t L_a;
+    uint32_t L_b;
+} GpupiMulItem;
+hipError_t gpupi_mul_ragged(BatchMPContext *ctx, const GpupiMulItem *items, uint32_t count);
 /* reference batch_arith.h:55-63: C = A * B (single host limb); C != A */
 hipError_t batch_mp_mul_small(BatchMPContext *ctx, BatchMPArray A, uint32_t B, BatchMPArray C);
 /* reference batch_arith.h:65-72: A /= B in place; B odd, A.length <= 128 */

@@ -146,3 +146,55 @@ def test_ntt_three_level(ctx):
             assert np.array_equal(device_mul(ctx, A, B), oracle_mul(A, B))
     finally:
         g.check(ctx.set_ntt_threshold(384))
+
+
+def ragged_case(ctx, lengths, seed):
+    """Upload random pairs, run one ragged call, return (device C rows, host A, B)."""
+    rng = np.random.default_rng(seed)
+    bufs, items, As, Bs, Cs = [], [], [], [], []
+    for la, lb in lengths:
+        A = rand_rows(rng, 1, la) if la else np.zeros((1, 0), np.uint32)
+        B = rand_rows(rng, 1, lb) if lb else np.zeros((1, 0), np.uint32)
+        ba = g.DeviceBuffer.from_array(A.reshape(-1)) if la else None
+        bb = g.DeviceBuffer.from_array(B.reshape(-1)) if lb else None
+        bc = g.DeviceBuffer.from_array(np.full(la + lb + 1, PAD, dtype=np.uint32))
+        bufs += [ba, bb, bc]
+        items.append((ba.ptr if ba else None, bb.ptr if bb else None, bc.ptr, la, lb))
+        As.append(A)
+        Bs.append(B)
+        Cs.append(bc)
+    err = ctx.mul_ragged(items)
+    g.synchronize()
+    out = [c.download(np.uint32, count=la + lb + 1) for c, (la, lb) in zip(Cs, lengths)]
+    return err, out, As, Bs, bufs
+
+
+def test_ragged_products(ctx):
+    # one call with schoolbook, one-tile, column+row and three-level sizes,
+    # zero-length operands and repeated transform sizes
+    lengths = [(0, 0), (5, 0), (0, 7), (1, 1), (3, 17), (100, 90), (1000, 900), (2048, 2048), (2049, 1),
+               (3000, 3000), (5000, 1200), (4095, 1), (65536, 30000), (40000, 50000), (1 << 20, 1 << 20),
+               (2097153, 5), (12, 70000), (700, 700), (1000, 900)]
+    err, out, As, Bs, _ = ragged_case(ctx, lengths, 21)
+    assert err == 0
+    for (la, lb), C, A, B in zip(lengths, out, As, Bs):
+        assert C[-1] == PAD, (la, lb)
+        if la and lb:
+            assert np.array_equal(C[:-1], oracle_mul(A, B)[0]), (la, lb)
+        else:
+            assert not np.any(C[:-1]), (la, lb)
+
+
+def test_ragged_matches_uniform_call(ctx):
+    # the same pairs through batch_mp_mul one by one
+    lengths = [(3000, 2000), (777, 4000), (20000, 1), (9000, 9000)]
+    err, out, As, Bs, _ = ragged_case(ctx, lengths, 5)
+    assert err == 0
+    for (la, lb), C, A, B in zip(lengths, out, As, Bs):
+        assert np.array_equal(C[:-1], device_mul(ctx, A, B)[0])
+
+
+def test_ragged_rejects_aliasing(ctx):
+    buf = g.DeviceBuffer.from_array(np.ones(64, dtype=np.uint32))
+    assert ctx.mul_ragged([(buf.ptr, buf.ptr + 64, buf.ptr, 8, 8)]) != 0
+    assert ctx.mul_ragged([(None, buf.ptr, buf.ptr + 128, 8, 8)]) != 0

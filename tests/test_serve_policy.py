@@ -1,0 +1,42 @@
+"""CPU: the serving loop's admission rule and footprint estimate
+(gpupi_serve.h), called through the C ABI with no device work."""
+import gpupi_amd as g
+
+GB = 10 ** 9
+PI, MUL = g.JOB_PI, g.JOB_MUL
+
+
+def test_fcfs_within_budget():
+    # everything fits: admitted in arrival order
+    assert g.serve_admit([1 * GB, 2 * GB, 3 * GB], [MUL, PI, MUL], 200 * GB, 0, 4) == [0, 1, 2]
+
+
+def test_head_of_line_waits_for_memory():
+    # the head does not fit: nothing behind it may pass (no starvation of big jobs)
+    assert g.serve_admit([150 * GB, 1 * GB], [MUL, MUL], 200 * GB, 60 * GB, 4) == []
+    # it fits once memory is released
+    assert g.serve_admit([150 * GB, 1 * GB], [MUL, MUL], 200 * GB, 40 * GB, 4) == [0, 1]
+    # budget is consumed in order: the third job waits
+    assert g.serve_admit([90 * GB, 90 * GB, 30 * GB, 1 * GB], [MUL] * 4, 200 * GB, 0, 4) == [0, 1]
+
+
+def test_pi_without_worker_lets_later_jobs_pass():
+    assert g.serve_admit([1 * GB, 1 * GB, 1 * GB], [PI, MUL, PI], 200 * GB, 0, 0) == [1]
+    assert g.serve_admit([1 * GB, 1 * GB, 1 * GB, 1 * GB], [PI, PI, MUL, PI], 200 * GB, 0, 1) == [0, 2]
+
+
+def test_empty_queue():
+    assert g.serve_admit([], [], 200 * GB, 0, 4) == []
+
+
+def test_footprint_estimates():
+    lib = g.lib()
+    # multiply: operands + product + ragged NTT scratch, monotone in size
+    small = lib.gpupi_serve_job_bytes(MUL, 4096, 4096)
+    big = lib.gpupi_serve_job_bytes(MUL, 1 << 26, 1 << 26)
+    assert small < big
+    assert big >= 8 * (2 << 26) + 32 * (1 << 27)  # four planes of the 2^27 transform at least
+    assert big < 10 * GB
+    # pi: the largest job of the serving mix (10^8 digits) fits the budget many times over
+    assert lib.gpupi_serve_job_bytes(PI, 10 ** 8, 0) < 5 * GB
+    assert lib.gpupi_serve_job_bytes(PI, 10 ** 5, 0) < lib.gpupi_serve_job_bytes(PI, 10 ** 8, 0)
