@@ -283,6 +283,7 @@ def bench_serve(args, g) -> int:
 
     srv = g.Server(budget_bytes=200 * 10 ** 9)
     alone, service = {}, {}
+    print(f"serve: {n} jobs generated; running each alone", file=sys.stderr, flush=True)
     for i, (kind, x) in enumerate(jobs):
         jid = srv.submit_pi(x) if kind == "pi" else srv.submit_mul(*x)
         assert jid >= 0
@@ -292,6 +293,8 @@ def bench_serve(args, g) -> int:
         service[i] = r.finish_ms - r.start_ms
     mean_service = sum(service.values()) / n
     rate = 0.7 / (mean_service / 1e3)  # jobs per second
+    print(f"serve: alone done (mean service {mean_service:.1f} ms); mix at {rate:.1f} jobs/s", file=sys.stderr,
+          flush=True)
     arrivals = np.cumsum(gaps / rate)
 
     results, id2job = {}, {}
@@ -316,6 +319,7 @@ def bench_serve(args, g) -> int:
         id2job[jid] = i
     th.join()
     srv.close()
+    print("serve: mix done", file=sys.stderr, flush=True)
     lat = {"pi": [], "mul": []}
     same = True
     first_submit = min(r.submit_ms for r, _ in results.values())

@@ -110,8 +110,10 @@ class HostArray:
         return cls(np.zeros((n, length), dtype=np.uint32), stride, pad)
 
     def oarray(self, length: int | None = None, batch: int | None = None, stride: int | None = None) -> OArray:
-        return OArray(self.buf.ctypes.data, self.length if length is None else length,
-                      self.batch if batch is None else batch, self.stride if stride is None else stride)
+        o = OArray(self.buf.ctypes.data, self.length if length is None else length,
+                   self.batch if batch is None else batch, self.stride if stride is None else stride)
+        o._owner = self  # the struct points into self.buf: keep it alive as long as the struct
+        return o
 
     def rows(self, length: int | None = None) -> np.ndarray:
         length = self.length if length is None else length

@@ -40,3 +40,20 @@ def test_footprint_estimates():
     # pi: the largest job of the serving mix (10^8 digits) fits the budget many times over
     assert lib.gpupi_serve_job_bytes(PI, 10 ** 8, 0) < 5 * GB
     assert lib.gpupi_serve_job_bytes(PI, 10 ** 5, 0) < lib.gpupi_serve_job_bytes(PI, 10 ** 8, 0)
+
+
+def test_server_threading_simulation(tmp_path):
+    """serve.hip's threads, queues and admission run against host stand-ins
+    for HIP and the library (tests/native/serve_sim.cpp): random mixes and a
+    one-job-at-a-time budget; every job finishes once with its own result,
+    reservations never exceed the budget, no stall."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "serve_sim"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-x", "c++", "-I", os.path.join(root, "tests", "native", "stub"),
+                    "-I", os.path.join(root, "include"), os.path.join(root, "tests", "native", "serve_sim.cpp"),
+                    "-o", str(exe), "-pthread"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "serve sim: ok" in out.stdout
