@@ -51,6 +51,19 @@ def hex_preview(limbs) -> str:
     return f"{top}...{low}" if n > 4 else top
 
 
+def measured_traffic(kernel: str):
+    """Calibrated HBM bytes per launch of `kernel` at the default workload,
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
+    path = os.path.join(ROOT, "profiles", "pi_1e9_ntt_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "traffic" and parts[1] == kernel:
+            return float(parts[3])
+    return None
+
+
 def cpu_baseline(digits_small: int, digits_large: int, target: int) -> dict:
     """Times the oracle (single-threaded GMP restatement of the pipeline) on
     this host at two sizes and scales to the target with the measured growth
@@ -456,8 +469,15 @@ def main() -> int:
         achieved = total_bytes / (total_ms / 1e3) / 1e9
         roofline = {"bound": "hbm", "kernel": name, "launches": int(launches),
                     "avg_launch_ms": round(total_ms / launches, 4),
+                    "algorithmic_bytes_per_launch": round(total_bytes / launches),
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None}
+        if args.digits == 1_000_000_000 and conv == g.PREC_ROUNDED:
+            t = measured_traffic(name)
+            if t is not None:
+                roofline["traffic"] = t
+                roofline["traffic_source"] = ("profiles/pi_1e9_ntt_traffic.md: rocprofv3 FETCH_SIZE x2.000 + "
+                                              "WRITE_SIZE x1.000 bytes per launch (copy-kernel calibration)")
     ntt = {s[0]: {"launches": int(s[1]), "ms": round(s[2], 2), "GBps": round(s[3] / (s[2] / 1e3) / 1e9, 1)}
            for s in stats}
 
