@@ -354,7 +354,7 @@ class Context:
                      rhs=e.rhs_length, ms=e.elapsed_ms, ntt=bool(e.uses_ntt)) for e in arr[:n]]
 
 
-LEAF_PLAIN, LEAF_BATCHED, LEAF_GRAPH = 0, 1, 2
+LEAF_PLAIN, LEAF_BATCHED, LEAF_GRAPH, LEAF_FUSED = 0, 1, 2, 3
 JOB_PI, JOB_MUL = 0, 1
 
 
@@ -422,7 +422,7 @@ class Server:
 
 class LeafStage:
     """The 17-term group stage for `groups` groups in one of three forms
-    (LEAF_PLAIN, LEAF_BATCHED, LEAF_GRAPH); run() returns (P, Q, R) views."""
+    (LEAF_PLAIN, LEAF_BATCHED, LEAF_GRAPH, LEAF_FUSED); run() returns (P, Q, R) views."""
 
     def __init__(self, ctx: "Context", groups: int, mode: int):
         self._lib = lib()

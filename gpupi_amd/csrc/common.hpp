@@ -89,6 +89,8 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
                           uint32_t stride_C, hipStream_t stream);
 size_t ntt_workspace_bytes(uint32_t N, uint32_t L_a, uint32_t L_b);
 int ntt_transform_log2(uint32_t L_a, uint32_t L_b);
+// The fused 17-term group stage: N rows each of P (51 limbs), Q (68), R (83).
+hipError_t launch_group_stage(uint32_t N, uint32_t *P, uint32_t *Q, uint32_t *R, hipStream_t stream);
 size_t ntt_ragged_workspace_bytes(uint32_t N, int log2K);
 hipError_t launch_mul_ntt_ragged(const GpupiMulItem *items, const GpupiMulItem *host_items, uint32_t N, int log2K,
                                  void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,

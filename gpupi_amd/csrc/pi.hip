@@ -19,6 +19,7 @@
 #include "common.hpp"
 #include "convert_decimal.h"
 #include "gpupi_pi.h"
+#include "leaf_terms.hpp"
 
 namespace gpupi {
 hipError_t launch_extract_shifted(const BatchMPArray &src, BatchMPArray &dst, uint32_t rows, uint32_t shift_bits,
@@ -32,75 +33,12 @@ namespace {
 using gpupi::ceil_div;
 
 // ------------------------------------------------------------------ leaf
-__device__ __forceinline__ void reduce_pair(uint32_t &a, uint32_t &b, uint32_t d) {
-    if (a % d == 0u && b % d == 0u) {
-        a /= d;
-        b /= d;
-    }
-}
-
-__device__ __forceinline__ void reduce_3_5_23_29(uint32_t &a, uint32_t &b) {
-    reduce_pair(a, b, 3u);
-    reduce_pair(a, b, 5u);
-    reduce_pair(a, b, 23u);
-    reduce_pair(a, b, 29u);
-}
-
-template <int W>
-__device__ __forceinline__ void mul_word(uint32_t (&v)[W], uint32_t m) {
-    uint32_t carry = 0;
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-        const uint64_t t = (uint64_t)v[j] * m + carry;
-        v[j] = (uint32_t)t;
-        carry = (uint32_t)(t >> 32);
-    }
-}
-
-// Term i: p = (6i+1)(2i+1)(6i+5), q = (i+1)^3 * 10005^2 * 3335 (= (i+1)^3
-// 640320^3 / 24 / 2^15), r = (13591409 + 545140134 i) * q / 3335, with the
-// common factors of p and q cancelled as in pi_bs_gpu.cu:17-74.  Head rows
-// are terms 17t + 16, body rows terms t + t/16 (pi_bs_gpu.cu:40-46).
+// Head rows are terms 17t + 16, body rows terms t + t/16 (pi_bs_gpu.cu:40-46).
 __global__ void __launch_bounds__(256) leaf17_kernel(int is_head, uint32_t *P, uint32_t *Q, uint32_t *R,
                                                      uint32_t count) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += blockDim.x * gridDim.x) {
         const int64_t i = is_head ? (int64_t)t * 17 + 16 : (int64_t)t + (int64_t)(t >> 4);
-        uint32_t p1 = (uint32_t)(6 * i + 1), p2 = (uint32_t)(2 * i + 1), p3 = (uint32_t)(6 * i + 5);
-        uint32_t q1 = (uint32_t)(i + 1), q2 = q1, q3 = q1;
-        uint32_t q4 = 10005u, q5 = 10005u;
-        const uint32_t q6 = 3335u;
-        const uint64_t r1 = (uint64_t)(545140134ll * i + 13591409ll);
-        reduce_pair(p1, q1, 5u);
-        reduce_pair(p1, q2, 5u);
-        reduce_pair(p1, q3, 5u);
-        reduce_3_5_23_29(p1, q4);
-        reduce_3_5_23_29(p1, q5);
-        reduce_3_5_23_29(p2, q4);
-        reduce_3_5_23_29(p2, q5);
-        reduce_3_5_23_29(p3, q4);
-        reduce_3_5_23_29(p3, q5);
-        uint32_t pv[3] = {p1, 0, 0};
-        mul_word(pv, p2);
-        mul_word(pv, p3);
-        uint32_t qv[4] = {q1, 0, 0, 0};
-        mul_word(qv, q2);
-        mul_word(qv, q3);
-        mul_word(qv, q4);
-        mul_word(qv, q5);
-        mul_word(qv, q6);
-        uint32_t rv[6] = {(uint32_t)r1, (uint32_t)(r1 >> 32), 0, 0, 0, 0};
-        mul_word(rv, q1);
-        mul_word(rv, q2);
-        mul_word(rv, q3);
-        mul_word(rv, q4);
-        mul_word(rv, q5);
-        uint32_t *po = P + (size_t)t * 3, *qo = Q + (size_t)t * 4, *ro = R + (size_t)t * 6;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) po[j] = pv[j];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) qo[j] = qv[j];
-#pragma unroll
-        for (int j = 0; j < 6; ++j) ro[j] = rv[j];
+        gpupi::leaf_term(i, P + (size_t)t * 3, Q + (size_t)t * 4, R + (size_t)t * 6);
     }
 }
 
@@ -530,6 +468,21 @@ hipError_t leaf_stage(BatchMPContext *ctx, TreeBufs &t, uint32_t N, bool compact
     return head_merge(ctx, t, 2 * N, compacting, P0, Q0, R0, P, Q, R);
 }
 
+// The same stage as one fused kernel (rows of the bound lengths 51, 68, 83,
+// exactly the batched form's before compaction).
+hipError_t leaf_stage_fused(BatchMPContext *ctx, TreeBufs &t, uint32_t N, BatchMPArray &P, BatchMPArray &Q,
+                            BatchMPArray &R) {
+    GPUPI_RETURN_IF(t.bP.view(N, 51, P));
+    GPUPI_RETURN_IF(t.bQ.view(N, 68, Q));
+    GPUPI_RETURN_IF(t.bR.view(N, 83, R));
+    // the tree's next levels write into the partner buffers
+    GPUPI_RETURN_IF(t.bPn.reserve((size_t)N * 51));
+    GPUPI_RETURN_IF(t.bQn.reserve((size_t)N * 68));
+    GPUPI_RETURN_IF(t.bRn.reserve((size_t)N * 83));
+    return stage(ctx, "leaf", "fused_group_stage", N * 17u, N, 51, 0, false,
+                 [&] { return gpupi::launch_group_stage(N, P.data, Q.data, R.data, ctx->stream); });
+}
+
 }  // namespace
 
 // ------------------------------------------------ 17-term group stage
@@ -608,7 +561,7 @@ hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t N, BatchMPArray *Q_o
     BatchMPArray P, Q, R;
     // the 17-term group stage with bound lengths (no synchronization inside),
     // then one compaction of each array
-    GPUPI_RETURN_IF(leaf_stage(ctx, t, N, false, P, Q, R));
+    GPUPI_RETURN_IF(leaf_stage_fused(ctx, t, N, P, Q, R));
     GPUPI_RETURN_IF(compact(ctx, P, "P", 2 * N));
     GPUPI_RETURN_IF(compact(ctx, Q, "Q", 2 * N));
     GPUPI_RETURN_IF(compact(ctx, R, "R", 2 * N));
@@ -1078,7 +1031,7 @@ int gpupi_stage_profile_query(const BatchMPContext *ctx, GpupiStageEntry *out, i
 
 hipError_t gpupi_leaf_stage_create(BatchMPContext *ctx, uint32_t groups, int mode, GpupiLeafStage **out) {
     if (ctx == nullptr || out == nullptr || groups == 0 || (groups & (groups - 1)) != 0 ||
-        groups > (1u << 26) || mode < GPUPI_LEAF_PLAIN || mode > GPUPI_LEAF_GRAPH)
+        groups > (1u << 26) || mode < GPUPI_LEAF_PLAIN || mode > GPUPI_LEAF_FUSED)
         return hipErrorInvalidValue;
     *out = nullptr;
     GpupiLeafStage *s = new (std::nothrow) GpupiLeafStage();
@@ -1114,7 +1067,8 @@ hipError_t gpupi_leaf_stage_run(GpupiLeafStage *s, BatchMPArray *P, BatchMPArray
         }));
     } else {
         const bool plain = s->mode == GPUPI_LEAF_PLAIN;
-        GPUPI_RETURN_IF(leaf_stage(ctx, s->t, s->N, plain, p, q, r));
+        if (s->mode == GPUPI_LEAF_FUSED) GPUPI_RETURN_IF(leaf_stage_fused(ctx, s->t, s->N, p, q, r));
+        else GPUPI_RETURN_IF(leaf_stage(ctx, s->t, s->N, plain, p, q, r));
         if (!plain) {
             GPUPI_RETURN_IF(compact(ctx, p, "P", 2 * s->N));
             GPUPI_RETURN_IF(compact(ctx, q, "Q", 2 * s->N));

@@ -82,11 +82,13 @@ int gpupi_kernel_stats_query(GpupiKernelStat *out, int max_entries);
  *                       compaction per array at the end;
  *   GPUPI_LEAF_GRAPH    the batched form captured once as a hipGraph on the
  *                       stage's own stream (with its own scratch) and
- *                       replayed by every run.
+ *                       replayed by every run;
+ *   GPUPI_LEAF_FUSED    one kernel: each wave builds one group in LDS
+ *                       (leaf terms, levels, head merge, exact division).
  * No counterpart in the reference (its tree always runs the plain form).
  * run returns views of P, Q, R that stay valid until the next run or
  * destroy.  A GRAPH stage must not outlive its context. */
-enum { GPUPI_LEAF_PLAIN = 0, GPUPI_LEAF_BATCHED = 1, GPUPI_LEAF_GRAPH = 2 };
+enum { GPUPI_LEAF_PLAIN = 0, GPUPI_LEAF_BATCHED = 1, GPUPI_LEAF_GRAPH = 2, GPUPI_LEAF_FUSED = 3 };
 typedef struct GpupiLeafStage GpupiLeafStage;
 hipError_t gpupi_leaf_stage_create(BatchMPContext *ctx, uint32_t groups, int mode, GpupiLeafStage **out);
 hipError_t gpupi_leaf_stage_run(GpupiLeafStage *stage, BatchMPArray *P, BatchMPArray *Q, BatchMPArray *R);

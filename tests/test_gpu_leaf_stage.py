@@ -15,7 +15,7 @@ import oracle_ffi as of
 from helpers import rand_rows
 
 pytestmark = pytest.mark.gpu
-MODES = {"plain": g.LEAF_PLAIN, "batched": g.LEAF_BATCHED, "graph": g.LEAF_GRAPH}
+MODES = {"plain": g.LEAF_PLAIN, "batched": g.LEAF_BATCHED, "graph": g.LEAF_GRAPH, "fused": g.LEAF_FUSED}
 
 
 def oracle_stage(groups):
