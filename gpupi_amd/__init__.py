@@ -168,6 +168,9 @@ def _declare(lib):
         "batch_mul_ntt_workspace_size": (ctypes.c_size_t, [u32, u32, u32]),
         "gpupi_kernel_stats_reset": (None, [ctypes.c_int]),
         "gpupi_kernel_stats_query": (ctypes.c_int, [ctypes.POINTER(GpupiKernelStat), ctypes.c_int]),
+        "gpupi_kernel_stats_query_sizes": (ctypes.c_int, [ctypes.POINTER(GpupiKernelStat), ctypes.c_int]),
+        "gpupi_ntt_set_split": (err, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "gpupi_ntt_get_split": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "gpupi_stage_profile_enable": (err, [c_ctx, ctypes.c_int]),
         "gpupi_mul_ragged": (err, [c_ctx, ctypes.POINTER(GpupiMulItem), u32]),
         "gpupi_server_create": (ctypes.c_void_p, [ctypes.POINTER(GpupiServeConfig)]),
@@ -479,11 +482,29 @@ def kernel_stats_reset(enable: bool) -> None:
     lib().gpupi_kernel_stats_reset(1 if enable else 0)
 
 
-def kernel_stats() -> list:
+def kernel_stats(by_size: bool = False) -> list:
     """[(name, launches, total_ms, total_bytes)] for the NTT kernels timed
-    since the last kernel_stats_reset(True)."""
-    arr = (GpupiKernelStat * 16)()
-    n = lib().gpupi_kernel_stats_query(arr, 16)
+    since the last kernel_stats_reset(True); with by_size one entry per
+    (kernel, transform size), named "<kernel>@2^<log2 K>"."""
+    cap = 256
+    arr = (GpupiKernelStat * cap)()
+    fn = lib().gpupi_kernel_stats_query_sizes if by_size else lib().gpupi_kernel_stats_query
+    n = fn(arr, cap)
     if n < 0:
         raise RuntimeError("gpupi_kernel_stats_query failed")
     return [(arr[i].name.decode(), arr[i].launches, arr[i].total_ms, arr[i].total_bytes) for i in range(n)]
+
+
+def ntt_set_split(log2_k: int, log2_m0: int, log2_m1: int = 0) -> None:
+    """Column level sizes for transforms of 2^log2_k points (0 = built-in);
+    raises ValueError for a split the kernels do not cover."""
+    e = lib().gpupi_ntt_set_split(log2_k, log2_m0, log2_m1)
+    if e != 0:
+        raise ValueError(f"invalid NTT split 2^{log2_k} = 2^{log2_m0} x 2^{log2_m1} x rest (hip error {e})")
+
+
+def ntt_get_split(log2_k: int) -> tuple:
+    """log2 sizes of the levels used for transforms of 2^log2_k points."""
+    m = (ctypes.c_int * 3)()
+    n = lib().gpupi_ntt_get_split(log2_k, m)
+    return tuple(m[i] for i in range(n))

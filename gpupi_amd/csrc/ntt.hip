@@ -30,6 +30,7 @@
 // lookups in a two-level table of w_{2^30} powers.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdio>
 #include <type_traits>
 #include <vector>
@@ -640,7 +641,7 @@ const char *const kKindNames[kKindCount] = {"ntt_col_fwd_kernel<limbs>", "ntt_co
                                             "ntt_row_fused_kernel", "ntt_col_inv_kernel",
                                             "ntt_col_inv_limbs_kernel", "ntt_local_kernel"};
 struct StatRecord {
-    int kind;
+    int kind, log2K;
     hipEvent_t start, stop;
     double bytes;
 };
@@ -663,12 +664,12 @@ hipEvent_t stats_event() {
 
 // Records `launch` between two events when stats are enabled.
 template <typename F>
-hipError_t timed(int kind, double bytes, hipStream_t stream, F &&launch) {
+hipError_t timed(int kind, int log2K, double bytes, hipStream_t stream, F &&launch) {
     if (!g_stats.enabled) {
         launch();
         return hipGetLastError();
     }
-    StatRecord r{kind, stats_event(), stats_event(), bytes};
+    StatRecord r{kind, log2K, stats_event(), stats_event(), bytes};
     if (r.start == nullptr || r.stop == nullptr) return hipErrorOutOfMemory;
     GPUPI_RETURN_IF(hipEventRecord(r.start, stream));
     launch();
@@ -677,6 +678,20 @@ hipError_t timed(int kind, double bytes, hipStream_t stream, F &&launch) {
     g_stats.records.push_back(r);
     return hipSuccess;
 }
+
+// Level sizes chosen at run time per transform size (gpupi_ntt_set_split):
+// m0 | m1 << 8, zero for make_plan's choice.  Read once per product launch
+// sequence, so a change takes effect from the next multiply on.
+std::atomic<uint32_t> g_split[kMaxLog2 + 1];
+
+Plan active_plan(int k) {
+    const uint32_t s = g_split[k].load(std::memory_order_relaxed);
+    return make_plan_split(k, (int)(s & 0xffu), (int)(s >> 8));
+}
+
+// Spill words per product for any plan of size 2^k: runs are never shorter
+// than run_log2_for(kColMaxLog2) limbs.
+size_t spill_runs_bound(int k) { return (size_t(1) << k) >> run_log2_for(kColMaxLog2); }
 
 }  // namespace
 
@@ -687,12 +702,10 @@ bool ntt_supports(uint32_t L_a, uint32_t L_b) {
 size_t ntt_workspace_bytes(uint32_t N, uint32_t L_a, uint32_t L_b) {
     if (!ntt_supports(L_a, L_b) || N == 0) return 0;
     const int k = transform_log2(L_a, L_b);
-    const Plan p = make_plan(k);
-    if (p.levels == 1) return 0;
+    if (make_plan(k).levels == 1) return 0;
     const size_t nc = chunk_products(N, k);
     const size_t K = size_t(1) << k;
-    const size_t runs = K >> final_run_log2(p);
-    return 4 * align256(nc * K * sizeof(double)) + align256(nc * runs * sizeof(uint64_t)) +
+    return 4 * align256(nc * K * sizeof(double)) + align256(nc * spill_runs_bound(k) * sizeof(uint64_t)) +
            align256(carry_flags_bytes((uint32_t)nc, L_a + L_b));
 }
 
@@ -789,12 +802,10 @@ struct NttJob {
 };
 
 size_t job_workspace_bytes(uint32_t N, int k, bool ragged, uint32_t Lc) {
-    const Plan p = make_plan(k);
-    if (p.levels == 1) return 0;
+    if (make_plan(k).levels == 1) return 0;
     const size_t nc = chunk_products(N, k);
     const size_t K = size_t(1) << k;
-    const size_t runs = K >> final_run_log2(p);
-    return 4 * align256(nc * K * sizeof(double)) + align256(nc * runs * sizeof(uint64_t)) +
+    return 4 * align256(nc * K * sizeof(double)) + align256(nc * spill_runs_bound(k) * sizeof(uint64_t)) +
            align256(carry_flags_bytes((uint32_t)nc, ragged ? (uint32_t)K : Lc)) +
            (ragged ? align256(nc * K * sizeof(uint32_t)) : 0);
 }
@@ -804,7 +815,7 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
     if (j.N == 0) return hipSuccess;
     const bool ragged = j.items != nullptr;
     const int k = j.k;
-    const Plan p = make_plan(k);
+    const Plan p = active_plan(k);
     const size_t K = size_t(1) << k;
     const uint32_t Lc = ragged ? (uint32_t)K : j.La + j.Lb;
     auto limbs_in = [&](size_t r0, uint32_t n) {  // sum of L_a + L_b over products [r0, r0 + n)
@@ -834,7 +845,7 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
         a.N = j.N;
         a.items = j.items;
         const double bytes = 8.0 * limbs_in(0, j.N);
-        return timed(kLocal, bytes, stream, [&] {
+        return timed(kLocal, k, bytes, stream, [&] {
             dispatch_m<kLocalLo, kLocalHi>(k, [&](auto mc) {
                 hipLaunchKernelGGL((ntt_local_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256), 0,
                                    stream, a);
@@ -849,7 +860,8 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
     a.T = reinterpret_cast<double *>(ws);
     a.plane = plane_bytes / sizeof(double);
     a.spill = reinterpret_cast<uint64_t *>(ws + 4 * plane_bytes);
-    uint8_t *flags = reinterpret_cast<uint8_t *>(ws + 4 * plane_bytes + align256(nc * runs * sizeof(uint64_t)));
+    uint8_t *flags =
+        reinterpret_cast<uint8_t *>(ws + 4 * plane_bytes + align256(nc * spill_runs_bound(k) * sizeof(uint64_t)));
     uint32_t *staging = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(flags) +
                                                      align256(carry_flags_bytes((uint32_t)nc, Lc)));
     const uint32_t tiles = tiles_per_product(p);
@@ -874,35 +886,35 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
         // 4 B per limb
         const double tb = 16.0 * (double)n * (double)K;
         const double lin = limbs_in(r0, n);
-        GPUPI_RETURN_IF(timed(kColFwdLimbs, 4.0 * lin + 2 * tb, stream, [&] {
+        GPUPI_RETURN_IF(timed(kColFwdLimbs, k, 4.0 * lin + 2 * tb, stream, [&] {
             dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
                 hipLaunchKernelGGL((ntt_col_fwd_kernel<true, decltype(mc)::value>), dim3((uint32_t)fwd_blocks),
                                    dim3(256), 0, stream, a, 0);
             });
         }));
         if (p.levels == 3) {
-            GPUPI_RETURN_IF(timed(kColFwd, 4 * tb, stream, [&] {
+            GPUPI_RETURN_IF(timed(kColFwd, k, 4 * tb, stream, [&] {
                 dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
                     hipLaunchKernelGGL((ntt_col_fwd_kernel<false, decltype(mc)::value>), dim3((uint32_t)fwd_blocks),
                                        dim3(256), 0, stream, a, 1);
                 });
             }));
         }
-        GPUPI_RETURN_IF(timed(kRowFused, 3 * tb, stream, [&] {
+        GPUPI_RETURN_IF(timed(kRowFused, k, 3 * tb, stream, [&] {
             dispatch_m<kRowLo, kRowHi>(p.log2M[p.levels - 1], [&](auto mc) {
                 hipLaunchKernelGGL((ntt_row_fused_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256),
                                    0, stream, a);
             });
         }));
         if (p.levels == 3) {
-            GPUPI_RETURN_IF(timed(kColInv, 2 * tb, stream, [&] {
+            GPUPI_RETURN_IF(timed(kColInv, k, 2 * tb, stream, [&] {
                 dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
                     hipLaunchKernelGGL((ntt_col_inv_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256),
                                        0, stream, a, 1);
                 });
             }));
         }
-        GPUPI_RETURN_IF(timed(kColInvLimbs, tb + 4.0 * n * (double)a.Lc + 8.0 * n * runs, stream, [&] {
+        GPUPI_RETURN_IF(timed(kColInvLimbs, k, tb + 4.0 * n * (double)a.Lc + 8.0 * n * runs, stream, [&] {
             dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
                 hipLaunchKernelGGL((ntt_col_inv_limbs_kernel<decltype(mc)::value>), dim3((uint32_t)blocks),
                                    dim3(256), 0, stream, a);
@@ -1000,22 +1012,64 @@ void gpupi_kernel_stats_reset(int enable) {
     g_stats.enabled = enable != 0;
 }
 
-int gpupi_kernel_stats_query(GpupiKernelStat *out, int max_entries) {
+namespace {
+// Aggregates the records per kernel kind, or per (kind, log2 K).
+int stats_query(GpupiKernelStat *out, int max_entries, bool by_size) {
     using namespace gpupi;
-    GpupiKernelStat agg[kKindCount] = {};
-    for (int k = 0; k < kKindCount; ++k) std::snprintf(agg[k].name, sizeof(agg[k].name), "%s", kKindNames[k]);
+    constexpr int kSizes = ntt::kMaxLog2 + 1;
+    std::vector<GpupiKernelStat> agg((size_t)kKindCount * kSizes);
     for (auto &r : g_stats.records) {
         if (hipEventSynchronize(r.stop) != hipSuccess) return -1;
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, r.start, r.stop) != hipSuccess) return -1;
-        agg[r.kind].launches += 1;
-        agg[r.kind].total_ms += ms;
-        agg[r.kind].total_bytes += r.bytes;
+        GpupiKernelStat &a = agg[(size_t)r.kind * kSizes + (by_size ? r.log2K : 0)];
+        a.launches += 1;
+        a.total_ms += ms;
+        a.total_bytes += r.bytes;
     }
     int n = 0;
-    for (int k = 0; k < kKindCount && n < max_entries; ++k)
-        if (agg[k].launches > 0) out[n++] = agg[k];
+    for (int k = 0; k < kKindCount; ++k)
+        for (int s = 0; s < kSizes; ++s) {
+            GpupiKernelStat a = agg[(size_t)k * kSizes + s];
+            if (a.launches == 0) continue;
+            if (n == max_entries) return n;
+            if (by_size) std::snprintf(a.name, sizeof(a.name), "%s@2^%d", kKindNames[k], s);
+            else std::snprintf(a.name, sizeof(a.name), "%s", kKindNames[k]);
+            out[n++] = a;
+        }
     return n;
+}
+}  // namespace
+
+int gpupi_kernel_stats_query(GpupiKernelStat *out, int max_entries) { return stats_query(out, max_entries, false); }
+
+int gpupi_kernel_stats_query_sizes(GpupiKernelStat *out, int max_entries) {
+    return stats_query(out, max_entries, true);
+}
+
+hipError_t gpupi_ntt_set_split(int log2_k, int log2_m0, int log2_m1) {
+    using namespace gpupi;
+    using namespace gpupi::ntt;
+    if (log2_k < kMinLog2 || log2_k > kMaxLog2 || log2_m0 < 0 || log2_m1 < 0 || log2_m0 > 255 || log2_m1 > 255)
+        return hipErrorInvalidValue;
+    if (log2_m0 == 0) {
+        if (log2_m1 != 0) return hipErrorInvalidValue;
+        g_split[log2_k].store(0);
+        return hipSuccess;
+    }
+    const Plan p = make_plan_split(log2_k, log2_m0, log2_m1);
+    if (p.levels == 1 || !plan_ok(p) || (p.levels == 2 && log2_m1 != 0)) return hipErrorInvalidValue;
+    g_split[log2_k].store((uint32_t)log2_m0 | (uint32_t)log2_m1 << 8);
+    return hipSuccess;
+}
+
+int gpupi_ntt_get_split(int log2_k, int *log2_m) {
+    using namespace gpupi;
+    using namespace gpupi::ntt;
+    if (log2_k < kMinLog2 || log2_k > kMaxLog2 || log2_m == nullptr) return 0;
+    const Plan p = active_plan(log2_k);
+    for (int l = 0; l < kMaxLevels; ++l) log2_m[l] = l < p.levels ? p.log2M[l] : 0;
+    return p.levels;
 }
 
 }  // extern "C"

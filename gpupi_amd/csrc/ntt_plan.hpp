@@ -72,6 +72,41 @@ GPUPI_HD Plan make_plan(int log2K) {
     return p;
 }
 
+// Level sizes the kernels are instantiated for: column levels 2^5..2^9,
+// the contiguous level 2^6..2^12 (local products 2^4..2^12).
+constexpr int kColMinLog2 = 5, kRowMinLog2 = 6;
+
+// A plan with the same number of levels as make_plan(log2K) and explicit
+// column sizes: log2M0 for the first level and, with three levels, log2M1
+// for the second (the contiguous level takes the rest).  Zero keeps
+// make_plan's choice.  Check the result with plan_ok.
+GPUPI_HD Plan make_plan_split(int log2K, int log2M0, int log2M1) {
+    Plan p = make_plan(log2K);
+    if (p.levels == 1 || log2M0 == 0) return p;
+    p.log2M[0] = log2M0;
+    if (p.levels == 2) {
+        p.log2M[1] = log2K - log2M0;
+    } else {
+        p.log2M[1] = log2M1 != 0 ? log2M1 : log2K - kTileLog2 - log2M0;
+        p.log2M[2] = log2K - log2M0 - p.log2M[1];
+    }
+    return p;
+}
+
+// Whether every level of p has a kernel instantiation and the sizes add up.
+GPUPI_HD bool plan_ok(const Plan &p) {
+    if (p.log2K < kMinLog2 || p.log2K > kMaxLog2) return false;
+    if (p.levels != make_plan(p.log2K).levels) return false;
+    if (p.levels == 1) return p.log2M[0] == p.log2K;
+    int sum = 0;
+    for (int l = 0; l < p.levels; ++l) sum += p.log2M[l];
+    if (sum != p.log2K) return false;
+    for (int l = 0; l + 1 < p.levels; ++l)
+        if (p.log2M[l] < kColMinLog2 || p.log2M[l] > kColMaxLog2) return false;
+    const int last = p.log2M[p.levels - 1];
+    return last >= kRowMinLog2 && last <= kTileLog2;
+}
+
 // log2 of the slab size K_l at level l
 GPUPI_HD int slab_log2(const Plan &p, int level) {
     int s = 0;

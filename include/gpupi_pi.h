@@ -71,6 +71,25 @@ typedef struct GpupiKernelStat {
 } GpupiKernelStat;
 void gpupi_kernel_stats_reset(int enable);
 int gpupi_kernel_stats_query(GpupiKernelStat *out, int max_entries);
+/* The same records split by transform size: one entry per (kernel, log2 K)
+ * pair, named "<kernel>@2^<log2 K>". */
+int gpupi_kernel_stats_query_sizes(GpupiKernelStat *out, int max_entries);
+
+/* NTT plan tuning (no reference counterpart; the reference hard-codes its
+ * pass schedules, batch_mul_ntt.cu:1810-1915).  A transform of 2^k points
+ * runs as one local level (k <= 12), a column level and a contiguous level
+ * (13 <= k <= 21), or two column levels and a contiguous level (k >= 22).
+ * set_split picks the column sizes for one k: log2_m0 for the first level
+ * and, with three levels, log2_m1 for the second (0 = the rest after a
+ * 4096-point contiguous level).  Column levels take 2^5..2^9 points, the
+ * contiguous level 2^6..2^12.  log2_m0 = 0 restores the built-in choice.
+ * Returns hipErrorInvalidValue for a split outside those limits.  Results
+ * are bit-identical for every split; only speed changes.  The workspace
+ * size does not depend on the split.  get_split writes the level sizes
+ * (log2) in use for k to log2_m[0..3) and returns the number of levels
+ * (0 for an unsupported k). */
+hipError_t gpupi_ntt_set_split(int log2_k, int log2_m0, int log2_m1);
+int gpupi_ntt_get_split(int log2_k, int *log2_m);
 
 /* The 17-term group stage on its own (pi_bs_gpu.cu:555-779) for `groups`
  * groups (a power of two), in three forms that leave the same P, Q, R rows

@@ -71,8 +71,8 @@ static void check_bitrev_split() {
                 EXPECT(bitrev(r0 + (u << (lm - 4)), lm) == bitrev(r0, lm) + bitrev(u, 4), "bitrev split lm=%d", lm);
 }
 
-static void check_plan(int k, bool full) {
-    const Plan p = make_plan(k);
+static void check_plan(const Plan &p, bool full) {
+    const int k = p.log2K;
     const uint64_t K = 1ull << k;
     int sum = 0;
     for (int l = 0; l < p.levels; ++l) sum += p.log2M[l];
@@ -164,7 +164,22 @@ int main() {
     check_tile_steps();
     check_xcd_tile();
     check_bitrev_split();
-    for (int k = kMinLog2; k <= kMaxLog2; ++k) check_plan(k, k <= 23);
+    // the built-in plans, then every split the run-time table accepts
+    int splits = 0;
+    for (int k = kMinLog2; k <= kMaxLog2; ++k) {
+        const Plan d = make_plan(k);
+        EXPECT(plan_ok(d), "default plan k=%d", k);
+        check_plan(d, k <= 23);
+        for (int m0 = 1; m0 <= 12; ++m0)
+            for (int m1 = 0; m1 <= 12; ++m1) {
+                const Plan p = make_plan_split(k, m0, m1);
+                if (p.levels == 1 || (p.levels == 2 && m1 != 0) || !plan_ok(p)) continue;
+                ++splits;
+                check_plan(p, k <= 21);
+            }
+    }
+    EXPECT(splits > 60, "split count %d", splits);
+    printf("plans with explicit splits checked: %d\n", splits);
     printf(fails ? "index check: %d failures\n" : "index check: ok\n", fails);
     return fails ? 1 : 0;
 }

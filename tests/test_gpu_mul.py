@@ -134,6 +134,46 @@ def test_ntt_every_transform_size(ctx, k):
         g.check(ctx.set_ntt_threshold(384))
 
 
+def valid_splits(k):
+    # every column split the kernels cover: (m0, m1) with m1 = 0 for two levels
+    out = []
+    for m0 in range(5, 10):
+        for m1 in ([0] if k <= 21 else range(5, 10)):
+            rest = k - m0 - m1
+            if 6 <= rest <= 12:
+                out.append((m0, m1))
+    return out
+
+
+SPLIT_K = [13, 16, 18, 20, 22, 23, 24]
+
+
+@pytest.mark.parametrize("k", SPLIT_K, ids=[f"K2^{k}" for k in SPLIT_K])
+def test_ntt_every_split(ctx, k):
+    # the same product under every level split: bit-identical to the
+    # built-in plan's product and to GMP
+    g.check(ctx.set_ntt_threshold(2))
+    default = g.ntt_get_split(k)
+    try:
+        n, la, lb = every_size_case(k)
+        n = min(n, 4)
+        rng = np.random.default_rng(100 + k)
+        A = rand_rows(rng, n, la)
+        B = rand_rows(rng, n, lb)
+        want = oracle_mul(A, B)
+        splits = valid_splits(k)
+        assert len(splits) >= 2
+        for m0, m1 in splits:
+            g.ntt_set_split(k, m0, m1)
+            levels = g.ntt_get_split(k)
+            assert levels[0] == m0 and sum(levels) == k and (m1 == 0 or levels[1] == m1)
+            assert np.array_equal(device_mul(ctx, A, B), want), (m0, m1)
+    finally:
+        g.ntt_set_split(k, 0)
+        g.check(ctx.set_ntt_threshold(384))
+    assert g.ntt_get_split(k) == default
+
+
 @pytest.mark.slow
 def test_ntt_three_level(ctx):
     # K = 2^23 and 2^25: two column levels + the contiguous level
