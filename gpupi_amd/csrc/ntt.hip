@@ -200,19 +200,24 @@ __device__ __forceinline__ void inv_dft(double *v, double *lds, const double *ta
     }
 }
 
-// w_{2^30}^E (or its inverse) from the two-level table; E < 2^30.
-__device__ __forceinline__ double big_twiddle(const double *tab, uint32_t E, bool inv, Mod m) {
-    const double lo = tab[(inv ? kOffLoInv : kOffLo) + (E & 32767u)];
-    const double hi = tab[(inv ? kOffHiInv : kOffHi) + (E >> 15)];
-    return mulp(lo, hi, m);
+// Table entries for one thread's column twiddles: w^(cg b0) and w^cg as
+// lo/hi pairs of the two-level table (w_{2^30}^E = lo[E mod 2^15] *
+// hi[E >> 15], E < 2^30), loaded early (they depend only on the tile geometry) so their
+// latency overlaps the data loads.
+struct ColTw {
+    double lb, hb, ls, hs;
+};
+__device__ __forceinline__ ColTw col_tw_load(const double *tab, uint32_t cg, uint32_t b0, bool inv) {
+    const double *lo = tab + (inv ? kOffLoInv : kOffLo), *hi = tab + (inv ? kOffHiInv : kOffHi);
+    const uint32_t eb = cg * b0;
+    return ColTw{lo[eb & 32767u], hi[eb >> 15], lo[cg & 32767u], hi[cg >> 15]};
 }
 
 // Multiplies x[u] (u < 16) by w^(cg * (b0 + bitrev(u, 4))): base = w^(cg b0),
 // step = w^cg, and two interleaved chains base * step^v.
-__device__ __forceinline__ void column_twiddles(double *x, const double *tab, uint32_t cg, uint32_t b0, bool inv,
-                                                Mod m) {
-    const double base = big_twiddle(tab, cg * b0, inv, m);
-    const double step = big_twiddle(tab, cg, inv, m);
+__device__ __forceinline__ void column_twiddles(double *x, const ColTw &tw, Mod m) {
+    const double base = mulp(tw.lb, tw.hb, m);
+    const double step = mulp(tw.ls, tw.hs, m);
     const double step2 = mulp(step, step, m);
     double c0 = base, c1 = mulp(base, step, m);
 #pragma unroll
@@ -281,9 +286,32 @@ struct ColThread {
     __device__ __forceinline__ uint64_t off(int u) const { return off0 + ((uint64_t)u << ustep); }
 };
 
+// One prime of a forward column level on the thread's step-0 elements v:
+// DFT down the columns, then the inter-level twiddle; v is left holding the
+// elements to store at the same offsets.  Ends with a barrier, so the next
+// prime may reuse `lds`.
+template <int M, int P>
+__device__ __forceinline__ void col_fwd_prime(double *v, double *lds, const ColThread<M> &ct, const double *tables,
+                                              const ColTw &tw) {
+    const Mod m = mod_of(P);
+    const double *tab = tables + P * kPrimeTable;
+    fwd_dft<M, true, kDifToTwiddle>(v, lds, tab, P, m);
+    lds_store(lds, v, StepMap<M, step_count(M) - 1, true>());
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = lds[ct.lds0 + 272u * u];
+    column_twiddles(v, tw, m);
+    __syncthreads();
+}
+
 // Forward column level: blocks cover (product, operand, tile); both primes
 // pass through the tile in turn; the inter-level twiddle is applied on
-// store.  Level 0 reads limbs.
+// store.  Level 0 reads limbs.  At level 0 every global load (the limbs and
+// both primes' column twiddles) is issued before the first store and both
+// primes' results are stored at the end: loads and stores share one
+// in-order counter, so a load issued after a store also waits for the
+// store.  Above level 0 the second prime's data is loaded after the first
+// prime's store, which keeps the kernel within 128 VGPRs.
 template <bool FROM_LIMBS, int M>
 __global__ void __launch_bounds__(256) ntt_col_fwd_kernel(PassArgs a, int level) {
     __shared__ double lds[kLdsSlots];
@@ -294,7 +322,9 @@ __global__ void __launch_bounds__(256) ntt_col_fwd_kernel(PassArgs a, int level)
     const uint32_t prod = bid / (2u * tiles);
     const uint64_t K = 1ull << a.plan.log2K;
     const ColThread<M> ct(a.plan, level, t);
-    uint32_t xin[16];
+    double *T0 = plane_of(a, (int)op, 0) + (size_t)prod * K;
+    double *T1 = plane_of(a, (int)op, 1) + (size_t)prod * K;
+    double v0[16], v1[16];
     if (FROM_LIMBS) {
         const uint32_t *X;
         uint32_t LX;
@@ -309,27 +339,32 @@ __global__ void __launch_bounds__(256) ntt_col_fwd_kernel(PassArgs a, int level)
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const uint64_t o = ct.off(u);
-            xin[u] = o < LX ? X[o] : 0u;
+            const uint32_t x = o < LX ? X[o] : 0u;
+            v0[u] = (double)x;
+            v1[u] = (double)x;
         }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v0[u] = T0[ct.off(u)];
     }
-#pragma clang loop unroll(disable)
-    for (int P = 0; P < 2; ++P) {
-        const Mod m = mod_of(P);
-        const double *tab = a.tables + P * kPrimeTable;
-        double *T = plane_of(a, (int)op, P) + (size_t)prod * K;
-        double v[16];
+    const ColTw tw0 = col_tw_load(a.tables, ct.cg, ct.b0, false);
+    const ColTw tw1 = col_tw_load(a.tables + kPrimeTable, ct.cg, ct.b0, false);
+    col_fwd_prime<M, 0>(v0, lds, ct, a.tables, tw0);
+    if (!FROM_LIMBS) {
+        // register budget (occupancy 4 needs <= 128 VGPRs): prime 0 goes out
+        // before prime 1 is loaded and transformed
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = FROM_LIMBS ? (double)xin[u] : T[ct.off(u)];
-        fwd_dft<M, true, kDifToTwiddle>(v, lds, tab, P, m);
-        lds_store(lds, v, StepMap<M, step_count(M) - 1, true>());
-        __syncthreads();
+        for (int u = 0; u < 16; ++u) T0[ct.off(u)] = v0[u];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = lds[ct.lds0 + 272u * u];
-        column_twiddles(v, tab, ct.cg, ct.b0, false, m);
-#pragma unroll
-        for (int u = 0; u < 16; ++u) T[ct.off(u)] = v[u];
-        __syncthreads();
+        for (int u = 0; u < 16; ++u) v1[u] = T1[ct.off(u)];
     }
+    col_fwd_prime<M, 1>(v1, lds, ct, a.tables, tw1);
+    if (FROM_LIMBS) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) T0[ct.off(u)] = v0[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) T1[ct.off(u)] = v1[u];
 }
 
 // Step-0 elements of a row tile (DFT size 2^M): v[j] is tile index
@@ -374,18 +409,16 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     }
 }
 
-// Inverse column level of prime P: inverse twiddle on load, inverse DFT;
-// returns the thread's elements tid + 256 u in v.
-template <int M>
-__device__ __forceinline__ void col_inv_prime(double *v, const PassArgs &a, double *lds, const ColThread<M> &ct,
-                                              uint32_t prod, int P) {
+// Inverse column level of prime P on the thread's elements v (as loaded):
+// inverse twiddle, inverse DFT; leaves the thread's elements tid + 256 u in
+// v.  The caller must make sure no thread still reads `lds`.
+template <int M, int P>
+__device__ __forceinline__ void col_inv_prime(double *v, double *lds, const ColThread<M> &ct, const double *tables,
+                                              const ColTw &tw) {
     constexpr int kLast = step_count(M) - 1;
     const Mod m = mod_of(P);
-    const double *tab = a.tables + P * kPrimeTable;
-    const double *T = plane_of(a, 0, P) + (size_t)prod * (1ull << a.plan.log2K);
-#pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = T[ct.off(u)];
-    column_twiddles(v, tab, ct.cg, ct.b0, true, m);
+    const double *tab = tables + P * kPrimeTable;
+    column_twiddles(v, tw, m);
 #pragma unroll
     for (int u = 0; u < 16; ++u) lds[ct.lds0 + 272u * u] = v[u];
     __syncthreads();
@@ -395,7 +428,23 @@ __device__ __forceinline__ void col_inv_prime(double *v, const PassArgs &a, doub
     step_regs<M, 0, true, true, kDitLast>(v, s0, tab, P, m);
 }
 
-// Inverse column level above the first: both primes through one tile.
+// Loads both primes' elements of a column tile and their inverse column
+// twiddle table entries (every load before any store).
+template <int M>
+__device__ __forceinline__ void col_inv_load(double *v0, double *v1, ColTw &tw0, ColTw &tw1, const PassArgs &a,
+                                             const ColThread<M> &ct, uint32_t prod) {
+    const size_t base = (size_t)prod * (1ull << a.plan.log2K);
+    const double *T0 = plane_of(a, 0, 0) + base, *T1 = plane_of(a, 0, 1) + base;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v0[u] = T0[ct.off(u)];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v1[u] = T1[ct.off(u)];
+    tw0 = col_tw_load(a.tables, ct.cg, ct.b0, true);
+    tw1 = col_tw_load(a.tables + kPrimeTable, ct.cg, ct.b0, true);
+}
+
+// Inverse column level above the first: both primes through one tile,
+// stored together at the end.
 template <int M>
 __global__ void __launch_bounds__(256) ntt_col_inv_kernel(PassArgs a, int level) {
     __shared__ double lds[kLdsSlots];
@@ -404,15 +453,18 @@ __global__ void __launch_bounds__(256) ntt_col_inv_kernel(PassArgs a, int level)
     const uint32_t t = bid % tiles;
     const uint32_t prod = bid / tiles;
     const ColThread<M> ct(a.plan, level, t);
-#pragma clang loop unroll(disable)
-    for (int P = 0; P < 2; ++P) {
-        double v[16];
-        col_inv_prime<M>(v, a, lds, ct, prod, P);
-        double *T = plane_of(a, 0, P) + (size_t)prod * (1ull << a.plan.log2K);
+    double v0[16], v1[16];
+    ColTw tw0, tw1;
+    col_inv_load<M>(v0, v1, tw0, tw1, a, ct, prod);
+    col_inv_prime<M, 0>(v0, lds, ct, a.tables, tw0);
+    __syncthreads();
+    col_inv_prime<M, 1>(v1, lds, ct, a.tables, tw1);
+    const size_t base = (size_t)prod * (1ull << a.plan.log2K);
+    double *T0 = plane_of(a, 0, 0) + base, *T1 = plane_of(a, 0, 1) + base;
 #pragma unroll
-        for (int u = 0; u < 16; ++u) T[ct.off(u)] = v[u];
-        __syncthreads();
-    }
+    for (int u = 0; u < 16; ++u) T0[ct.off(u)] = v0[u];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) T1[ct.off(u)] = v1[u];
 }
 
 // First inverse column level: both primes (two tiles), 1/K, Garner, limbs of
@@ -431,15 +483,17 @@ __global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
     const int log2K = a.plan.log2K;
     const ColThread<M> ct(a.plan, 0, t);
     {
-        double v[16];
-        col_inv_prime<M>(v, a, l0, ct, prod, 0);
+        double v0[16], v1[16];
+        ColTw tw0, tw1;
+        col_inv_load<M>(v0, v1, tw0, tw1, a, ct, prod);
+        col_inv_prime<M, 0>(v0, l0, ct, a.tables, tw0);
         const double kinv0 = c_ntt.kinv[0][log2K];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) l0[ct.lds0 + 272u * u] = mulp(v[u], kinv0, mod_of(0));
-        col_inv_prime<M>(v, a, l1, ct, prod, 1);
+        for (int u = 0; u < 16; ++u) l0[ct.lds0 + 272u * u] = mulp(v0[u], kinv0, mod_of(0));
+        col_inv_prime<M, 1>(v1, l1, ct, a.tables, tw1);
         const double kinv1 = c_ntt.kinv[1][log2K];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) l1[ct.lds0 + 272u * u] = mulp(v[u], kinv1, mod_of(1));
+        for (int u = 0; u < 16; ++u) l1[ct.lds0 + 272u * u] = mulp(v1[u], kinv1, mod_of(1));
     }
     __syncthreads();
     const uint32_t col0 = col_tile(a.plan, 0, t).col0;

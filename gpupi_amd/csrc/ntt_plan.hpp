@@ -46,28 +46,30 @@ struct Plan {
     int log2M[kMaxLevels];   // per level, first level first
 };
 
-// k <= 12: one local level.  k <= 21: column level of ceil(k/2) (<= 9) and
-// a contiguous level.  Otherwise two column levels and a 4096-point
-// contiguous level.
+// k <= 12: one local level.  k <= 21: a column level and a contiguous
+// level.  Otherwise two column levels and a contiguous level.  The column
+// sizes are the fastest measured on MI355X for every k (batched products of
+// 2^28 coefficients in all, every split the kernels cover): first levels of
+// 2^8 points (256-byte runs per row), and contiguous levels of 2^8..2^12.
+// k = 29, 30 (not measured) keep a 4096-point contiguous level.
 GPUPI_HD Plan make_plan(int log2K) {
+    //                          k: 13 14 15 16 17 18 19 20 21 22 23 24 25 26 27 28 29 30
+    constexpr int8_t kM0[18] = {5, 6, 8, 8, 8, 8, 8, 8, 9, 8, 8, 8, 8, 7, 8, 8, 8, 9};
+    constexpr int8_t kM1[18] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 6, 7, 8, 6, 8, 8, 8, 9, 9};
     Plan p{};
     p.log2K = log2K;
     if (log2K <= kTileLog2) {
         p.levels = 1;
         p.log2M[0] = log2K;
     } else if (log2K <= kTileLog2 + kColMaxLog2) {
-        int c = (log2K + 1) / 2;
-        if (c > kColMaxLog2) c = kColMaxLog2;
-        if (log2K - c > kTileLog2) c = log2K - kTileLog2;
         p.levels = 2;
-        p.log2M[0] = c;
-        p.log2M[1] = log2K - c;
+        p.log2M[0] = kM0[log2K - 13];
+        p.log2M[1] = log2K - p.log2M[0];
     } else {
-        const int rest = log2K - kTileLog2;  // 10..18
         p.levels = 3;
-        p.log2M[0] = (rest + 1) / 2;
-        p.log2M[1] = rest - p.log2M[0];
-        p.log2M[2] = kTileLog2;
+        p.log2M[0] = kM0[log2K - 13];
+        p.log2M[1] = kM1[log2K - 13];
+        p.log2M[2] = log2K - p.log2M[0] - p.log2M[1];
     }
     return p;
 }

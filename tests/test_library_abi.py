@@ -67,18 +67,18 @@ def test_ntt_split_table_validates_and_restores():
         return tuple(m[i] for i in range(n))
 
     assert levels(10) == (10,)
-    assert levels(20) == (9, 11)
+    assert levels(20) == (8, 12)
     assert levels(28) == (8, 8, 12)
     assert levels(31) == () and levels(3) == ()
     # two levels: column 2^5..2^9 and contiguous 2^6..2^12
-    assert lib.gpupi_ntt_set_split(20, 8, 0) == 0 and levels(20) == (8, 12)
+    assert lib.gpupi_ntt_set_split(20, 9, 0) == 0 and levels(20) == (9, 11)
     for bad in [(20, 7, 0), (20, 4, 0), (20, 10, 0), (20, 8, 5), (10, 5, 0), (28, 10, 6), (28, 5, 5),
                 (28, 9, 3), (31, 8, 8), (20, 0, 3), (20, -1, 0)]:
         assert lib.gpupi_ntt_set_split(*bad) != 0, bad
-    assert levels(20) == (8, 12)
+    assert levels(20) == (9, 11)
     # three levels: m1 = 0 leaves a 4096-point contiguous level
     assert lib.gpupi_ntt_set_split(28, 7, 0) == 0 and levels(28) == (7, 9, 12)
     assert lib.gpupi_ntt_set_split(28, 9, 10) != 0 and lib.gpupi_ntt_set_split(28, 8, 5) != 0
     assert lib.gpupi_ntt_set_split(28, 9, 8) == 0 and levels(28) == (9, 8, 11)
-    assert lib.gpupi_ntt_set_split(20, 0, 0) == 0 and levels(20) == (9, 11)
+    assert lib.gpupi_ntt_set_split(20, 0, 0) == 0 and levels(20) == (8, 12)
     assert lib.gpupi_ntt_set_split(28, 0, 0) == 0 and levels(28) == (8, 8, 12)
