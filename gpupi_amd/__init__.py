@@ -134,6 +134,7 @@ def _declare(lib):
         "batch_mp_workspace_size": (ctypes.c_size_t, [c_ctx]),
         "batch_mp_ensure_workspace": (err, [c_ctx, ctypes.c_size_t]),
         "batch_mp_mul": (err, [c_ctx, A, A, A]),
+        "batch_mp_mul_shared": (err, [c_ctx, A, A, A, A, A]),
         "batch_mp_mul_small": (err, [c_ctx, A, u32, A]),
         "batch_mp_exactdiv_small": (err, [c_ctx, A, u32]),
         "batch_mp_add": (err, [c_ctx, A, A, A]),

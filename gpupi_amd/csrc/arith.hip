@@ -183,6 +183,42 @@ hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, Bat
                                  C.stride, ctx->stream);
 }
 
+hipError_t batch_mp_mul_shared(BatchMPContext *ctx, BatchMPArray A1, BatchMPArray A2, BatchMPArray B,
+                               BatchMPArray C1, BatchMPArray C2) {
+    if (ctx == nullptr || !valid_array(A1) || !valid_array(A2) || !valid_array(B) || !valid_output(C1) ||
+        !valid_output(C2))
+        return hipErrorInvalidValue;
+    if (!same_batch(A1, C1) || !same_batch(A2, C2)) return hipErrorInvalidValue;
+    const uint32_t rows = std::max(C1.batch_size, C2.batch_size);
+    if (B.stride != 0 && B.batch_size < rows) return hipErrorInvalidValue;
+    if (B.stride == 0 && rows > 0 && B.batch_size == 0) return hipErrorInvalidValue;
+    if ((uint64_t)C1.length != (uint64_t)A1.length + B.length || (uint64_t)C2.length != (uint64_t)A2.length + B.length)
+        return hipErrorInvalidValue;
+    for (const BatchMPArray *c : {&C1, &C2})
+        if (c->batch_size > 0 && (c->data == A1.data || c->data == A2.data || c->data == B.data))
+            return hipErrorInvalidValue;
+    if (C1.batch_size > 0 && C2.batch_size > 0 && C1.data == C2.data) return hipErrorInvalidValue;
+    const auto ntt_sized = [&](uint32_t La) {
+        const uint32_t total = La + B.length;
+        return La > 0 && B.length > 0 &&
+               !((int64_t)total < ctx->ntt_min_limbs && gpupi::mul_naive_supports(La, B.length));
+    };
+    if (C1.batch_size > 0 && C2.batch_size > 0 && C1.length > 0 && C2.length > 0 && ntt_sized(A1.length) &&
+        ntt_sized(A2.length) && gpupi::ntt_shared_supports(A1.length, A2.length, B.length)) {
+        GPUPI_RETURN_IF(ensure(ctx, gpupi::ntt_shared_workspace_bytes(A1.length, C1.batch_size, A2.length,
+                                                                      C2.batch_size, B.length, B.stride)));
+        return gpupi::launch_mul_ntt_shared(A1.data, A1.length, A1.stride, C1.batch_size, C1.data, C1.stride,
+                                            A2.data, A2.length, A2.stride, C2.batch_size, C2.data, C2.stride,
+                                            B.data, B.length, B.stride, ctx->workspace, ctx->workspace_size_bytes,
+                                            ctx->ntt_tables, ctx->stream);
+    }
+    BatchMPArray B1 = B, B2 = B;
+    B1.batch_size = C1.batch_size;
+    B2.batch_size = C2.batch_size;
+    GPUPI_RETURN_IF(batch_mp_mul(ctx, A1, B1, C1));
+    return batch_mp_mul(ctx, A2, B2, C2);
+}
+
 hipError_t gpupi_mul_ragged(BatchMPContext *ctx, const GpupiMulItem *items, uint32_t count) {
     if (ctx == nullptr || (count > 0 && items == nullptr)) return hipErrorInvalidValue;
     // validate everything before launching anything

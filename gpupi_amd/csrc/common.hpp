@@ -96,6 +96,17 @@ hipError_t launch_mul_ntt_ragged(const GpupiMulItem *items, const GpupiMulItem *
                                  void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
                                  hipStream_t stream);
 bool ntt_supports(uint32_t L_a, uint32_t L_b);
+// Two NTT products sharing operand B (C1 = A1 B over N1 rows, C2 = A2 B over
+// N2 rows; B has max(N1, N2) rows, or one row at stride 0): B is
+// transformed once.  Supported when both products have the same transform
+// size above one tile.
+bool ntt_shared_supports(uint32_t L1, uint32_t L2, uint32_t Lb);
+size_t ntt_shared_workspace_bytes(uint32_t L1, uint32_t N1, uint32_t L2, uint32_t N2, uint32_t Lb, uint32_t sB);
+hipError_t launch_mul_ntt_shared(const uint32_t *A1, uint32_t L1, uint32_t s1, uint32_t N1, uint32_t *C1,
+                                 uint32_t sC1, const uint32_t *A2, uint32_t L2, uint32_t s2, uint32_t N2,
+                                 uint32_t *C2, uint32_t sC2, const uint32_t *B, uint32_t Lb, uint32_t sB,
+                                 void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
+                                 hipStream_t stream);
 hipError_t ntt_tables_init(NTTPrecomputedTables *tables);
 void ntt_tables_release(NTTPrecomputedTables *tables);
 

@@ -241,25 +241,65 @@ __device__ __forceinline__ unsigned __int128 recombine(double v0, double v1) {
     return (unsigned __int128)kPrime0 * (uint64_t)t + (uint64_t)r0;
 }
 
+// Forward operand o of a multi-level launch: limbs of row i at X + i s (the
+// first level reads them), residues of prime P of row i at
+// T + base + P pstride + i K.
+struct NttOp {
+    const uint32_t *X;
+    uint64_t base, pstride;
+    uint32_t L, s;
+};
+
+// Product j of a multi-level launch: operand a times operand b (row 0 of b
+// for every row when b_bcast); the product overwrites a's residues, then
+// goes to C + i sC (Lc limbs) with its spill words from spill + spill_base.
+struct NttProd {
+    uint32_t *C;
+    uint64_t spill_base;
+    uint32_t Lc, sC;
+    uint32_t a, b, b_bcast;
+};
+
 struct PassArgs {
+    // one-tile products: A, B, C with strides, N products
     const uint32_t *A;
     const uint32_t *B;
     uint32_t La, Lb, sA, sB;
-    double *T;        // planes: A p0, A p1, B p0, B p1; each `plane` elements
-    uint64_t plane;
     uint32_t *C;
     uint32_t Lc, sC;
+    uint32_t N;
+    // multi-level products: residue planes, spill words, operands and
+    // products; forward launches cover operand o's rows from block
+    // op_start[o], the others product j's rows from block prod_start[j]
+    double *T;
     uint64_t *spill;
-    uint32_t N;  // products in this launch
+    NttOp ops[3];
+    NttProd prods[2];
+    uint32_t op_start[3];
+    uint32_t prod_start[2];
+    uint32_t n_ops, n_prods;
     Plan plan;
     const double *tables;
     // ragged launches: per-product operands and lengths (device array), or
-    // null when every product uses A, B, C with the strides above
+    // null when the operands above describe every product
     const GpupiMulItem *items;
 };
 
-__device__ __forceinline__ double *plane_of(const PassArgs &a, int op, int P) {
-    return a.T + (size_t)(2 * op + P) * a.plane;
+// The (segment, row, tile) a block works on: segments are operands (forward
+// launches) or products (the others), each `rows x tiles` blocks in turn.
+struct BlockPos {
+    uint32_t seg, row, t;
+};
+__device__ __forceinline__ BlockPos block_pos(const uint32_t *start, uint32_t nseg, uint32_t tiles) {
+    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t seg = 0, first = 0;
+    for (uint32_t i = 1; i < nseg; ++i)
+        if (bid >= start[i]) {
+            seg = i;
+            first = start[i];
+        }
+    const uint32_t local = bid - first;
+    return BlockPos{seg, local / tiles, local % tiles};
 }
 
 // Per-thread geometry of a column tile (level `level`, DFT size 2^M): the
@@ -315,26 +355,20 @@ __device__ __forceinline__ void col_fwd_prime(double *v, double *lds, const ColT
 template <bool FROM_LIMBS, int M>
 __global__ void __launch_bounds__(256) ntt_col_fwd_kernel(PassArgs a, int level) {
     __shared__ double lds[kLdsSlots];
-    const uint32_t tiles = tiles_per_product(a.plan);
-    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t t = bid % tiles;
-    const uint32_t op = (bid / tiles) & 1u;
-    const uint32_t prod = bid / (2u * tiles);
+    const BlockPos bp = block_pos(a.op_start, a.n_ops, tiles_per_product(a.plan));
+    const NttOp o = a.ops[bp.seg];
     const uint64_t K = 1ull << a.plan.log2K;
-    const ColThread<M> ct(a.plan, level, t);
-    double *T0 = plane_of(a, (int)op, 0) + (size_t)prod * K;
-    double *T1 = plane_of(a, (int)op, 1) + (size_t)prod * K;
+    const ColThread<M> ct(a.plan, level, bp.t);
+    double *T0 = a.T + o.base + (size_t)bp.row * K;
+    double *T1 = T0 + o.pstride;
     double v0[16], v1[16];
     if (FROM_LIMBS) {
-        const uint32_t *X;
-        uint32_t LX;
+        const uint32_t *X = o.X + (size_t)bp.row * o.s;
+        uint32_t LX = o.L;
         if (a.items) {
-            const GpupiMulItem it = a.items[prod];
-            X = op ? it.B : it.A;
-            LX = op ? it.L_b : it.L_a;
-        } else {
-            X = (op ? a.B : a.A) + (size_t)prod * (op ? a.sB : a.sA);
-            LX = op ? a.Lb : a.La;
+            const GpupiMulItem it = a.items[bp.row];
+            X = bp.seg ? it.B : it.A;
+            LX = bp.seg ? it.L_b : it.L_a;
         }
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
@@ -380,18 +414,19 @@ template <int M>
 __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     __shared__ double lds[kLdsSlots];
     constexpr uint32_t kJ = 1u << (M - 4);
-    const uint32_t tiles = tiles_per_product(a.plan);
-    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t t = bid % tiles;
-    const uint32_t prod = bid / tiles;
+    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan));
+    const NttProd pr = a.prods[bp.seg];
+    const NttOp oa = a.ops[pr.a], ob = a.ops[pr.b];
     const uint64_t K = 1ull << a.plan.log2K;
-    const size_t g0 = (size_t)prod * K + (size_t)t * kTile + row_step0_base<M>();
+    const size_t off = (size_t)bp.t * kTile + row_step0_base<M>();
+    double *TA0 = a.T + oa.base + (size_t)bp.row * K + off;
+    const double *TB0 = a.T + ob.base + (size_t)(pr.b_bcast ? 0u : bp.row) * K + off;
 #pragma clang loop unroll(disable)
     for (int P = 0; P < 2; ++P) {
         const Mod m = mod_of(P);
         const double *tab = a.tables + P * kPrimeTable;
-        double *TA = plane_of(a, 0, P) + g0;
-        const double *TB = plane_of(a, 1, P) + g0;
+        double *TA = TA0 + P * oa.pstride;
+        const double *TB = TB0 + P * ob.pstride;
         double ra[16], v[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) ra[j] = TA[j * kJ];
@@ -432,9 +467,7 @@ __device__ __forceinline__ void col_inv_prime(double *v, double *lds, const ColT
 // twiddle table entries (every load before any store).
 template <int M>
 __device__ __forceinline__ void col_inv_load(double *v0, double *v1, ColTw &tw0, ColTw &tw1, const PassArgs &a,
-                                             const ColThread<M> &ct, uint32_t prod) {
-    const size_t base = (size_t)prod * (1ull << a.plan.log2K);
-    const double *T0 = plane_of(a, 0, 0) + base, *T1 = plane_of(a, 0, 1) + base;
+                                             const ColThread<M> &ct, const double *T0, const double *T1) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) v0[u] = T0[ct.off(u)];
 #pragma unroll
@@ -448,23 +481,27 @@ __device__ __forceinline__ void col_inv_load(double *v0, double *v1, ColTw &tw0,
 template <int M>
 __global__ void __launch_bounds__(256) ntt_col_inv_kernel(PassArgs a, int level) {
     __shared__ double lds[kLdsSlots];
-    const uint32_t tiles = tiles_per_product(a.plan);
-    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t t = bid % tiles;
-    const uint32_t prod = bid / tiles;
-    const ColThread<M> ct(a.plan, level, t);
-    double v0[16], v1[16];
-    ColTw tw0, tw1;
-    col_inv_load<M>(v0, v1, tw0, tw1, a, ct, prod);
-    col_inv_prime<M, 0>(v0, lds, ct, a.tables, tw0);
+    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan));
+    const NttOp oa = a.ops[a.prods[bp.seg].a];
+    const ColThread<M> ct(a.plan, level, bp.t);
+    double *T0 = a.T + oa.base + (size_t)bp.row * (1ull << a.plan.log2K);
+    double *T1 = T0 + oa.pstride;
+    // register budget (occupancy 4 needs <= 128 VGPRs): one prime's
+    // elements at a time
+    const ColTw tw0 = col_tw_load(a.tables, ct.cg, ct.b0, true);
+    const ColTw tw1 = col_tw_load(a.tables + kPrimeTable, ct.cg, ct.b0, true);
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = T0[ct.off(u)];
+    col_inv_prime<M, 0>(v, lds, ct, a.tables, tw0);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) T0[ct.off(u)] = v[u];
     __syncthreads();
-    col_inv_prime<M, 1>(v1, lds, ct, a.tables, tw1);
-    const size_t base = (size_t)prod * (1ull << a.plan.log2K);
-    double *T0 = plane_of(a, 0, 0) + base, *T1 = plane_of(a, 0, 1) + base;
 #pragma unroll
-    for (int u = 0; u < 16; ++u) T0[ct.off(u)] = v0[u];
+    for (int u = 0; u < 16; ++u) v[u] = T1[ct.off(u)];
+    col_inv_prime<M, 1>(v, lds, ct, a.tables, tw1);
 #pragma unroll
-    for (int u = 0; u < 16; ++u) T1[ct.off(u)] = v1[u];
+    for (int u = 0; u < 16; ++u) T1[ct.off(u)] = v[u];
 }
 
 // First inverse column level: both primes (two tiles), 1/K, Garner, limbs of
@@ -476,16 +513,17 @@ __global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
     constexpr int LC = kTileLog2 - M;
     constexpr int RL = run_log2_for(M);
     constexpr uint32_t kRunLen = 1u << RL;
-    const uint32_t tiles = tiles_per_product(a.plan);
-    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t t = bid % tiles;
-    const uint32_t prod = bid / tiles;
+    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan));
+    const uint32_t t = bp.t, prod = bp.row;
+    const NttProd pr = a.prods[bp.seg];
+    const NttOp oa = a.ops[pr.a];
     const int log2K = a.plan.log2K;
     const ColThread<M> ct(a.plan, 0, t);
     {
+        const double *T0 = a.T + oa.base + (size_t)prod * (1ull << log2K);
         double v0[16], v1[16];
         ColTw tw0, tw1;
-        col_inv_load<M>(v0, v1, tw0, tw1, a, ct, prod);
+        col_inv_load<M>(v0, v1, tw0, tw1, a, ct, T0, T0 + oa.pstride);
         col_inv_prime<M, 0>(v0, l0, ct, a.tables, tw0);
         const double kinv0 = c_ntt.kinv[0][log2K];
 #pragma unroll
@@ -498,7 +536,9 @@ __global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
     __syncthreads();
     const uint32_t col0 = col_tile(a.plan, 0, t).col0;
     const int ls = log2K - M;
-    uint32_t *out = a.C + (size_t)prod * a.sC;
+    uint32_t *out = pr.C + (size_t)prod * pr.sC;
+    const uint32_t Lc = pr.Lc;
+    uint64_t *spill = a.spill + pr.spill_base + (size_t)prod * ((1ull << log2K) >> RL);
     const bool vec = (((uintptr_t)out) & 15u) == 0;
 #pragma unroll
     for (uint32_t q = threadIdx.x; q < (kTile >> RL); q += 256u) {
@@ -513,16 +553,16 @@ __global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
             limb[j] = (uint32_t)pending;
             pending >>= 32;
         }
-        if (vec && n0 + kRunLen <= a.Lc) {
+        if (vec && n0 + kRunLen <= Lc) {
 #pragma unroll
             for (uint32_t j = 0; j < kRunLen; j += 4)
                 *reinterpret_cast<uint4 *>(out + n0 + j) = make_uint4(limb[j], limb[j + 1], limb[j + 2], limb[j + 3]);
         } else {
 #pragma unroll
             for (uint32_t j = 0; j < kRunLen; ++j)
-                if (n0 + j < a.Lc) out[n0 + j] = limb[j];
+                if (n0 + j < Lc) out[n0 + j] = limb[j];
         }
-        a.spill[(size_t)prod * ((1ull << log2K) >> RL) + (n0 >> RL)] = (uint64_t)pending;
+        spill[n0 >> RL] = (uint64_t)pending;
     }
 }
 
@@ -753,16 +793,6 @@ bool ntt_supports(uint32_t L_a, uint32_t L_b) {
     return L_a > 0 && L_b > 0 && transform_log2(L_a, L_b) <= kMaxLog2;
 }
 
-size_t ntt_workspace_bytes(uint32_t N, uint32_t L_a, uint32_t L_b) {
-    if (!ntt_supports(L_a, L_b) || N == 0) return 0;
-    const int k = transform_log2(L_a, L_b);
-    if (make_plan(k).levels == 1) return 0;
-    const size_t nc = chunk_products(N, k);
-    const size_t K = size_t(1) << k;
-    return 4 * align256(nc * K * sizeof(double)) + align256(nc * spill_runs_bound(k) * sizeof(uint64_t)) +
-           align256(carry_flags_bytes((uint32_t)nc, L_a + L_b));
-}
-
 hipError_t ntt_tables_init(NTTPrecomputedTables *tables) {
     const uint64_t primes[2] = {kPrime0, kPrime1};
     const uint64_t gens[2] = {kGen0, kGen1};
@@ -842,143 +872,269 @@ __global__ void __launch_bounds__(256) ragged_copy_kernel(const uint32_t *stagin
     for (uint32_t i = slice * 1024u + threadIdx.x; i < Lc && i < (slice + 1) * 1024u; i += 256u) it.C[i] = src[i];
 }
 
-// One batched NTT product launch sequence: N products of one transform size
-// 2^k, either uniform (A, B, C with strides) or ragged (per-product items,
-// `host_items` mirroring `items` for the byte accounting).
+// One NTT launch sequence over products of one transform size 2^k: up to
+// three forward operands and two products with rows at strides (a product's
+// A operand receives the product and is used by nothing else; a B operand
+// may be shared by both products, or be one row at stride 0 that every row
+// multiplies), or ragged items (operands 0 = A and 1 = B of each item, one
+// product).
 struct NttJob {
-    const uint32_t *A = nullptr, *B = nullptr;
-    uint32_t *C = nullptr;
-    uint32_t La = 0, Lb = 0, sA = 0, sB = 0, sC = 0;
-    const GpupiMulItem *items = nullptr;
+    struct Op {
+        const uint32_t *X = nullptr;
+        uint32_t L = 0, s = 0;
+    };
+    struct Prod {
+        uint32_t a = 0, b = 1;
+        uint32_t *C = nullptr;
+        uint32_t sC = 0, N = 0;
+    };
+    Op ops[3];
+    Prod prods[2];
+    uint32_t n_ops = 0, n_prods = 0;
+    bool ragged = false;
+    const GpupiMulItem *items = nullptr;  // ragged: device items
     const GpupiMulItem *host_items = nullptr;
-    uint32_t N = 0;
     int k = 0;
 };
 
-size_t job_workspace_bytes(uint32_t N, int k, bool ragged, uint32_t Lc) {
-    if (make_plan(k).levels == 1) return 0;
-    const size_t nc = chunk_products(N, k);
-    const size_t K = size_t(1) << k;
-    return 4 * align256(nc * K * sizeof(double)) + align256(nc * spill_runs_bound(k) * sizeof(uint64_t)) +
-           align256(carry_flags_bytes((uint32_t)nc, ragged ? (uint32_t)K : Lc)) +
-           (ragged ? align256(nc * K * sizeof(uint32_t)) : 0);
+// Rows each operand needs (the most of any product using it; one for a
+// broadcast row), the rows of the whole job and each product's length.
+struct JobShape {
+    uint32_t op_rows[3] = {0, 0, 0};
+    bool op_bcast[3] = {false, false, false};
+    uint32_t rows = 0;
+    uint32_t Lc[2] = {0, 0};
+};
+
+JobShape shape_of(const NttJob &j) {
+    JobShape sh;
+    const size_t K = size_t(1) << j.k;
+    bool used_as_a[3] = {false, false, false};
+    for (uint32_t q = 0; q < j.n_prods; ++q) {
+        const NttJob::Prod &pr = j.prods[q];
+        sh.rows = std::max(sh.rows, pr.N);
+        sh.op_rows[pr.a] = std::max(sh.op_rows[pr.a], pr.N);
+        sh.op_rows[pr.b] = std::max(sh.op_rows[pr.b], pr.N);
+        used_as_a[pr.a] = true;
+        sh.Lc[q] = j.ragged ? (uint32_t)K : j.ops[pr.a].L + j.ops[pr.b].L;
+    }
+    for (uint32_t o = 0; o < j.n_ops; ++o) {
+        sh.op_bcast[o] = !j.ragged && !used_as_a[o] && j.ops[o].s == 0 && sh.op_rows[o] > 1;
+        if (sh.op_bcast[o]) sh.op_rows[o] = 1;
+    }
+    return sh;
 }
 
-hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
-                   hipStream_t stream) {
-    if (j.N == 0) return hipSuccess;
-    const bool ragged = j.items != nullptr;
+// Whether the operands and products describe a job run_ntt can do.
+bool job_ok(const NttJob &j) {
+    if (j.n_ops < 2 || j.n_ops > 3 || j.n_prods < 1 || j.n_prods > 2) return false;
+    if (j.ragged && (j.items == nullptr || j.host_items == nullptr || j.n_ops != 2 || j.n_prods != 1)) return false;
+    for (uint32_t q = 0; q < j.n_prods; ++q) {
+        const NttJob::Prod &pr = j.prods[q];
+        if (pr.a >= j.n_ops || pr.b >= j.n_ops || pr.a == pr.b) return false;
+        for (uint32_t r = 0; r < j.n_prods; ++r)
+            if (r != q && (j.prods[r].a == pr.a || j.prods[r].b == pr.a)) return false;
+        if (!j.ragged && transform_log2(j.ops[pr.a].L, j.ops[pr.b].L) != j.k) return false;
+    }
+    return true;
+}
+
+struct JobLayout {
+    size_t plane_elems[3] = {0, 0, 0};  // per prime, per operand (elements)
+    size_t op_base[3] = {0, 0, 0};      // element offset of operand o's prime-0 plane
+    size_t spill_elems = 0;             // per product
+    size_t spill_off = 0, flags_off = 0, staging_off = 0, total = 0;  // bytes
+    size_t nc = 0;
+};
+
+JobLayout layout_of(const NttJob &j, const JobShape &sh) {
+    JobLayout l;
+    const size_t K = size_t(1) << j.k;
+    l.nc = chunk_products(sh.rows, j.k);
+    size_t off = 0;
+    for (uint32_t o = 0; o < j.n_ops; ++o) {
+        const size_t cap = sh.op_bcast[o] ? 1 : std::min<size_t>(l.nc, sh.op_rows[o]);
+        l.plane_elems[o] = align256(cap * K * sizeof(double)) / sizeof(double);
+        l.op_base[o] = off / sizeof(double);
+        off += 2 * l.plane_elems[o] * sizeof(double);
+    }
+    l.spill_off = off;
+    l.spill_elems = l.nc * spill_runs_bound(j.k);
+    off += j.n_prods * align256(l.spill_elems * sizeof(uint64_t));
+    l.flags_off = off;
+    size_t fl = 0;
+    for (uint32_t q = 0; q < j.n_prods; ++q) fl = std::max(fl, carry_flags_bytes((uint32_t)l.nc, sh.Lc[q]));
+    off += align256(fl);
+    l.staging_off = off;
+    if (j.ragged) off += align256(l.nc * K * sizeof(uint32_t));
+    l.total = off;
+    return l;
+}
+
+size_t job_workspace_bytes(const NttJob &j) {
+    if (make_plan(j.k).levels == 1) return 0;
+    const JobShape sh = shape_of(j);
+    if (sh.rows == 0) return 0;
+    return layout_of(j, sh).total;
+}
+
+// One-tile products (K <= 4096): every product in one launch of the local
+// kernel.
+hipError_t run_local(const NttJob &j, const NTTPrecomputedTables &tables, hipStream_t stream) {
     const int k = j.k;
-    const Plan p = active_plan(k);
-    const size_t K = size_t(1) << k;
-    const uint32_t Lc = ragged ? (uint32_t)K : j.La + j.Lb;
-    auto limbs_in = [&](size_t r0, uint32_t n) {  // sum of L_a + L_b over products [r0, r0 + n)
-        if (!ragged) return (double)n * ((double)j.La + j.Lb);
-        double t = 0;
-        for (uint32_t i = 0; i < n; ++i) t += (double)j.host_items[r0 + i].L_a + j.host_items[r0 + i].L_b;
-        return t;
-    };
-
-    PassArgs a{};
-    a.La = j.La;
-    a.Lb = j.Lb;
-    a.sA = j.sA;
-    a.sB = j.sB;
-    a.Lc = j.La + j.Lb;
-    a.sC = j.sC;
-    a.plan = p;
-    a.tables = tables.roots;
-
-    if (p.levels == 1) {
-        const uint32_t G = kTile >> k;
-        const uint64_t blocks = ceil_div64(j.N, G);
-        if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-        a.A = j.A;
-        a.B = j.B;
-        a.C = j.C;
-        a.N = j.N;
+    for (uint32_t q = 0; q < j.n_prods; ++q) {
+        const NttJob::Prod &pr = j.prods[q];
+        if (pr.N == 0) continue;
+        PassArgs a{};
+        a.plan = make_plan(k);
+        a.tables = tables.roots;
+        a.A = j.ops[pr.a].X;
+        a.B = j.ops[pr.b].X;
+        a.La = j.ops[pr.a].L;
+        a.Lb = j.ops[pr.b].L;
+        a.sA = j.ops[pr.a].s;
+        a.sB = j.ops[pr.b].s;
+        a.C = pr.C;
+        a.Lc = a.La + a.Lb;
+        a.sC = pr.sC;
+        a.N = pr.N;
         a.items = j.items;
-        const double bytes = 8.0 * limbs_in(0, j.N);
-        return timed(kLocal, k, bytes, stream, [&] {
+        const uint32_t G = kTile >> k;
+        const uint64_t blocks = ceil_div64(pr.N, G);
+        if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+        double limbs = 0;
+        if (j.ragged) {
+            for (uint32_t i = 0; i < pr.N; ++i) limbs += (double)j.host_items[i].L_a + j.host_items[i].L_b;
+        } else {
+            limbs = (double)pr.N * ((double)a.La + a.Lb);
+        }
+        GPUPI_RETURN_IF(timed(kLocal, k, 8.0 * limbs, stream, [&] {
             dispatch_m<kLocalLo, kLocalHi>(k, [&](auto mc) {
                 hipLaunchKernelGGL((ntt_local_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256), 0,
                                    stream, a);
             });
-        });
+        }));
     }
-    if (workspace == nullptr || workspace_bytes < job_workspace_bytes(j.N, k, ragged, Lc)) return hipErrorInvalidValue;
-    const size_t nc = chunk_products(j.N, k);
+    return hipSuccess;
+}
+
+hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
+                   hipStream_t stream) {
+    if (!job_ok(j)) return hipErrorInvalidValue;
+    const JobShape sh = shape_of(j);
+    if (sh.rows == 0) return hipSuccess;
+    const int k = j.k;
+    if (make_plan(k).levels == 1) return run_local(j, tables, stream);
+    const Plan p = active_plan(k);
+    const size_t K = size_t(1) << k;
+    const JobLayout l = layout_of(j, sh);
+    if (workspace == nullptr || workspace_bytes < l.total) return hipErrorInvalidValue;
     const size_t runs = K >> final_run_log2(p);
-    const size_t plane_bytes = align256(nc * K * sizeof(double));
     char *ws = static_cast<char *>(workspace);
-    a.T = reinterpret_cast<double *>(ws);
-    a.plane = plane_bytes / sizeof(double);
-    a.spill = reinterpret_cast<uint64_t *>(ws + 4 * plane_bytes);
-    uint8_t *flags =
-        reinterpret_cast<uint8_t *>(ws + 4 * plane_bytes + align256(nc * spill_runs_bound(k) * sizeof(uint64_t)));
-    uint32_t *staging = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(flags) +
-                                                     align256(carry_flags_bytes((uint32_t)nc, Lc)));
+    uint8_t *flags = reinterpret_cast<uint8_t *>(ws + l.flags_off);
+    uint32_t *staging = reinterpret_cast<uint32_t *>(ws + l.staging_off);
     const uint32_t tiles = tiles_per_product(p);
-    for (size_t r0 = 0; r0 < j.N; r0 += nc) {
-        const uint32_t n = (uint32_t)((j.N - r0) < nc ? (j.N - r0) : nc);
-        if (ragged) {
-            // limbs of the last level go to a staging row per product (stride
-            // K), the spill pass runs there, then each row moves to its C
-            a.items = j.items + r0;
-            a.C = staging;
-            a.sC = (uint32_t)K;
-            a.Lc = (uint32_t)K;
-        } else {
-            a.A = j.A + r0 * j.sA;
-            a.B = j.B + r0 * j.sB;
-            a.C = j.C + r0 * j.sC;
+
+    PassArgs a{};
+    a.plan = p;
+    a.tables = tables.roots;
+    a.T = reinterpret_cast<double *>(ws);
+    a.spill = reinterpret_cast<uint64_t *>(ws + l.spill_off);
+    a.n_ops = j.n_ops;
+    a.n_prods = j.n_prods;
+    const size_t spill_stride = align256(l.spill_elems * sizeof(uint64_t)) / sizeof(uint64_t);
+    for (size_t r0 = 0; r0 < sh.rows; r0 += l.nc) {
+        // operands: rows of this chunk (a broadcast row only in the first)
+        uint64_t fwd_blocks = 0, limb_reads = 0;
+        uint32_t op_rows[3] = {0, 0, 0};
+        for (uint32_t o = 0; o < j.n_ops; ++o) {
+            const NttJob::Op &op = j.ops[o];
+            if (sh.op_bcast[o]) op_rows[o] = r0 == 0 ? 1u : 0u;
+            else op_rows[o] = (uint32_t)std::min<size_t>(l.nc, sh.op_rows[o] > r0 ? sh.op_rows[o] - r0 : 0);
+            a.ops[o].X = op.X + (sh.op_bcast[o] ? 0 : r0 * op.s);
+            a.ops[o].L = op.L;
+            a.ops[o].s = op.s;
+            a.ops[o].base = l.op_base[o];
+            a.ops[o].pstride = l.plane_elems[o];
+            if (o < 3) a.op_start[o] = (uint32_t)fwd_blocks;
+            fwd_blocks += (uint64_t)op_rows[o] * tiles;
+            if (!j.ragged) limb_reads += (uint64_t)op_rows[o] * op.L;
         }
-        a.N = n;
-        const uint64_t fwd_blocks = (uint64_t)n * 2 * tiles, blocks = (uint64_t)n * tiles;
-        if (fwd_blocks > 0x7fffffffull) return hipErrorInvalidValue;
+        uint64_t prod_blocks = 0, prod_rows = 0, limb_writes = 0;
+        uint32_t rows[2] = {0, 0};
+        for (uint32_t q = 0; q < j.n_prods; ++q) {
+            const NttJob::Prod &pr = j.prods[q];
+            rows[q] = (uint32_t)std::min<size_t>(l.nc, pr.N > r0 ? pr.N - r0 : 0);
+            NttProd &d = a.prods[q];
+            d.a = pr.a;
+            d.b = pr.b;
+            d.b_bcast = sh.op_bcast[pr.b] ? 1u : 0u;
+            d.C = j.ragged ? staging : pr.C + r0 * pr.sC;
+            d.sC = j.ragged ? (uint32_t)K : pr.sC;
+            d.Lc = sh.Lc[q];
+            d.spill_base = q * spill_stride;
+            a.prod_start[q] = (uint32_t)prod_blocks;
+            prod_blocks += (uint64_t)rows[q] * tiles;
+            prod_rows += rows[q];
+            limb_writes += (uint64_t)rows[q] * d.Lc;
+        }
+        if (fwd_blocks > 0x7fffffffull || prod_blocks > 0x7fffffffull) return hipErrorInvalidValue;
+        if (j.ragged) {
+            a.items = j.items + r0;
+            for (uint32_t i = 0; i < rows[0]; ++i)
+                limb_reads += (uint64_t)j.host_items[r0 + i].L_a + j.host_items[r0 + i].L_b;
+        }
         // algorithmic bytes: 16 B per residue pair (one binary64 per prime),
         // 4 B per limb
-        const double tb = 16.0 * (double)n * (double)K;
-        const double lin = limbs_in(r0, n);
-        GPUPI_RETURN_IF(timed(kColFwdLimbs, k, 4.0 * lin + 2 * tb, stream, [&] {
-            dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
-                hipLaunchKernelGGL((ntt_col_fwd_kernel<true, decltype(mc)::value>), dim3((uint32_t)fwd_blocks),
-                                   dim3(256), 0, stream, a, 0);
+        const double fwd_tb = 16.0 * (double)(fwd_blocks / tiles) * (double)K;
+        const double prod_tb = 16.0 * (double)prod_rows * (double)K;
+        if (fwd_blocks > 0) {
+            GPUPI_RETURN_IF(timed(kColFwdLimbs, k, 4.0 * (double)limb_reads + fwd_tb, stream, [&] {
+                dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
+                    hipLaunchKernelGGL((ntt_col_fwd_kernel<true, decltype(mc)::value>), dim3((uint32_t)fwd_blocks),
+                                       dim3(256), 0, stream, a, 0);
+                });
+            }));
+            if (p.levels == 3) {
+                GPUPI_RETURN_IF(timed(kColFwd, k, 2 * fwd_tb, stream, [&] {
+                    dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
+                        hipLaunchKernelGGL((ntt_col_fwd_kernel<false, decltype(mc)::value>),
+                                           dim3((uint32_t)fwd_blocks), dim3(256), 0, stream, a, 1);
+                    });
+                }));
+            }
+        }
+        if (prod_blocks == 0) continue;
+        GPUPI_RETURN_IF(timed(kRowFused, k, 3 * prod_tb, stream, [&] {
+            dispatch_m<kRowLo, kRowHi>(p.log2M[p.levels - 1], [&](auto mc) {
+                hipLaunchKernelGGL((ntt_row_fused_kernel<decltype(mc)::value>), dim3((uint32_t)prod_blocks),
+                                   dim3(256), 0, stream, a);
             });
         }));
         if (p.levels == 3) {
-            GPUPI_RETURN_IF(timed(kColFwd, k, 4 * tb, stream, [&] {
+            GPUPI_RETURN_IF(timed(kColInv, k, 2 * prod_tb, stream, [&] {
                 dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
-                    hipLaunchKernelGGL((ntt_col_fwd_kernel<false, decltype(mc)::value>), dim3((uint32_t)fwd_blocks),
+                    hipLaunchKernelGGL((ntt_col_inv_kernel<decltype(mc)::value>), dim3((uint32_t)prod_blocks),
                                        dim3(256), 0, stream, a, 1);
                 });
             }));
         }
-        GPUPI_RETURN_IF(timed(kRowFused, k, 3 * tb, stream, [&] {
-            dispatch_m<kRowLo, kRowHi>(p.log2M[p.levels - 1], [&](auto mc) {
-                hipLaunchKernelGGL((ntt_row_fused_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256),
-                                   0, stream, a);
-            });
-        }));
-        if (p.levels == 3) {
-            GPUPI_RETURN_IF(timed(kColInv, k, 2 * tb, stream, [&] {
-                dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
-                    hipLaunchKernelGGL((ntt_col_inv_kernel<decltype(mc)::value>), dim3((uint32_t)blocks), dim3(256),
-                                       0, stream, a, 1);
-                });
-            }));
+        GPUPI_RETURN_IF(timed(kColInvLimbs, k, prod_tb + 4.0 * (double)limb_writes + 8.0 * prod_rows * runs, stream,
+                              [&] {
+                                  dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
+                                      hipLaunchKernelGGL((ntt_col_inv_limbs_kernel<decltype(mc)::value>),
+                                                         dim3((uint32_t)prod_blocks), dim3(256), 0, stream, a);
+                                  });
+                              }));
+        for (uint32_t q = 0; q < j.n_prods; ++q) {
+            if (rows[q] == 0) continue;
+            const NttProd &d = a.prods[q];
+            GPUPI_RETURN_IF(launch_add_spills(d.C, a.spill + d.spill_base, rows[q], d.Lc, d.sC,
+                                              1u << final_run_log2(p), (uint32_t)runs, flags, stream));
         }
-        GPUPI_RETURN_IF(timed(kColInvLimbs, k, tb + 4.0 * n * (double)a.Lc + 8.0 * n * runs, stream, [&] {
-            dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
-                hipLaunchKernelGGL((ntt_col_inv_limbs_kernel<decltype(mc)::value>), dim3((uint32_t)blocks),
-                                   dim3(256), 0, stream, a);
-            });
-        }));
-        GPUPI_RETURN_IF(launch_add_spills(a.C, a.spill, n, a.Lc, a.sC, 1u << final_run_log2(p), (uint32_t)runs,
-                                          flags, stream));
-        if (ragged) {
+        if (j.ragged) {
             const uint32_t slices = (uint32_t)ceil_div64(K, 1024);
-            hipLaunchKernelGGL(ragged_copy_kernel, dim3(n * slices), dim3(256), 0, stream, staging, (uint64_t)K,
+            hipLaunchKernelGGL(ragged_copy_kernel, dim3(rows[0] * slices), dim3(256), 0, stream, staging, (uint64_t)K,
                                a.items, slices);
             GPUPI_RETURN_IF(hipGetLastError());
         }
@@ -986,7 +1142,60 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
     return hipSuccess;
 }
 
+// A uniform product C = A B of N rows; a broadcast A (stride 0) is swapped
+// into the B position so that its transform is computed once.
+NttJob uniform_job(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t N, uint32_t L_a, uint32_t L_b,
+                   uint32_t stride_A, uint32_t stride_B, uint32_t stride_C) {
+    NttJob j;
+    if (stride_A == 0 && stride_B != 0) {
+        std::swap(A, B);
+        std::swap(L_a, L_b);
+        std::swap(stride_A, stride_B);
+    }
+    j.n_ops = 2;
+    j.n_prods = 1;
+    j.ops[0] = NttJob::Op{A, L_a, stride_A};
+    j.ops[1] = NttJob::Op{B, L_b, stride_B};
+    j.prods[0].a = 0;
+    j.prods[0].b = 1;
+    j.prods[0].C = C;
+    j.prods[0].sC = stride_C;
+    j.prods[0].N = N;
+    j.k = transform_log2(L_a, L_b);
+    return j;
+}
+
+// Two products sharing B: C1 = A1 B (N1 rows), C2 = A2 B (N2 rows).
+NttJob shared_job(const uint32_t *A1, uint32_t L1, uint32_t s1, uint32_t N1, uint32_t *C1, uint32_t sC1,
+                  const uint32_t *A2, uint32_t L2, uint32_t s2, uint32_t N2, uint32_t *C2, uint32_t sC2,
+                  const uint32_t *B, uint32_t Lb, uint32_t sB) {
+    NttJob j;
+    j.n_ops = 3;
+    j.n_prods = 2;
+    j.ops[0] = NttJob::Op{A1, L1, s1};
+    j.ops[1] = NttJob::Op{B, Lb, sB};
+    j.ops[2] = NttJob::Op{A2, L2, s2};
+    j.prods[0].a = 0;
+    j.prods[0].b = 1;
+    j.prods[0].C = C1;
+    j.prods[0].sC = sC1;
+    j.prods[0].N = N1;
+    j.prods[1].a = 2;
+    j.prods[1].b = 1;
+    j.prods[1].C = C2;
+    j.prods[1].sC = sC2;
+    j.prods[1].N = N2;
+    j.k = transform_log2(L1, Lb);
+    return j;
+}
+
 }  // namespace
+
+size_t ntt_workspace_bytes(uint32_t N, uint32_t L_a, uint32_t L_b) {
+    if (!ntt_supports(L_a, L_b) || N == 0) return 0;
+    // rows at a stride: the most any call of this shape needs
+    return job_workspace_bytes(uniform_job(nullptr, nullptr, nullptr, N, L_a, L_b, L_a, L_b, L_a + L_b));
+}
 
 hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, void *workspace,
                           size_t workspace_bytes, const NTTPrecomputedTables &tables, uint32_t N,
@@ -994,23 +1203,51 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
                           uint32_t stride_C, hipStream_t stream) {
     if (N == 0) return hipSuccess;
     if (!ntt_supports(L_a, L_b) || tables.roots == nullptr) return hipErrorInvalidValue;
-    NttJob j;
-    j.A = A;
-    j.B = B;
-    j.C = C;
-    j.La = L_a;
-    j.Lb = L_b;
-    j.sA = stride_A;
-    j.sB = stride_B;
-    j.sC = stride_C;
-    j.N = N;
-    j.k = transform_log2(L_a, L_b);
-    return run_ntt(j, workspace, workspace_bytes, tables, stream);
+    return run_ntt(uniform_job(A, B, C, N, L_a, L_b, stride_A, stride_B, stride_C), workspace, workspace_bytes,
+                   tables, stream);
+}
+
+bool ntt_shared_supports(uint32_t L1, uint32_t L2, uint32_t Lb) {
+    if (!ntt_supports(L1, Lb) || !ntt_supports(L2, Lb)) return false;
+    const int k = transform_log2(L1, Lb);
+    return k == transform_log2(L2, Lb) && make_plan(k).levels > 1;
+}
+
+size_t ntt_shared_workspace_bytes(uint32_t L1, uint32_t N1, uint32_t L2, uint32_t N2, uint32_t Lb, uint32_t sB) {
+    if (!ntt_shared_supports(L1, L2, Lb)) return 0;
+    return job_workspace_bytes(shared_job(nullptr, L1, L1, N1, nullptr, L1 + Lb, nullptr, L2, L2, N2, nullptr,
+                                          L2 + Lb, nullptr, Lb, sB));
+}
+
+hipError_t launch_mul_ntt_shared(const uint32_t *A1, uint32_t L1, uint32_t s1, uint32_t N1, uint32_t *C1,
+                                 uint32_t sC1, const uint32_t *A2, uint32_t L2, uint32_t s2, uint32_t N2,
+                                 uint32_t *C2, uint32_t sC2, const uint32_t *B, uint32_t Lb, uint32_t sB,
+                                 void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
+                                 hipStream_t stream) {
+    if (!ntt_shared_supports(L1, L2, Lb) || tables.roots == nullptr) return hipErrorInvalidValue;
+    return run_ntt(shared_job(A1, L1, s1, N1, C1, sC1, A2, L2, s2, N2, C2, sC2, B, Lb, sB), workspace,
+                   workspace_bytes, tables, stream);
 }
 
 int ntt_transform_log2(uint32_t L_a, uint32_t L_b) { return transform_log2(L_a, L_b); }
 
-size_t ntt_ragged_workspace_bytes(uint32_t N, int log2K) { return job_workspace_bytes(N, log2K, true, 0); }
+namespace {
+NttJob ragged_job(const GpupiMulItem *items, const GpupiMulItem *host_items, uint32_t N, int log2K) {
+    NttJob j;
+    j.n_ops = 2;
+    j.n_prods = 1;
+    j.prods[0].N = N;
+    j.ragged = true;
+    j.items = items;
+    j.host_items = host_items;
+    j.k = log2K;
+    return j;
+}
+}  // namespace
+
+size_t ntt_ragged_workspace_bytes(uint32_t N, int log2K) {
+    return job_workspace_bytes(ragged_job(nullptr, nullptr, N, log2K));
+}
 
 hipError_t launch_mul_ntt_ragged(const GpupiMulItem *items, const GpupiMulItem *host_items, uint32_t N, int log2K,
                                  void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
@@ -1018,12 +1255,7 @@ hipError_t launch_mul_ntt_ragged(const GpupiMulItem *items, const GpupiMulItem *
     if (N == 0) return hipSuccess;
     if (items == nullptr || host_items == nullptr || tables.roots == nullptr || log2K < kMinLog2 || log2K > kMaxLog2)
         return hipErrorInvalidValue;
-    NttJob j;
-    j.items = items;
-    j.host_items = host_items;
-    j.N = N;
-    j.k = log2K;
-    return run_ntt(j, workspace, workspace_bytes, tables, stream);
+    return run_ntt(ragged_job(items, host_items, N, log2K), workspace, workspace_bytes, tables, stream);
 }
 
 }  // namespace gpupi

@@ -160,6 +160,13 @@ hipError_t p_mul(BatchMPContext *ctx, const char *label, uint32_t n1, const Batc
                  [&] { return batch_mp_mul(ctx, A, B, C); });
 }
 
+// C1 = A1 B and C2 = A2 B with B's transform shared (one stage entry).
+hipError_t p_mul_shared(BatchMPContext *ctx, const char *label, uint32_t n1, const BatchMPArray &A1,
+                        const BatchMPArray &A2, const BatchMPArray &B, const BatchMPArray &C1, const BatchMPArray &C2) {
+    return stage(ctx, "mul", label, n1, A2.batch_size, A2.length, B.length, mul_uses_ntt(ctx, A2, B),
+                 [&] { return batch_mp_mul_shared(ctx, A1, A2, B, C1, C2); });
+}
+
 hipError_t compact(BatchMPContext *ctx, BatchMPArray &a, const char *label = "", uint32_t n1 = 0) {
     return stage(ctx, "compact", label, n1, a.batch_size, a.length, 0, false,
                  [&] { return batch_mp_array_compact(ctx, &a); });
@@ -361,29 +368,30 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     const uint32_t ignore_last = (n1 <= N) ? 1u : 0u;
     const uint32_t half = n1 / 2;
     BatchMPArray Pn, Qn, Rn, R1, R2;
-    // P: the last group's P is never used once rows are whole groups
-    GPUPI_RETURN_IF(t.bPn.view(half - ignore_last, P.length * 2, Pn));
-    BatchMPArray Pe = view_of(P.data, P.length, half - ignore_last, P.stride * 2);
-    BatchMPArray Po = view_of(P.data + P.stride, P.length, half - ignore_last, P.stride * 2);
-    if (Po.batch_size > 0) {
-        GPUPI_RETURN_IF(p_mul(ctx, "P", n1, Pe, Po, Pn));
-        if (compacting) GPUPI_RETURN_IF(compact(ctx, Pn, "P", n1));
-    }
-    GPUPI_RETURN_IF(t.bQn.view(half, Q.length * 2, Qn));
+    const BatchMPArray Pe = view_of(P.data, P.length, half, P.stride * 2);
+    const BatchMPArray Po = view_of(P.data + P.stride, P.length, half - ignore_last, P.stride * 2);
     const BatchMPArray Qe = view_of(Q.data, Q.length, half, Q.stride * 2);
     const BatchMPArray Qo = view_of(Q.data + Q.stride, Q.length, half, Q.stride * 2);
-    GPUPI_RETURN_IF(p_mul(ctx, "Q", n1, Qe, Qo, Qn));
-    if (compacting) GPUPI_RETURN_IF(compact(ctx, Qn, "Q", n1));
     const BatchMPArray Re = view_of(R.data, R.length, half, R.stride * 2);
     const BatchMPArray Ro = view_of(R.data + R.stride, R.length, half, R.stride * 2);
+    // P = P_e P_o (the last group's P is never used once rows are whole
+    // groups) and P*R = P_e R_o share P_e's transform
+    GPUPI_RETURN_IF(t.bPn.view(half - ignore_last, P.length * 2, Pn));
+    GPUPI_RETURN_IF(t.bR2.view(half, R.length + P.length, R2));
+    if (Po.batch_size > 0) {
+        GPUPI_RETURN_IF(p_mul_shared(ctx, "P,P*R", n1, Po, Ro, Pe, Pn, R2));
+        if (compacting) GPUPI_RETURN_IF(compact(ctx, Pn, "P", n1));
+    } else {
+        GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, Pe, Ro, R2));
+    }
+    // Q = Q_e Q_o and R*Q = R_e Q_o share Q_o's transform
+    GPUPI_RETURN_IF(t.bQn.view(half, Q.length * 2, Qn));
     GPUPI_RETURN_IF(t.bR1.view(half, R.length + Q.length, R1));
-    GPUPI_RETURN_IF(p_mul(ctx, "R*Q", n1, Re, Qo, R1));
+    GPUPI_RETURN_IF(p_mul_shared(ctx, "Q,R*Q", n1, Qe, Re, Qo, Qn, R1));
+    if (compacting) GPUPI_RETURN_IF(compact(ctx, Qn, "Q", n1));
     // R(a,c) = R(a,b) Q(b,c) 2^(15 terms(b,c)) +- P(a,b) R(b,c)
     const uint32_t terms_right = (n / n1) + ((n / n1) >> 4);
     const uint32_t shift = 15u * terms_right;
-    Pe.batch_size = half;
-    GPUPI_RETURN_IF(t.bR2.view(half, R.length + P.length, R2));
-    GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, Pe, Ro, R2));
     const uint32_t shifted_len = R1.length + (shift >> 5) + ((shift & 31u) != 0);
     const uint32_t rlen = std::max(shifted_len, R2.length) + (subtract ? 0u : 1u);
     GPUPI_RETURN_IF(t.bRn.view(half, rlen, Rn));

@@ -116,6 +116,17 @@ hipStream_t batch_mp_get_stream(const BatchMPContext *ctx);
  * batch_arith.cu:182). */
 hipError_t batch_mp_set_ntt_threshold(BatchMPContext *ctx, uint32_t min_total_limbs);
 
+/* Extension (no reference counterpart): two products with a common operand,
+ * C1 = A1 * B and C2 = A2 * B, row for row (C1.length = A1.length +
+ * B.length, likewise C2).  A1/C1 and A2/C2 have equal batch sizes, which may
+ * differ; B has at least the larger batch size, or is one row at stride 0.
+ * Results equal two batch_mp_mul calls; when both products take the NTT
+ * path with one transform size, B's transform is computed once (the
+ * binary-splitting tree multiplies P_e by P_o and R_o, and Q_o by Q_e and
+ * R_e, pi_bs_gpu.cu:637-711).  Outputs must not alias inputs or each other. */
+hipError_t batch_mp_mul_shared(BatchMPContext *ctx, BatchMPArray A1, BatchMPArray A2, BatchMPArray B,
+                               BatchMPArray C1, BatchMPArray C2);
+
 #ifdef __cplusplus
 }
 

@@ -134,6 +134,86 @@ def test_ntt_every_transform_size(ctx, k):
         g.check(ctx.set_ntt_threshold(384))
 
 
+def device_mul_shared(ctx, A1, A2, B, b_rows=None, b_broadcast=False):
+    # C1 = A1 B, C2 = A2 B through one batch_mp_mul_shared call; B has
+    # b_rows >= max(rows) rows, or one row at stride 0
+    n1, n2 = A1.shape[0], A2.shape[0]
+    lb = B.shape[1]
+    b1, A1a = g.upload_rows(A1, A1.shape[1] + 2)
+    b2, A2a = g.upload_rows(A2, A2.shape[1] + 1)
+    bb, Ba = g.upload_rows(B[:1] if b_broadcast else B, lb + 3)
+    if b_broadcast:
+        Ba = Ba.view(batch_size=max(n1, n2), stride=0)
+    outs = []
+    arrs = []
+    for n, la in ((n1, A1.shape[1]), (n2, A2.shape[1])):
+        lc = la + lb
+        bc, Ca = g.upload_rows(np.full((n, lc + 1), PAD, dtype=np.uint32), lc + 1)
+        Ca.length = lc
+        outs.append((bc, n, lc))
+        arrs.append(Ca)
+    g.check(ctx.mul_shared(A1a, A2a, Ba, arrs[0], arrs[1]), "mul_shared")
+    g.synchronize()
+    res = []
+    for bc, n, lc in outs:
+        full = bc.download(np.uint32, count=n * (lc + 1)).reshape(n, lc + 1)
+        assert np.all(full[:, lc:] == PAD), "write past the product"
+        res.append(full[:, :lc])
+    return res
+
+
+SHARED = [  # (N1, L1, N2, L2, rows of B, L_b)
+    (7, 3000, 8, 4000, 8, 2500),          # two levels, one transform size, N1 < N2
+    (3, 300000, 3, 290000, 4, 240000),    # two levels, one row of B unused
+    (2, 2500000, 3, 2200000, 3, 2000000),  # three levels
+    (1, 40000, 2, 30000, 2, 20000),
+    (5, 100, 5, 90, 5, 80),               # schoolbook: two plain products
+    (4, 1000, 4, 5000, 4, 700),           # different transform sizes: two plain products
+    (6, 600, 6, 700, 6, 900),             # one tile each: two plain products
+]
+
+
+@pytest.mark.parametrize("n1,l1,n2,l2,nb,lb", SHARED, ids=[f"{a}x{b}_{c}x{d}_B{e}x{f}" for a, b, c, d, e, f in SHARED])
+def test_mul_shared(ctx, n1, l1, n2, l2, nb, lb):
+    rng = np.random.default_rng(n1 * 31 + l1)
+    A1 = rand_rows(rng, n1, l1)
+    A2 = rand_rows(rng, n2, l2)
+    B = rand_rows(rng, nb, lb)
+    C1, C2 = device_mul_shared(ctx, A1, A2, B)
+    assert np.array_equal(C1, oracle_mul(A1, B[:n1]))
+    assert np.array_equal(C2, oracle_mul(A2, B[:n2]))
+
+
+def test_mul_shared_broadcast_and_tree_shape(ctx):
+    rng = np.random.default_rng(77)
+    # B one row at stride 0
+    A1 = rand_rows(rng, 5, 9000)
+    A2 = rand_rows(rng, 4, 7000)
+    B = rand_rows(rng, 1, 6000)
+    C1, C2 = device_mul_shared(ctx, A1, A2, B, b_broadcast=True)
+    assert np.array_equal(C1, oracle_mul(A1, np.repeat(B, 5, axis=0)))
+    assert np.array_equal(C2, oracle_mul(A2, np.repeat(B, 4, axis=0)))
+    # carry chains: all-ones operands
+    A1 = rand_rows(rng, 2, 50000, "ones")
+    A2 = rand_rows(rng, 3, 60000, "ones")
+    B = rand_rows(rng, 3, 40000, "ones")
+    C1, C2 = device_mul_shared(ctx, A1, A2, B)
+    assert np.array_equal(C1, oracle_mul(A1, B[:2]))
+    assert np.array_equal(C2, oracle_mul(A2, B))
+
+
+def test_mul_shared_rejects_bad_shapes(ctx):
+    rng = np.random.default_rng(3)
+    A = rand_rows(rng, 4, 5000)
+    B = rand_rows(rng, 2, 5000)  # fewer rows than the products
+    ba, Aa = g.upload_rows(A)
+    bb, Ba = g.upload_rows(B)
+    bc1, C1 = g.upload_rows(np.zeros((4, 10000), dtype=np.uint32))
+    bc2, C2 = g.upload_rows(np.zeros((4, 10000), dtype=np.uint32))
+    assert ctx.mul_shared(Aa, Aa, Ba, C1, C2) != 0
+    assert ctx.mul_shared(Aa, Aa, Aa, C1, C1) != 0  # outputs alias each other
+
+
 def valid_splits(k):
     # every column split the kernels cover: (m0, m1) with m1 = 0 for two levels
     out = []
