@@ -92,6 +92,17 @@ def test_pi_matches_oracle(ctx, digits):
     assert dec == odec
 
 
+@pytest.mark.parametrize("digits", [1, 64, 763, 768, 20000])
+def test_pi_digits_match_golden_fixture(ctx, digits):
+    # tests/golden/pi_digits.txt: pi from an independent implementation
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pi_digits.txt")) as f:
+        want = f.read().strip()
+    for conv in (g.PREC_REQUESTED, g.PREC_ROUNDED):
+        _, dec, _ = ctx.pi(digits, conv)
+        assert "3" + dec == want[:digits + 1], conv
+
+
 @pytest.mark.parametrize("digits", [1000, 123457])
 def test_pi_readme_convention(ctx, digits):
     ret, dec, res = ctx.pi(digits, g.PREC_ROUNDED)
