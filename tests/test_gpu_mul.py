@@ -318,3 +318,13 @@ def test_ragged_rejects_aliasing(ctx):
     buf = g.DeviceBuffer.from_array(np.ones(64, dtype=np.uint32))
     assert ctx.mul_ragged([(buf.ptr, buf.ptr + 64, buf.ptr, 8, 8)]) != 0
     assert ctx.mul_ragged([(None, buf.ptr, buf.ptr + 128, 8, 8)]) != 0
+
+
+def test_tuner_times_every_split_and_restores_the_plan(ctx):
+    from gpupi_amd import tune
+    ops = tune.Operands(20, seed=5)
+    before = g.ntt_get_split(15)
+    res = tune.tune_splits(ctx, ops, 15, reps=1)
+    assert sorted(lv for lv, _ in res) == sorted((m0, 15 - m0) for m0, _ in tune.splits(15))
+    assert all(ms > 0 for _, ms in res) and res[0][1] <= res[-1][1]
+    assert g.ntt_get_split(15) == before

@@ -82,3 +82,19 @@ def test_ntt_split_table_validates_and_restores():
     assert lib.gpupi_ntt_set_split(28, 9, 8) == 0 and levels(28) == (9, 8, 11)
     assert lib.gpupi_ntt_set_split(20, 0, 0) == 0 and levels(20) == (8, 12)
     assert lib.gpupi_ntt_set_split(28, 0, 0) == 0 and levels(28) == (8, 8, 12)
+
+
+def test_tuner_enumerates_exactly_the_accepted_splits():
+    import sys
+    sys.path.insert(0, ROOT)
+    from gpupi_amd import tune
+    lib = ctypes.CDLL(LIB)
+    lib.gpupi_ntt_set_split.argtypes = [ctypes.c_int] * 3
+    for k in range(13, 31):
+        accepted = set()
+        for m0 in range(1, 13):
+            for m1 in ([0] if k <= 21 else range(1, 13)):
+                if lib.gpupi_ntt_set_split(k, m0, m1) == 0:
+                    accepted.add((m0, m1))
+        assert lib.gpupi_ntt_set_split(k, 0, 0) == 0
+        assert accepted == set(tune.splits(k)), k
