@@ -2,17 +2,23 @@
 // formats as the reference CLI (pi_bs_gpu.cu:2073-2431):
 //
 //   pi_bs_gpu <digits> [--benchmark|--profile-stages|--full-print] [--readme-prec]
+//             [--output FILE] [--verify-bbp POS]...
 //
-// default: "3" + digits on stdout, the stats line on stderr.  --readme-prec
-// (an addition) sizes the precision from the rounded-up digit count, the
+// default: "3" + digits on stdout, the stats line on stderr.  Additions:
+// --readme-prec sizes the precision from the rounded-up digit count, the
 // convention under which the reference README's RET length and hash were
-// published.
+// published; --output writes "3." and the digits to FILE instead of stdout;
+// --verify-bbp checks eight hexadecimal digits of RET from fractional
+// position POS (1-based) against the Bailey-Borwein-Plouffe digit-extraction
+// formula evaluated independently on the host (exit status 2 on a mismatch).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "batch_arith.h"
@@ -58,7 +64,61 @@ void print_hex_preview(const char *label, const std::vector<uint32_t> &v) {
 }
 
 void usage(const char *prog) {
-    fprintf(stderr, "Usage: %s <target_digits> [--benchmark|--profile-stages|--full-print] [--readme-prec]\n", prog);
+    fprintf(stderr,
+            "Usage: %s <target_digits> [--benchmark|--profile-stages|--full-print] [--readme-prec] "
+            "[--output FILE] [--verify-bbp POS]...\n",
+            prog);
+}
+
+// 16^e mod m for m < 2^32 (products fit in 64 bits).
+uint64_t pow16_mod(uint64_t e, uint64_t m) {
+    if (m == 1) return 0;
+    uint64_t r = 1, b = 16 % m;
+    while (e) {
+        if (e & 1) r = r * b % m;
+        b = b * b % m;
+        e >>= 1;
+    }
+    return r;
+}
+
+// frac(sum_k 16^(n-k) / (8k + j)), k >= 0 (Bailey, Borwein and Plouffe,
+// "On the rapid computation of various polylogarithmic constants", 1997).
+long double bbp_series(uint64_t n, uint32_t j) {
+    long double s = 0.0L;
+    for (uint64_t k = 0; k <= n; ++k) {
+        const uint64_t m = 8 * k + j;
+        s += (long double)pow16_mod(n - k, m) / (long double)m;
+        s -= floorl(s);
+    }
+    long double p = 1.0L / 16.0L;
+    for (uint64_t k = n + 1; k <= n + 24; ++k, p /= 16.0L) s += p / (long double)(8 * k + j);
+    return s - floorl(s);
+}
+
+// Eight hexadecimal digits of pi from fractional position n + 1:
+// frac(16^n pi) = frac(4 S1 - 2 S4 - S5 - S6).
+uint32_t bbp_hex8(uint64_t n) {
+    const uint32_t js[4] = {1, 4, 5, 6};
+    long double s[4];
+    std::vector<std::thread> th;
+    for (int i = 0; i < 4; ++i) th.emplace_back([&, i] { s[i] = bbp_series(n, js[i]); });
+    for (auto &t : th) t.join();
+    long double x = 4.0L * s[0] - 2.0L * s[1] - s[2] - s[3];
+    x -= floorl(x);
+    return (uint32_t)(x * 4294967296.0L);
+}
+
+// Eight hexadecimal digits of RET / 2^(32 (len - 1)) from fractional
+// position p (1-based): RET's top limb is the integer part.
+uint32_t ret_hex8(const std::vector<uint32_t> &ret, uint64_t p) {
+    uint32_t v = 0;
+    for (uint64_t q = p; q < p + 8; ++q) {
+        const uint64_t limb = ret.size() - 2 - (q - 1) / 8;
+        const uint32_t nib = (ret[limb] >> (28 - 4 * ((q - 1) % 8))) & 15u;
+        v = (v << 4) | nib;
+    }
+    return v;
 }
 
 bool check(hipError_t e, const char *what) {
@@ -113,17 +173,29 @@ int main(int argc, char **argv) {
     bool benchmark = false, profile = false, full = false;
     int convention = GPUPI_PREC_REQUESTED;
     int mode_flags = 0;
+    const char *output = nullptr;
+    std::vector<uint64_t> bbp_positions;
     for (int i = 2; i < argc; ++i) {
         if (!strcmp(argv[i], "--benchmark")) benchmark = true, ++mode_flags;
         else if (!strcmp(argv[i], "--profile-stages")) profile = true, ++mode_flags;
         else if (!strcmp(argv[i], "--full-print")) full = true, ++mode_flags;
         else if (!strcmp(argv[i], "--readme-prec")) convention = GPUPI_PREC_ROUNDED;
-        else {
+        else if (!strcmp(argv[i], "--output") && i + 1 < argc) output = argv[++i];
+        else if (!strcmp(argv[i], "--verify-bbp") && i + 1 < argc) {
+            char *e = nullptr;
+            const long long pos = strtoll(argv[++i], &e, 10);
+            // 16^e mod (8k + j) needs 8n + 6 < 2^32
+            if (e == argv[i] || *e != '\0' || pos < 1 || pos > 500000000ll) {
+                usage(argv[0]);
+                return 1;
+            }
+            bbp_positions.push_back((uint64_t)pos);
+        } else {
             usage(argv[0]);
             return 1;
         }
     }
-    if (mode_flags > 1) {
+    if (mode_flags > 1 || (output != nullptr && mode_flags > 0)) {
         usage(argv[0]);
         return 1;
     }
@@ -177,13 +249,38 @@ int main(int argc, char **argv) {
             printf("\nDEC = 3");
             fwrite(digits.data(), 1, requested, stdout);
             printf("\n");
+        } else if (output != nullptr) {
+            FILE *f = fopen(output, "wb");
+            const bool wrote = f != nullptr && fputs("3.", f) >= 0 &&
+                               fwrite(digits.data(), 1, requested, f) == requested && fputc('\n', f) != EOF;
+            if (f == nullptr || fclose(f) != 0 || !wrote) {
+                fprintf(stderr, "could not write %s\n", output);
+                ok = false;
+            }
         } else {
             fputc('3', stdout);
             fwrite(digits.data(), 1, requested, stdout);
             fputc('\n', stdout);
         }
     }
+    int status = ok ? 0 : 1;
+    // the last two limbs of RET carry the truncation and inverse rounding
+    const uint64_t exact_hex = ret.size() >= 4 ? 8ull * (ret.size() - 3) : 0;
+    for (uint64_t p : bbp_positions) {
+        if (p + 7 > exact_hex) {
+            fprintf(stderr, "BBP position %llu is beyond RET's exact hexadecimal digits (%llu)\n",
+                    (unsigned long long)p, (unsigned long long)exact_hex);
+            status = status ? status : 1;
+            continue;
+        }
+        const uint32_t want = bbp_hex8(p - 1), got = ret_hex8(ret, p);
+        // long double carries ~64 bits; compare the leading 6 hex digits
+        const bool match = (want >> 8) == (got >> 8);
+        fprintf(stderr, "BBP hex digits at position %llu: formula %08x RET %08x %s\n", (unsigned long long)p, want,
+                got, match ? "match" : "MISMATCH");
+        if (!match) status = 2;
+    }
     gpupi_pi_free(&res);
     batch_mp_destroy(ctx);
-    return 0;
+    return status;
 }

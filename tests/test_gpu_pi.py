@@ -144,6 +144,28 @@ def decimal_hash(digits: str) -> int:
     return v
 
 
+def test_cli_output_file_and_bbp_check(tmp_path):
+    # --output writes "3." and the digits; --verify-bbp compares RET's hex
+    # digits with the BBP digit-extraction formula evaluated on the host
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(g.LIB_PATH), "pi_bs_gpu")
+    out_file = tmp_path / "pi.txt"
+    r = subprocess.run([exe, "100000", "--output", str(out_file), "--verify-bbp", "1", "--verify-bbp", "1000",
+                        "--verify-bbp", "40000", "--verify-bbp", "82000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stderr.count(" match") == 4 and "MISMATCH" not in r.stderr
+    assert "formula 243f6a88 RET 243f6a88" in r.stderr
+    text = out_file.read_text()
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pi_digits.txt")) as f:
+        want = f.read().strip()
+    assert len(text) == 100003 and text.startswith("3.") and text.endswith("\n")
+    assert "3" + text[2:20002] == want[:20001]
+    # a position past RET's exact digits is refused
+    r = subprocess.run([exe, "1000", "--verify-bbp", "100000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "beyond" in r.stderr
+
+
 def test_cli_profile_stages():
     import os
     import re

@@ -49,7 +49,10 @@ def test_library_does_not_link_the_oracle():
     assert needed and not any("gmp" in n or "oracle" in n for n in needed), needed
 
 
-@pytest.mark.parametrize("args", [[], ["abc"], ["10", "--bogus"], ["-5"], ["10", "--benchmark", "--full-print"]])
+@pytest.mark.parametrize("args", [[], ["abc"], ["10", "--bogus"], ["-5"], ["10", "--benchmark", "--full-print"],
+                                  ["10", "--verify-bbp"], ["10", "--verify-bbp", "0"], ["10", "--verify-bbp", "x"],
+                                  ["10", "--verify-bbp", "600000000"], ["10", "--output"],
+                                  ["10", "--output", "f", "--benchmark"]])
 def test_cli_rejects_bad_arguments(args):
     r = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "Usage" in r.stderr
