@@ -504,65 +504,69 @@ __global__ void __launch_bounds__(256) ntt_col_inv_kernel(PassArgs a, int level)
     for (int u = 0; u < 16; ++u) T1[ct.off(u)] = v[u];
 }
 
-// First inverse column level: both primes (two tiles), 1/K, Garner, limbs of
-// each run of <= 16 adjacent coefficients and the run's spill word.
+// First inverse column level: both primes through one tile, 1/K, Garner,
+// and the limbs of each run of adjacent coefficients with the run's spill
+// word.  A thread's element u is (row r0 + u 2^(M-4), column g) and lanes
+// hold adjacent columns, so a run of 2^RL adjacent coefficients sits in
+// 2^RL adjacent lanes: coefficient c = w0 + 2^32 w1 + 2^64 w2 adds to the
+// limb column sums s_j = w0_j + w1_(j-1) + w2_(j-2), which become limbs with
+// one neighbour shift and one ballot carry resolution per run; what passes
+// the run's top goes to the spill word (the carry engine adds it).
 template <int M>
 __global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
-    __shared__ double l0[kLdsSlots];
-    __shared__ double l1[kLdsSlots];
+    __shared__ double lds[kLdsSlots];
     constexpr int LC = kTileLog2 - M;
     constexpr int RL = run_log2_for(M);
-    constexpr uint32_t kRunLen = 1u << RL;
+    constexpr uint32_t kRun = 1u << RL;
+    static_assert(kRun <= (1u << LC) && kRun <= kWave, "a run is one tile row inside one wave");
     const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan));
-    const uint32_t t = bp.t, prod = bp.row;
+    const uint32_t prod = bp.row;
     const NttProd pr = a.prods[bp.seg];
     const NttOp oa = a.ops[pr.a];
     const int log2K = a.plan.log2K;
-    const ColThread<M> ct(a.plan, 0, t);
-    {
-        const double *T0 = a.T + oa.base + (size_t)prod * (1ull << log2K);
-        double v0[16], v1[16];
-        ColTw tw0, tw1;
-        col_inv_load<M>(v0, v1, tw0, tw1, a, ct, T0, T0 + oa.pstride);
-        col_inv_prime<M, 0>(v0, l0, ct, a.tables, tw0);
-        const double kinv0 = c_ntt.kinv[0][log2K];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) l0[ct.lds0 + 272u * u] = mulp(v0[u], kinv0, mod_of(0));
-        col_inv_prime<M, 1>(v1, l1, ct, a.tables, tw1);
-        const double kinv1 = c_ntt.kinv[1][log2K];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) l1[ct.lds0 + 272u * u] = mulp(v1[u], kinv1, mod_of(1));
-    }
+    const ColThread<M> ct(a.plan, 0, bp.t);
+    const double *T0 = a.T + oa.base + (size_t)prod * (1ull << log2K);
+    double v0[16], v1[16];
+    ColTw tw0, tw1;
+    col_inv_load<M>(v0, v1, tw0, tw1, a, ct, T0, T0 + oa.pstride);
+    col_inv_prime<M, 0>(v0, lds, ct, a.tables, tw0);
     __syncthreads();
-    const uint32_t col0 = col_tile(a.plan, 0, t).col0;
-    const int ls = log2K - M;
+    col_inv_prime<M, 1>(v1, lds, ct, a.tables, tw1);
+
+    const double kinv0 = c_ntt.kinv[0][log2K], kinv1 = c_ntt.kinv[1][log2K];
+    const uint32_t lane = threadIdx.x & (kWave - 1u);
+    const uint32_t jr = lane & (kRun - 1u);  // position inside the run
+    uint64_t last_mask = 0;                   // last lanes of runs pass no carry on
+#pragma unroll
+    for (uint32_t l = kRun - 1; l < kWave; l += kRun) last_mask |= 1ull << l;
     uint32_t *out = pr.C + (size_t)prod * pr.sC;
     const uint32_t Lc = pr.Lc;
     uint64_t *spill = a.spill + pr.spill_base + (size_t)prod * ((1ull << log2K) >> RL);
-    const bool vec = (((uintptr_t)out) & 15u) == 0;
+    const uint64_t n_u0 = ct.off0;  // coefficient index of element u = 0
 #pragma unroll
-    for (uint32_t q = threadIdx.x; q < (kTile >> RL); q += 256u) {
-        const Run run = run_of(M, q);
-        const uint32_t e0 = lds_index((run.r << LC) + run.g0);
-        const uint64_t n0 = ((uint64_t)run.r << ls) + col0 + run.g0;
-        uint32_t limb[kRunLen];
-        unsigned __int128 pending = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < kRunLen; ++j) {
-            pending += recombine(l0[e0 + j], l1[e0 + j]);
-            limb[j] = (uint32_t)pending;
-            pending >>= 32;
+    for (int u = 0; u < 16; ++u) {
+        const unsigned __int128 c =
+            recombine(mulp(v0[u], kinv0, mod_of(0)), mulp(v1[u], kinv1, mod_of(1)));
+        const uint32_t w0 = (uint32_t)c, w1 = (uint32_t)(c >> 32), w2 = (uint32_t)(c >> 64);
+        const uint32_t w1a = __shfl_up(w1, 1), w2a = __shfl_up(w2, 1), w2b = __shfl_up(w2, 2);
+        const uint64_t s = (uint64_t)w0 + (jr >= 1 ? w1a : 0u) + (jr >= 2 ? w2b : 0u);
+        const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+        const uint32_t hia = __shfl_up(hi, 1);
+        const uint64_t t = (uint64_t)lo + (jr >= 1 ? hia : 0u);
+        const uint32_t lo2 = (uint32_t)t;
+        const bool gen = (t >> 32) != 0, prop = lo2 == 0xffffffffu;
+        const uint64_t G = ballot64(gen) & ~last_mask, P = ballot64(prop) & ~last_mask;
+        uint32_t unused;
+        const uint64_t cm = resolve_carries(G, P, 0u, &unused);
+        const uint32_t ci = (uint32_t)(cm >> lane) & 1u;
+        const uint64_t n = n_u0 + ((uint64_t)u << ct.ustep);
+        if (n < Lc) out[n] = lo2 + ci;
+        if (jr == kRun - 1) {
+            // V >> 32 kRun: the carry out of the top limb, the top column's
+            // high part and the words of the last two coefficients past the run
+            const uint64_t cout = (gen || (prop && ci)) ? 1u : 0u;
+            spill[(n - jr) >> RL] = cout + hi + (uint64_t)w1 + (uint64_t)w2a + ((uint64_t)w2 << 32);
         }
-        if (vec && n0 + kRunLen <= Lc) {
-#pragma unroll
-            for (uint32_t j = 0; j < kRunLen; j += 4)
-                *reinterpret_cast<uint4 *>(out + n0 + j) = make_uint4(limb[j], limb[j + 1], limb[j + 2], limb[j + 3]);
-        } else {
-#pragma unroll
-            for (uint32_t j = 0; j < kRunLen; ++j)
-                if (n0 + j < Lc) out[n0 + j] = limb[j];
-        }
-        spill[n0 >> RL] = (uint64_t)pending;
     }
 }
 

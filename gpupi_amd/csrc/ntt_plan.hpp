@@ -233,21 +233,13 @@ GPUPI_HD uint32_t xcd_tile(uint32_t b, uint32_t n) {
 }
 
 // Run length of the final inverse column level (limbs per run): the C
-// adjacent coefficients of one tile row, cut into runs of at most 16 so
-// that every thread of the block owns at least one run.
+// adjacent coefficients of one tile row, cut into runs of at most 16.  Thread
+// tid of a first-level tile holds coefficients col_thread_offset(tid) +
+// u 2^(k-4) (u < 16); the 2^run_log2 threads with equal tid >> run_log2 hold
+// adjacent columns of one row, so each run lies in adjacent lanes of one wave.
 GPUPI_HD constexpr int run_log2_for(int lm0) { return (kTileLog2 - lm0) < 4 ? (kTileLog2 - lm0) : 4; }
 GPUPI_HD uint32_t final_run_log2(const Plan &p) {
     return p.levels == 1 ? 4u : (uint32_t)run_log2_for(p.log2M[0]);
-}
-
-// Run q (< 4096 >> run_log2) of a first-level tile: row r, first column g0.
-struct Run {
-    uint32_t r, g0;
-};
-GPUPI_HD Run run_of(int lm0, uint32_t q) {
-    const int lc = kTileLog2 - lm0;
-    const int rl = run_log2_for(lm0);
-    return Run{q >> (lc - rl), (q & ((1u << (lc - rl)) - 1u)) << rl};
 }
 
 }  // namespace ntt

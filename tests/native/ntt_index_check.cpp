@@ -120,29 +120,27 @@ static void check_plan(const Plan &p, bool full) {
         if (full)
             for (uint64_t x = 0; x < K; ++x) EXPECT(seen[x] == 1, "coverage k=%d l=%d x=%llu", k, l, (unsigned long long)x);
         if (l == 0) {
-            // final inverse level: runs of adjacent coefficients (<= 16, within
-            // one tile row) and their spill slots; runs cover every tile element
+            // final inverse level: thread tid's element u is coefficient
+            // col_thread_offset(tid) + u 2^(k-4); the 2^rl lanes of a run hold
+            // adjacent coefficients of one row, the run starts at a multiple
+            // of 2^rl inside the product, and its spill slot exists
             const uint32_t rl = final_run_log2(p);
-            EXPECT(rl <= 4 && (1u << rl) <= C && (kTile >> rl) >= 256, "run length k=%d", k);
-            const int ls = slab_log2(p, 0) - lm;
+            const uint32_t run = 1u << rl;
+            EXPECT(rl <= 4 && run <= C && 64 % run == 0, "run length k=%d", k);
             for (uint32_t t = 0; t < tiles; t += step) {
                 const ColTile ct = col_tile(p, 0, t);
                 EXPECT(ct.slab == 0, "slab 0 at level 0");
-                std::vector<int> hit(kTile, 0);
-                for (uint32_t q = 0; q < (kTile >> rl); ++q) {
-                    const Run run = run_of(lm, q);
-                    const uint64_t n0 = ((uint64_t)run.r << ls) + ct.col0 + run.g0;
-                    EXPECT(run.g0 + (1u << rl) <= C && run.r < (1u << lm), "run inside row k=%d", k);
-                    EXPECT(n0 % (1u << rl) == 0 && n0 + (1u << rl) <= K, "run start k=%d", k);
-                    EXPECT((n0 >> rl) < (K >> rl), "spill slot k=%d", k);
-                    for (uint32_t j = 0; j < (1u << rl); ++j) {
-                        const uint32_t e = run.r * C + run.g0 + j;
-                        if (e < kTile) hit[e]++;
-                        EXPECT(lds_index(e) == lds_index(run.r * C + run.g0) + j, "run lds offsets k=%d", k);
-                        EXPECT(col_offset(p, 0, ct, run.g0 + j, run.r) == n0 + j, "run coefficient k=%d", k);
+                for (uint32_t tid = 0; tid < 256; ++tid) {
+                    const uint32_t jr = tid & (run - 1);
+                    for (uint32_t u = 0; u < 16; ++u) {
+                        const uint64_t n = col_thread_offset(p, 0, ct, tid) + ((uint64_t)u << (k - 4));
+                        const uint64_t n0 = n - jr;
+                        EXPECT(n0 % run == 0 && n0 + run <= K && (n0 >> rl) < (K >> rl), "run start k=%d", k);
+                        for (uint32_t j = 0; j < run; ++j)
+                            EXPECT(col_thread_offset(p, 0, ct, tid - jr + j) + ((uint64_t)u << (k - 4)) == n0 + j,
+                                   "run lanes adjacent k=%d", k);
                     }
                 }
-                for (uint32_t e = 0; e < kTile; ++e) EXPECT(hit[e] == 1, "run coverage k=%d", k);
             }
         }
     }
