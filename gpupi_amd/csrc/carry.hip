@@ -27,7 +27,7 @@ struct Digit {
 
 struct SpillArgs {
     const uint64_t *spill;
-    uint32_t run_len;
+    uint32_t run_log2;  // runs of 2^run_log2 limbs
     uint32_t runs_per_row;
 };
 
@@ -69,10 +69,10 @@ __device__ __forceinline__ Digit digit_at(const AddSubArgs &a, const SpillArgs &
         }
         y = load_or_zero(a.B, row, a.stride_B, j, a.L_b);
     } else {
-        // spill-add: C[j] += word of spill[run] at (run + 1) * run_len (+1)
+        // spill-add: C[j] += word of spill[run] at (run + 1) 2^run_log2 (+1)
         x = load_or_zero(a.A, row, a.stride_A, j, a.L_a);
         y = 0;
-        const uint32_t q = j / sp.run_len, rem = j - q * sp.run_len;
+        const uint32_t q = j >> sp.run_log2, rem = j & ((1u << sp.run_log2) - 1u);
         if (q >= 1 && rem < 2) {
             const uint64_t s = sp.spill[(size_t)row * sp.runs_per_row + (q - 1)];
             y = rem == 0 ? (uint32_t)s : (uint32_t)(s >> 32);
@@ -185,6 +185,86 @@ __global__ void __launch_bounds__(256) carry_fixup_kernel(uint32_t *C, uint32_t 
     }
 }
 
+// The spill-add of NTT products with four limbs per lane (16-byte loads and
+// stores): one wave per chunk of 1024 limbs of one row, the same chunk and
+// flag layout as addsub_chunk_kernel<SpillAdd, 16>.  Each lane settles its
+// four limbs for a carry-in of zero, the lanes' (generate, propagate) pairs
+// go through one ballot resolution per 256 limbs, and the lane adds its
+// carry-in.  Every row must start 16-byte aligned.
+__global__ void __launch_bounds__(256) spill_add_vec_kernel(uint32_t *C, uint32_t stride_C, uint32_t L_c,
+                                                            SpillArgs sp, uint32_t chunks_per_row,
+                                                            uint64_t total_chunks, uint8_t *flags) {
+    constexpr int kSeg = 4;  // 4 x 256 limbs per wave
+    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    if (wave >= total_chunks) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t row = (uint32_t)(wave / chunks_per_row);
+    const uint32_t chunk = (uint32_t)(wave - (uint64_t)row * chunks_per_row);
+    uint32_t *out = C + (size_t)row * stride_C;
+    const uint64_t *rsp = sp.spill + (size_t)row * sp.runs_per_row;
+    const uint32_t run_mask = (1u << sp.run_log2) - 1u;
+    uint32_t x[kSeg][4];
+#pragma unroll
+    for (int s = 0; s < kSeg; ++s) {
+        const uint32_t j0 = chunk * (uint32_t)(kSeg * 4 * kWave) + (uint32_t)s * (4 * kWave) + 4 * lane;
+        if (j0 + 4 <= L_c) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(out + j0);
+            x[s][0] = v.x;
+            x[s][1] = v.y;
+            x[s][2] = v.z;
+            x[s][3] = v.w;
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x[s][t] = j0 + t < L_c ? out[j0 + t] : 0u;
+        }
+    }
+    uint32_t cin = 0;
+    bool all_prop = true;
+#pragma unroll
+    for (int s = 0; s < kSeg; ++s) {
+        const uint32_t j0 = chunk * (uint32_t)(kSeg * 4 * kWave) + (uint32_t)s * (4 * kWave) + 4 * lane;
+        uint32_t r[4];
+        uint32_t c = 0;
+        bool ones = true;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t j = j0 + t;
+            uint32_t y = 0;
+            const uint32_t q = j >> sp.run_log2, rem = j & run_mask;
+            if (q >= 1 && rem < 2 && j < L_c) {
+                const uint64_t w = rsp[q - 1];
+                y = rem == 0 ? (uint32_t)w : (uint32_t)(w >> 32);
+            }
+            const uint32_t d = x[s][t] + y;
+            const uint32_t g = d < y ? 1u : 0u;
+            r[t] = d + c;
+            c = g | (c & (d == 0xffffffffu ? 1u : 0u));
+            ones = ones && r[t] == 0xffffffffu && j < L_c;
+        }
+        const bool valid = j0 < L_c;
+        const uint64_t G = ballot64(valid && c != 0), P = ballot64(valid && ones);
+        uint32_t cout;
+        const uint64_t cm = resolve_carries(G, P, cin, &cout);
+        uint32_t ci = (uint32_t)(cm >> lane) & 1u;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t v = r[t] + ci;
+            ci = ci & (r[t] == 0xffffffffu ? 1u : 0u);
+            r[t] = v;
+        }
+        if (j0 + 4 <= L_c) {
+            *reinterpret_cast<uint4 *>(out + j0) = make_uint4(r[0], r[1], r[2], r[3]);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (j0 + t < L_c) out[j0 + t] = r[t];
+        }
+        all_prop = all_prop && (P == ~0ull);
+        cin = cout;
+    }
+    if (flags != nullptr && lane == 0) flags[wave] = (uint8_t)(cin | (all_prop ? 2u : 0u));
+}
+
 struct ChunkPlan {
     bool thread_rows;
     int seg;
@@ -254,7 +334,7 @@ size_t carry_flags_bytes(uint32_t N, uint32_t L_c) {
 }
 
 hipError_t launch_addsub(const AddSubArgs &a, uint8_t *flags, hipStream_t stream) {
-    const SpillArgs sp{nullptr, 1, 0};
+    const SpillArgs sp{nullptr, 0, 0};
     switch (a.kind) {
         case AddSubKind::Add: return run_addsub<AddSubKind::Add>(a, sp, flags, stream);
         case AddSubKind::Sub: return run_addsub<AddSubKind::Sub>(a, sp, flags, stream);
@@ -280,8 +360,30 @@ hipError_t launch_add_spills(uint32_t *C, const uint64_t *spill, uint32_t N, uin
     a.L_c = L_c;
     a.stride_A = stride_C;
     a.stride_C = stride_C;
-    const SpillArgs sp{spill, run_len, runs_per_row};
-    return run_addsub<AddSubKind::SpillAdd>(a, sp, flags, stream);
+    if (run_len == 0 || (run_len & (run_len - 1)) != 0) return hipErrorInvalidValue;
+    const SpillArgs sp{spill, (uint32_t)__builtin_ctz(run_len), runs_per_row};
+    const ChunkPlan p = plan_for(L_c);
+    if (N == 0 || L_c == 0) return hipSuccess;
+    if (p.thread_rows || p.chunk_len != 4u * 4u * kWave || (((uintptr_t)C) & 15u) != 0 ||
+        (N > 1 && (stride_C & 3u) != 0))
+        return run_addsub<AddSubKind::SpillAdd>(a, sp, flags, stream);
+    const uint64_t total = (uint64_t)N * p.chunks_per_row;
+    const bool need_fix = p.chunks_per_row > 1;
+    if (need_fix && flags == nullptr) return hipErrorInvalidValue;
+    const uint64_t blocks = ceil_div64(total, 4);
+    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(spill_add_vec_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, C, stride_C, L_c, sp,
+                       p.chunks_per_row, total, need_fix ? flags : nullptr);
+    GPUPI_RETURN_IF(hipGetLastError());
+    if (need_fix) {
+        const uint64_t fix_total = (uint64_t)N * (p.chunks_per_row - 1);
+        const uint64_t fblocks = ceil_div64(fix_total, 4);
+        if (fblocks > 0x7fffffffull) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(carry_fixup_kernel<false>, dim3((uint32_t)fblocks), dim3(256), 0, stream, C, stride_C, L_c,
+                           p.chunk_len, p.chunks_per_row, fix_total, (const uint8_t *)flags);
+        GPUPI_RETURN_IF(hipGetLastError());
+    }
+    return hipSuccess;
 }
 
 }  // namespace gpupi

@@ -61,7 +61,8 @@ struct AddSubArgs {
 hipError_t launch_addsub(const AddSubArgs &args, uint8_t *flags, hipStream_t stream);
 
 // C (limbs) += spill words: for each run r (run_len limbs), the 64-bit
-// value spill[r] is added at limb (r + 1) * run_len of the same row.
+// value spill[r] is added at limb (r + 1) * run_len of the same row
+// (run_len a power of two).
 hipError_t launch_add_spills(uint32_t *C, const uint64_t *spill, uint32_t N, uint32_t L_c,
                              uint32_t stride_C, uint32_t run_len, uint32_t runs_per_row,
                              uint8_t *flags, hipStream_t stream);

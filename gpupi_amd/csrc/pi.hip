@@ -76,6 +76,18 @@ struct Buf {
         out.stride = length;
         return hipSuccess;
     }
+    // The same with rows 16-byte aligned (stride a multiple of 4 limbs): the
+    // NTT's final carry pass reads and writes such rows four limbs per lane.
+    hipError_t view_aligned(uint32_t batch, uint32_t length, BatchMPArray &out) {
+        if (batch <= 1) return view(batch, length, out);
+        const uint32_t stride = (length + 3u) & ~3u;
+        GPUPI_RETURN_IF(reserve((size_t)batch * stride));
+        out.data = ptr;
+        out.length = length;
+        out.batch_size = batch;
+        out.stride = stride;
+        return hipSuccess;
+    }
     void swap(Buf &o) {
         std::swap(ptr, o.ptr);
         std::swap(cap, o.cap);
@@ -376,8 +388,14 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     const BatchMPArray Ro = view_of(R.data + R.stride, R.length, half, R.stride * 2);
     // P = P_e P_o (the last group's P is never used once rows are whole
     // groups) and P*R = P_e R_o share P_e's transform
-    GPUPI_RETURN_IF(t.bPn.view(half - ignore_last, P.length * 2, Pn));
-    GPUPI_RETURN_IF(t.bR2.view(half, R.length + P.length, R2));
+    // product rows 16-byte aligned, except in the batched leaf forms, whose
+    // buffers keep the reference's preallocated sizes (and may be frozen for
+    // graph capture)
+    auto out_view = [&](Buf &b, uint32_t batch, uint32_t length, BatchMPArray &v) {
+        return compacting ? b.view_aligned(batch, length, v) : b.view(batch, length, v);
+    };
+    GPUPI_RETURN_IF(out_view(t.bPn, half - ignore_last, P.length * 2, Pn));
+    GPUPI_RETURN_IF(out_view(t.bR2, half, R.length + P.length, R2));
     if (Po.batch_size > 0) {
         GPUPI_RETURN_IF(p_mul_shared(ctx, "P,P*R", n1, Po, Ro, Pe, Pn, R2));
         if (compacting) GPUPI_RETURN_IF(compact(ctx, Pn, "P", n1));
@@ -385,8 +403,8 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
         GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, Pe, Ro, R2));
     }
     // Q = Q_e Q_o and R*Q = R_e Q_o share Q_o's transform
-    GPUPI_RETURN_IF(t.bQn.view(half, Q.length * 2, Qn));
-    GPUPI_RETURN_IF(t.bR1.view(half, R.length + Q.length, R1));
+    GPUPI_RETURN_IF(out_view(t.bQn, half, Q.length * 2, Qn));
+    GPUPI_RETURN_IF(out_view(t.bR1, half, R.length + Q.length, R1));
     GPUPI_RETURN_IF(p_mul_shared(ctx, "Q,R*Q", n1, Qe, Re, Qo, Qn, R1));
     if (compacting) GPUPI_RETURN_IF(compact(ctx, Qn, "Q", n1));
     // R(a,c) = R(a,b) Q(b,c) 2^(15 terms(b,c)) +- P(a,b) R(b,c)
@@ -936,7 +954,7 @@ hipError_t convert_decimal_equal_split(BatchMPContext *ctx, BatchMPArray input, 
             GPUPI_RETURN_IF(gpupi::launch_extract_low(cur, mant, cur.batch_size, mant_bits, ctx->stream));
             const BatchMPArray &pw = table->powers[levels - 1 - l];
             const BatchMPArray power = view_of(pw.data, pw.length, cur.batch_size, 0);
-            GPUPI_RETURN_IF(bprod.view(cur.batch_size, mant.length + power.length, prod));
+            GPUPI_RETURN_IF(bprod.view_aligned(cur.batch_size, mant.length + power.length, prod));
             GPUPI_RETURN_IF(batch_mp_mul(ctx, mant, power, prod));
             GPUPI_RETURN_IF(gpupi::launch_extract_shifted(prod, next, cur.batch_size, mant_bits - child_bits, 1, 2,
                                                           ctx->stream));

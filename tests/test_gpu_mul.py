@@ -73,6 +73,26 @@ def test_ntt(ctx, n, la, lb):
         g.check(ctx.set_ntt_threshold(384))
 
 
+ALIGNED = [(4, 3000, 2900), (3, 40000, 30000), (2, 300000, 200000), (1, 1 << 20, 1 << 20)]
+
+
+@pytest.mark.parametrize("n,la,lb", ALIGNED, ids=[f"N{n}_{a}x{b}" for n, a, b in ALIGNED])
+def test_ntt_aligned_rows(ctx, n, la, lb):
+    # output rows 16-byte aligned (stride a multiple of 4 limbs): the final
+    # carry pass runs four limbs per lane; all-ones operands make carry
+    # chains cross its 1024-limb chunks
+    g.check(ctx.set_ntt_threshold(2))
+    try:
+        rng = np.random.default_rng(la)
+        for mode in ("random", "ones", "top"):
+            A = rand_rows(rng, n, la, mode)
+            B = rand_rows(rng, n, lb, mode)
+            extra = (-(la + lb)) % 4 + 4
+            assert np.array_equal(device_mul(ctx, A, B, sc_extra=extra), oracle_mul(A, B)), mode
+    finally:
+        g.check(ctx.set_ntt_threshold(384))
+
+
 def test_boundary_shapes(ctx):
     # batch_arith_test.cu:99-103: 512+512 (naive there), 513+512, 2200+2200
     rng = np.random.default_rng(7)
