@@ -172,11 +172,21 @@ hipError_t p_mul(BatchMPContext *ctx, const char *label, uint32_t n1, const Batc
                  [&] { return batch_mp_mul(ctx, A, B, C); });
 }
 
-// C1 = A1 B and C2 = A2 B with B's transform shared (one stage entry).
-hipError_t p_mul_shared(BatchMPContext *ctx, const char *label, uint32_t n1, const BatchMPArray &A1,
-                        const BatchMPArray &A2, const BatchMPArray &B, const BatchMPArray &C1, const BatchMPArray &C2) {
-    return stage(ctx, "mul", label, n1, A2.batch_size, A2.length, B.length, mul_uses_ntt(ctx, A2, B),
-                 [&] { return batch_mp_mul_shared(ctx, A1, A2, B, C1, C2); });
+// C1 = A1 B and C2 = A2 B.  When both are NTT products of one transform
+// size, one shared launch sequence (B transformed once) and one stage entry
+// labelled "label1,label2"; otherwise the reference's two products, with B
+// as the left operand when b_left (the order of pi_bs_gpu.cu's calls).
+hipError_t p_mul_pair(BatchMPContext *ctx, const char *label, const char *label1, const char *label2, uint32_t n1,
+                      bool b_left, const BatchMPArray &A1, const BatchMPArray &A2, BatchMPArray B,
+                      const BatchMPArray &C1, const BatchMPArray &C2) {
+    if (mul_uses_ntt(ctx, A1, B) && mul_uses_ntt(ctx, A2, B) &&
+        gpupi::ntt_shared_supports(A1.length, A2.length, B.length))
+        return stage(ctx, "mul", label, n1, A2.batch_size, A2.length, B.length, true,
+                     [&] { return batch_mp_mul_shared(ctx, A1, A2, B, C1, C2); });
+    B.batch_size = A1.batch_size;
+    GPUPI_RETURN_IF(b_left ? p_mul(ctx, label1, n1, B, A1, C1) : p_mul(ctx, label1, n1, A1, B, C1));
+    B.batch_size = A2.batch_size;
+    return b_left ? p_mul(ctx, label2, n1, B, A2, C2) : p_mul(ctx, label2, n1, A2, B, C2);
 }
 
 hipError_t compact(BatchMPContext *ctx, BatchMPArray &a, const char *label = "", uint32_t n1 = 0) {
@@ -397,7 +407,7 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     GPUPI_RETURN_IF(out_view(t.bPn, half - ignore_last, P.length * 2, Pn));
     GPUPI_RETURN_IF(out_view(t.bR2, half, R.length + P.length, R2));
     if (Po.batch_size > 0) {
-        GPUPI_RETURN_IF(p_mul_shared(ctx, "P,P*R", n1, Po, Ro, Pe, Pn, R2));
+        GPUPI_RETURN_IF(p_mul_pair(ctx, "P,P*R", "P", "P*R", n1, true, Po, Ro, Pe, Pn, R2));
         if (compacting) GPUPI_RETURN_IF(compact(ctx, Pn, "P", n1));
     } else {
         GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, Pe, Ro, R2));
@@ -405,7 +415,7 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     // Q = Q_e Q_o and R*Q = R_e Q_o share Q_o's transform
     GPUPI_RETURN_IF(out_view(t.bQn, half, Q.length * 2, Qn));
     GPUPI_RETURN_IF(out_view(t.bR1, half, R.length + Q.length, R1));
-    GPUPI_RETURN_IF(p_mul_shared(ctx, "Q,R*Q", n1, Qe, Re, Qo, Qn, R1));
+    GPUPI_RETURN_IF(p_mul_pair(ctx, "Q,R*Q", "Q", "R*Q", n1, false, Qe, Re, Qo, Qn, R1));
     if (compacting) GPUPI_RETURN_IF(compact(ctx, Qn, "Q", n1));
     // R(a,c) = R(a,b) Q(b,c) 2^(15 terms(b,c)) +- P(a,b) R(b,c)
     const uint32_t terms_right = (n / n1) + ((n / n1) >> 4);
