@@ -130,6 +130,8 @@ __global__ void __launch_bounds__(256) length_wave_kernel(const uint32_t *A, uin
          row += waves) {
         const uint32_t *a = A + (size_t)row * stride_A;
         for (int64_t top = L; top > 0; top -= kWave) {
+            // the rest of the row lies below this wave's best length so far
+            if ((bits ? 32 * top : top) <= (int64_t)best) break;
             const int64_t j = top - kWave + (int64_t)lane;
             const uint32_t v = (j >= 0) ? a[j] : 0u;
             const uint64_t nz = ballot64(v != 0u);
@@ -144,10 +146,10 @@ __global__ void __launch_bounds__(256) length_wave_kernel(const uint32_t *A, uin
     block_max_atomic(best, result);
 }
 
-// Few long rows: blocks scan 4096-limb chunks counted from the top of each
-// row.  A block whose chunk lies wholly below the best length found so far
-// leaves at once, so the work is about one chunk per row plus the zero tail
-// actually present above the top limb.
+// Long rows: blocks scan 4096-limb chunks counted from the top of each row.
+// A block whose chunk lies wholly below the best length found so far leaves
+// at once, so the work is about one chunk per row plus the zero tail above
+// the top limb of the longest rows (rows of a batch differ in length).
 constexpr uint32_t kLenChunk = 4096;
 
 __global__ void __launch_bounds__(256) length_chunk_kernel(const uint32_t *A, uint32_t L, uint32_t stride_A,
@@ -215,7 +217,7 @@ hipError_t launch_exactdiv_small(uint32_t *A, uint32_t B, uint32_t N, uint32_t L
 hipError_t launch_length_max(const uint32_t *A, uint32_t N, uint32_t L, uint32_t stride_A, bool bits,
                              uint32_t *d_result, hipStream_t stream) {
     if (N == 0 || L == 0) return hipSuccess;
-    if (L > 4 * kLenChunk && N <= 64) {
+    if (L > 4 * kLenChunk) {
         const uint32_t cpr = ceil_div(L, kLenChunk);
         hipLaunchKernelGGL(length_chunk_kernel, dim3(N * cpr), dim3(256), 0, stream, A, L, stride_A, cpr,
                            bits ? 1 : 0, d_result);
