@@ -222,6 +222,24 @@ def test_mul_shared_broadcast_and_tree_shape(ctx):
     assert np.array_equal(C2, oracle_mul(A2, B))
 
 
+def test_shared_and_broadcast_across_chunks(ctx):
+    # more rows than one launch chunk holds (rows * K > 2^28): the shared
+    # operand's transform per chunk, and a broadcast operand transformed once
+    # in the first chunk and reused by the next
+    rng = np.random.default_rng(11)
+    n, la, lb = 4100, 40000, 25000  # K = 2^16: 4096 rows per chunk
+    A1 = rand_rows(rng, n, la)
+    A2 = rand_rows(rng, n - 3, la - 7000)
+    B = rand_rows(rng, n, lb)
+    C1, C2 = device_mul_shared(ctx, A1, A2, B)
+    assert np.array_equal(C1, oracle_mul(A1, B))
+    assert np.array_equal(C2, oracle_mul(A2, B[:n - 3]))
+    Bb = rand_rows(rng, 1, lb)
+    C1, C2 = device_mul_shared(ctx, A1, A2, Bb, b_broadcast=True)
+    assert np.array_equal(C1, oracle_mul(A1, np.repeat(Bb, n, axis=0)))
+    assert np.array_equal(C2, oracle_mul(A2, np.repeat(Bb, n - 3, axis=0)))
+
+
 def test_mul_shared_rejects_bad_shapes(ctx):
     rng = np.random.default_rng(3)
     A = rand_rows(rng, 4, 5000)
