@@ -54,9 +54,16 @@ __global__ void __launch_bounds__(256) mul_thread_kernel(const uint32_t *A, cons
 constexpr uint32_t kPad = 4;
 
 // Dynamic LDS: a[LA] | 4 zero words | b[LB] | 4 zero words (b indexed -4..LB+3).
+// One wave per product; lane l computes the CPL columns k0 = g0 + CPL l ..
+// (CPL = 2 for products of at most 128 limbs, so that every lane works; 4
+// otherwise), sliding a CPL-wide window of b down the a loop.  Column k =
+// x + 2^32 y + 2^64 z becomes limbs as x_k + y_(k-1) + z_(k-2), and the
+// remaining carries go through one ballot resolution per 64 CPL columns.
+template <int CPL>
 __global__ void __launch_bounds__(64) mul_wave_kernel(const uint32_t *A, const uint32_t *B, uint32_t *C,
                                                       uint32_t LA, uint32_t LB, uint32_t sA,
                                                       uint32_t sB, uint32_t sC) {
+    static_assert(CPL == 2 || CPL == 4, "two or four columns per lane");
     extern __shared__ uint32_t lds[];
     const uint32_t row = blockIdx.x;
     const uint32_t lane = threadIdx.x;
@@ -74,74 +81,77 @@ __global__ void __launch_bounds__(64) mul_wave_kernel(const uint32_t *A, const u
 
     const uint32_t Lc = LA + LB;
     uint32_t *c = C + (size_t)row * sC;
-    // state handed from group to group (values of column 256g - 1 / - 2)
-    uint32_t prev_y3 = 0, prev_z2 = 0, prev_z3 = 0, prev_c3 = 0, bin = 0;
-    for (uint32_t g0 = 0; g0 < Lc; g0 += 4 * kWave) {
-        const int64_t k0 = (int64_t)g0 + 4 * lane;
-        uint32_t x[4] = {0, 0, 0, 0};
-        uint64_t h[4] = {0, 0, 0, 0};
+    // state handed from group to group (lane 63's top columns)
+    uint32_t prev_y = 0, prev_z2 = 0, prev_z1 = 0, prev_c = 0, bin = 0;
+    for (uint32_t g0 = 0; g0 < Lc; g0 += CPL * kWave) {
+        const int64_t k0 = (int64_t)g0 + CPL * lane;
+        uint32_t x[CPL];
+        uint64_t h[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            x[j] = 0;
+            h[j] = 0;
+        }
         int64_t ilo = k0 - (int64_t)LB + 1;
         if (ilo < 0) ilo = 0;
-        int64_t ihi = k0 + 3;
+        int64_t ihi = k0 + CPL - 1;
         if (ihi > (int64_t)LA - 1) ihi = (int64_t)LA - 1;
         if (ilo <= ihi) {
             // window bw[j] = b[k0 + j - i]
-            uint32_t bw[4];
+            uint32_t bw[CPL];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bw[j] = b_s[k0 + j - ilo];
+            for (int j = 0; j < CPL; ++j) bw[j] = b_s[k0 + j - ilo];
             for (int64_t i = ilo; i <= ihi; ++i) {
                 const uint32_t ai = a_s[i];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < CPL; ++j) {
                     const uint64_t t = (uint64_t)ai * bw[j] + x[j];
                     x[j] = (uint32_t)t;
                     h[j] += t >> 32;
                 }
-                bw[3] = bw[2];
-                bw[2] = bw[1];
-                bw[1] = bw[0];
+#pragma unroll
+                for (int j = CPL - 1; j > 0; --j) bw[j] = bw[j - 1];
                 bw[0] = b_s[k0 - i - 1];
             }
         }
-        uint32_t y[4], z[4];
+        uint32_t y[CPL], z[CPL];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < CPL; ++j) {
             y[j] = (uint32_t)h[j];
             z[j] = (uint32_t)(h[j] >> 32);
         }
-        // neighbours' top columns (lane 0 takes the previous group's lane 63)
-        uint32_t ly3 = __shfl_up(y[3], 1), lz2 = __shfl_up(z[2], 1), lz3 = __shfl_up(z[3], 1);
+        // the previous lane's top columns (lane 0 takes the previous group's lane 63)
+        uint32_t ly = __shfl_up(y[CPL - 1], 1), lz2 = __shfl_up(z[CPL - 2], 1), lz1 = __shfl_up(z[CPL - 1], 1);
         if (lane == 0) {
-            ly3 = prev_y3;
+            ly = prev_y;
             lz2 = prev_z2;
-            lz3 = prev_z3;
+            lz1 = prev_z1;
         }
-        uint64_t s[4];
-        s[0] = (uint64_t)x[0] + ly3 + lz2;
-        s[1] = (uint64_t)x[1] + y[0] + lz3;
-        s[2] = (uint64_t)x[2] + y[1] + z[0];
-        s[3] = (uint64_t)x[3] + y[2] + z[1];
-        uint32_t d[4], cc[4];
+        uint64_t s[CPL];
+        s[0] = (uint64_t)x[0] + ly + lz2;
+        s[1] = (uint64_t)x[1] + y[0] + lz1;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 2; j < CPL; ++j) s[j] = (uint64_t)x[j] + y[j - 1] + z[j - 2];
+        uint32_t d[CPL], cc[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
             d[j] = (uint32_t)s[j];
             cc[j] = (uint32_t)(s[j] >> 32);
         }
-        uint32_t lc3 = __shfl_up(cc[3], 1);
-        if (lane == 0) lc3 = prev_c3;
-        uint32_t e[4], gen[4];
-        const uint32_t cprev[4] = {lc3, cc[0], cc[1], cc[2]};
+        uint32_t lc = __shfl_up(cc[CPL - 1], 1);
+        if (lane == 0) lc = prev_c;
+        uint32_t e[CPL], gen[CPL];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint64_t u = (uint64_t)d[j] + cprev[j];
+        for (int j = 0; j < CPL; ++j) {
+            const uint64_t u = (uint64_t)d[j] + (j == 0 ? lc : cc[j - 1]);
             e[j] = (uint32_t)u;
             gen[j] = (uint32_t)(u >> 32);
         }
-        // lane aggregate of its 4 limbs for carry-in 0
+        // lane aggregate of its CPL limbs for carry-in 0
         uint32_t lg = 0;
         bool lp = true;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < CPL; ++j) {
             const bool p = e[j] == 0xffffffffu;
             lg = gen[j] | (lg & (uint32_t)p);
             lp = lp && p;
@@ -151,17 +161,17 @@ __global__ void __launch_bounds__(64) mul_wave_kernel(const uint32_t *A, const u
         const uint64_t cm = resolve_carries(G, P, bin, &cout);
         uint32_t ci = (uint32_t)(cm >> lane) & 1u;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < CPL; ++j) {
             const uint32_t r = e[j] + ci;
             const uint32_t co = gen[j] | (ci & (uint32_t)(e[j] == 0xffffffffu));
             const int64_t k = k0 + j;
             if (k < (int64_t)Lc) c[k] = r;
             ci = co;
         }
-        prev_y3 = __shfl(y[3], 63);
-        prev_z2 = __shfl(z[2], 63);
-        prev_z3 = __shfl(z[3], 63);
-        prev_c3 = __shfl(cc[3], 63);
+        prev_y = __shfl(y[CPL - 1], 63);
+        prev_z2 = __shfl(z[CPL - 2], 63);
+        prev_z1 = __shfl(z[CPL - 1], 63);
+        prev_c = __shfl(cc[CPL - 1], 63);
         bin = cout;
     }
 }
@@ -230,11 +240,16 @@ hipError_t launch_mul_naive(const uint32_t *A, const uint32_t *B, uint32_t *C, u
     const size_t lds = (size_t)(L_a + kPad + L_b + kPad) * sizeof(uint32_t);
     static bool attr_set = false;
     if (lds > 65536 && !attr_set) {
-        GPUPI_RETURN_IF(hipFuncSetAttribute((const void *)mul_wave_kernel,
+        GPUPI_RETURN_IF(hipFuncSetAttribute((const void *)mul_wave_kernel<4>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
-    hipLaunchKernelGGL(mul_wave_kernel, dim3(N), dim3(kWave), lds, stream, A, B, C, L_a, L_b, stride_A,
+    if (total <= 2u * kWave) {
+        hipLaunchKernelGGL(mul_wave_kernel<2>, dim3(N), dim3(kWave), lds, stream, A, B, C, L_a, L_b, stride_A,
+                           stride_B, stride_C);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(mul_wave_kernel<4>, dim3(N), dim3(kWave), lds, stream, A, B, C, L_a, L_b, stride_A,
                        stride_B, stride_C);
     return hipGetLastError();
 }
