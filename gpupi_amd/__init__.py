@@ -173,6 +173,8 @@ def _declare(lib):
         "gpupi_ntt_set_split": (err, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "gpupi_ntt_get_split": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "gpupi_stage_profile_enable": (err, [c_ctx, ctypes.c_int]),
+        "gpupi_cache_query": (err, [c_ctx, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(u64)]),
+        "gpupi_cache_trim": (err, [c_ctx]),
         "gpupi_mul_ragged": (err, [c_ctx, ctypes.POINTER(GpupiMulItem), u32]),
         "gpupi_server_create": (ctypes.c_void_p, [ctypes.POINTER(GpupiServeConfig)]),
         "gpupi_server_submit_pi": (ctypes.c_int64, [ctypes.c_void_p, u32, ctypes.c_void_p]),
@@ -337,6 +339,15 @@ class Context:
             self._lib.gpupi_pi_free(ctypes.byref(res))
         return ret, dec, res
 
+    def cache_stats(self) -> tuple:
+        """(bytes of device blocks cached now, device allocations made so far)."""
+        b, n = ctypes.c_size_t(0), ctypes.c_uint64(0)
+        check(self._lib.gpupi_cache_query(self.ptr, ctypes.byref(b), ctypes.byref(n)), "gpupi_cache_query")
+        return b.value, n.value
+
+    def cache_trim(self) -> None:
+        """Free the device blocks the context keeps for the pipeline."""
+        check(self._lib.gpupi_cache_trim(self.ptr), "gpupi_cache_trim")
 
     def mul_ragged(self, items) -> int:
         """items: sequence of (A_ptr, B_ptr, C_ptr, L_a, L_b); returns the hipError_t."""

@@ -90,6 +90,75 @@ constexpr int kDefaultNttMinLimbs = 384;
 
 }  // namespace
 
+namespace gpupi {
+
+namespace {
+// Requests round up to a power of two below 1 MiB and to 2 MiB steps above,
+// so blocks of neighbouring requests are interchangeable.
+size_t round_block(size_t bytes) {
+    if (bytes <= (1u << 20)) {
+        size_t b = 256;
+        while (b < bytes) b <<= 1;
+        return b;
+    }
+    const size_t step = 2u << 20;
+    return (bytes + step - 1) / step * step;
+}
+}  // namespace
+
+hipError_t BlockCache::acquire(hipStream_t stream, size_t bytes, void **ptr, size_t *got) {
+    *ptr = nullptr;
+    *got = 0;
+    if (!bound_ || stream != stream_) {
+        // blocks last used on another stream: let the device drain first
+        if (!free_.empty()) GPUPI_RETURN_IF(hipDeviceSynchronize());
+        stream_ = stream;
+        bound_ = true;
+    }
+    const size_t want = round_block(bytes);
+    // best fit, wasting at most half of the block
+    auto it = free_.lower_bound(want);
+    if (it != free_.end() && it->first <= 2 * want) {
+        *ptr = it->second;
+        *got = it->first;
+        cached_ -= it->first;
+        free_.erase(it);
+        return hipSuccess;
+    }
+    hipError_t err = hipMalloc(ptr, want);
+    if (err == hipErrorOutOfMemory && !free_.empty()) {
+        (void)hipGetLastError();
+        GPUPI_RETURN_IF(trim());
+        err = hipMalloc(ptr, want);
+    }
+    if (err != hipSuccess) {
+        *ptr = nullptr;
+        return err;
+    }
+    ++allocations_;
+    *got = want;
+    return hipSuccess;
+}
+
+void BlockCache::give_back(void *ptr, size_t bytes) {
+    if (ptr == nullptr) return;
+    free_.emplace(bytes, ptr);
+    cached_ += bytes;
+}
+
+hipError_t BlockCache::trim() {
+    hipError_t first = hipSuccess;
+    for (auto &kv : free_) {
+        const hipError_t e = hipFree(kv.second);
+        if (first == hipSuccess) first = e;
+    }
+    free_.clear();
+    cached_ = 0;
+    return first;
+}
+
+}  // namespace gpupi
+
 extern "C" {
 
 BatchMPArray batch_mp_array_create(uint32_t batch_size, uint32_t length, uint32_t stride) {
@@ -118,7 +187,8 @@ BatchMPContext *batch_mp_init(void) {
     if (ctx == nullptr) return nullptr;
     ctx->stream = nullptr;
     ctx->ntt_min_limbs = kDefaultNttMinLimbs;
-    if (gpupi::ntt_tables_init(&ctx->ntt_tables) != hipSuccess ||
+    ctx->cache = new (std::nothrow) gpupi::BlockCache();
+    if (ctx->cache == nullptr || gpupi::ntt_tables_init(&ctx->ntt_tables) != hipSuccess ||
         hipMalloc(&ctx->d_scalar, sizeof(uint32_t)) != hipSuccess ||
         hipHostMalloc(&ctx->h_scalar, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         batch_mp_destroy(ctx);
@@ -138,6 +208,7 @@ void batch_mp_destroy(BatchMPContext *ctx) {
     if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
     gpupi::ntt_tables_release(&ctx->ntt_tables);
     stage_profiler_free(ctx->profiler);
+    delete ctx->cache;
     delete ctx;
 }
 

@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <map>
+
 #include "batch_arith.h"
 #include "batch_kernels.h"
 
@@ -119,6 +121,35 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStrea
 
 }  // namespace gpupi
 
+namespace gpupi {
+
+// Device blocks kept by a context for reuse by the pi pipeline's buffers.
+// Every block serves one stream at a time, so a block handed back can serve
+// the next request at once: stream order runs the new user's kernels after
+// the old user's.  A request from another stream first waits for the device.
+// Only trim() frees blocks (hipFree waits for the whole device, which would
+// stall the host between stages).
+class BlockCache {
+  public:
+    // A block of at least `bytes` for work on `stream`; *got is its size.
+    hipError_t acquire(hipStream_t stream, size_t bytes, void **ptr, size_t *got);
+    // Hands a block back; `bytes` may be less than its size, never more.
+    void give_back(void *ptr, size_t bytes);
+    hipError_t trim();
+    size_t cached_bytes() const { return cached_; }
+    uint64_t allocations() const { return allocations_; }
+    ~BlockCache() { (void)trim(); }
+
+  private:
+    std::multimap<size_t, void *> free_;  // size -> block
+    hipStream_t stream_ = nullptr;
+    bool bound_ = false;
+    size_t cached_ = 0;
+    uint64_t allocations_ = 0;
+};
+
+}  // namespace gpupi
+
 // Context internals shared between arith.hip and the pipeline sources.
 // Per-stage event timings of the pi pipeline (defined with the pipeline).
 struct StageProfiler;
@@ -133,4 +164,5 @@ struct BatchMPContext {
     hipStream_t stream;
     int ntt_min_limbs;        // schoolbook -> NTT crossover on L_a + L_b
     StageProfiler *profiler;  // null unless stage profiling is on
+    gpupi::BlockCache *cache; // blocks of the pipeline's buffers
 };

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -44,27 +45,46 @@ __global__ void __launch_bounds__(256) leaf17_kernel(int is_head, uint32_t *P, u
 
 // ------------------------------------------------------- buffer helpers
 
-// A grow-only device buffer holding one batch array.
+// A grow-only device buffer holding one batch array.  Attached to a context,
+// it takes its blocks from the context's block cache and hands them back
+// (work on the context stream only); otherwise it allocates and frees them.
 struct Buf {
     uint32_t *ptr = nullptr;
     size_t cap = 0;       // words
     bool frozen = false;  // set while a graph is being captured: no reallocation
+    BatchMPContext *ctx = nullptr;
     Buf() = default;
+    explicit Buf(BatchMPContext *c) : ctx(c) {}
     Buf(const Buf &) = delete;
     Buf &operator=(const Buf &) = delete;
-    ~Buf() {
-        if (ptr) (void)hipFree(ptr);
+    ~Buf() { drop(); }
+    void drop() {
+        if (ptr == nullptr) return;
+        if (ctx) ctx->cache->give_back(ptr, cap * sizeof(uint32_t));
+        else (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
     }
     hipError_t reserve(size_t words) {
         if (words <= cap) return hipSuccess;
         if (frozen) return hipErrorInvalidValue;
-        if (ptr) {
-            GPUPI_RETURN_IF(hipFree(ptr));
+        if (ctx == nullptr) {
+            if (ptr) GPUPI_RETURN_IF(hipFree(ptr));
             ptr = nullptr;
             cap = 0;
+            GPUPI_RETURN_IF(hipMalloc(&ptr, std::max<size_t>(words, 1) * sizeof(uint32_t)));
+            cap = words;
+            return hipSuccess;
         }
-        GPUPI_RETURN_IF(hipMalloc(&ptr, std::max<size_t>(words, 1) * sizeof(uint32_t)));
-        cap = words;
+        // a growing buffer takes an eighth more than asked for: the tree's
+        // rows grow a little at every level
+        const size_t want = cap > 0 ? words + words / 8 : words;
+        drop();
+        void *p = nullptr;
+        size_t got = 0;
+        GPUPI_RETURN_IF(ctx->cache->acquire(ctx->stream, std::max<size_t>(want, 1) * sizeof(uint32_t), &p, &got));
+        ptr = static_cast<uint32_t *>(p);
+        cap = got / sizeof(uint32_t);
         return hipSuccess;
     }
     // A contiguous array view (stride = length) of `batch` rows.
@@ -88,6 +108,7 @@ struct Buf {
         out.stride = stride;
         return hipSuccess;
     }
+    // Buffers of one context (or both unattached) exchange their blocks.
     void swap(Buf &o) {
         std::swap(ptr, o.ptr);
         std::swap(cap, o.cap);
@@ -375,6 +396,10 @@ namespace {
 // ------------------------------------------------- the binary-split tree
 struct TreeBufs {
     Buf bP0, bQ0, bR0, bP, bQ, bR, bPn, bQn, bRn, bR1, bR2;
+    TreeBufs() = default;
+    explicit TreeBufs(BatchMPContext *ctx) {
+        for (Buf *b : {&bP0, &bQ0, &bR0, &bP, &bQ, &bR, &bPn, &bQn, &bRn, &bR1, &bR2}) b->ctx = ctx;
+    }
     void freeze(bool f) {
         for (Buf *b : {&bP0, &bQ0, &bR0, &bP, &bQ, &bR, &bPn, &bQn, &bRn, &bR1, &bR2}) b->frozen = f;
     }
@@ -593,7 +618,7 @@ hipError_t gpupi_construct_leaf_17(BatchMPContext *ctx, int is_head, uint32_t *P
 // pi_bs_gpu.cu:503-792
 hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t N, BatchMPArray *Q_out, BatchMPArray *R_out) {
     if (ctx == nullptr || Q_out == nullptr || R_out == nullptr || N == 0 || (N & (N - 1)) != 0) return hipErrorInvalidValue;
-    TreeBufs t;
+    TreeBufs t(ctx);
     BatchMPArray P, Q, R;
     // the 17-term group stage with bound lengths (no synchronization inside),
     // then one compaction of each array
@@ -611,7 +636,7 @@ hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t N, BatchMPArray *Q_o
 // pi_bs_gpu.cu:1346-1522
 hipError_t gpupi_sqrt_10005(BatchMPContext *ctx, int target_digits, BatchMPArray *P_out, BatchMPArray *Q_out) {
     if (ctx == nullptr || P_out == nullptr || Q_out == nullptr) return hipErrorInvalidValue;
-    Buf bP, bQ, bPQ, bPP, bQQ, bQQ5, bPn, bQn;
+    Buf bP(ctx), bQ(ctx), bPQ(ctx), bPP(ctx), bQQ(ctx), bQQ5(ctx), bPn(ctx), bQn(ctx);
     BatchMPArray P, Q;
     GPUPI_RETURN_IF(bP.view(1, 1, P));
     GPUPI_RETURN_IF(bQ.view(1, 1, Q));
@@ -665,7 +690,7 @@ hipError_t gpupi_recursive_inverse(BatchMPContext *ctx, BatchMPArray x, uint64_t
     }
     const size_t L_res = (size_t)((L + 31) / 32 + 2), L_res2 = (size_t)((L + 31) / 32 + 1),
                  L_y2 = (size_t)x.length + 1;
-    Buf by, bws;
+    Buf by(ctx), bws(ctx);
     BatchMPArray y;
     GPUPI_RETURN_IF(by.view(1, x.length, y));
     GPUPI_RETURN_IF(gpupi::launch_fill_u32(y.data, x.length, 0u, ctx->stream));
@@ -750,7 +775,7 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
 
     BatchMPArray Q_bs{}, R_bs{};
     GPUPI_RETURN_IF(gpupi_binary_split(ctx, (uint32_t)N, &Q_bs, &R_bs));
-    Buf bQbs, bRbs;
+    Buf bQbs(ctx), bRbs(ctx);
     bQbs.ptr = Q_bs.data;
     bQbs.cap = Q_bs.stride;
     bRbs.ptr = R_bs.data;
@@ -760,7 +785,7 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
     if ((uint32_t)rs_limbs >= R_bs.length) return hipErrorInvalidValue;
     const BatchMPArray R_pre = view_of(R_bs.data + rs_limbs, R_bs.length - (uint32_t)rs_limbs, 1,
                                        R_bs.length - (uint32_t)rs_limbs);
-    Buf bRsh;
+    Buf bRsh(ctx);
     BatchMPArray R_sh;
     GPUPI_RETURN_IF(bRsh.view(1, R_pre.length + 1, R_sh));
     const uint32_t td = (uint32_t)target_digits;
@@ -773,12 +798,12 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
 
     BatchMPArray P15{}, Q15{};
     GPUPI_RETURN_IF(gpupi_sqrt_10005(ctx, target_digits, &P15, &Q15));
-    Buf bP15, bQ15;
+    Buf bP15(ctx), bQ15(ctx);
     bP15.ptr = P15.data;
     bP15.cap = P15.stride;
     bQ15.ptr = Q15.data;
     bQ15.cap = Q15.stride;
-    Buf bPf, bQf;
+    Buf bPf(ctx), bQf(ctx);
     BatchMPArray P_final, Q_final;
     GPUPI_RETURN_IF(bPf.view(1, P15.length + Q_tr.length, P_final));
     GPUPI_RETURN_IF(bQf.view(1, Q15.length + R_tr.length, Q_final));
@@ -801,16 +826,16 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
 
     BatchMPArray inv{};
     GPUPI_RETURN_IF(gpupi_recursive_inverse(ctx, Q_tf, inverse_bits, &inv));
-    Buf binv;
+    Buf binv(ctx);
     binv.ptr = inv.data;
     binv.cap = inv.stride;
-    Buf bprod;
+    Buf bprod(ctx);
     BatchMPArray prod;
     GPUPI_RETURN_IF(bprod.view(1, P_tf.length + inv.length, prod));
     GPUPI_RETURN_IF(p_mul(ctx, "P*invQ", td, P_tf, inv, prod));
     GPUPI_RETURN_IF(compact(ctx, prod, "P*invQ", td));
 
-    Buf bret;
+    Buf bret(ctx);
     BatchMPArray ret;
     GPUPI_RETURN_IF(bret.view(1, prod.length - (uint32_t)(inverse_bits / 32u) + (uint32_t)prec_limbs, ret));
     const uint64_t final_shift = inverse_bits - (uint64_t)prec_limbs * 32u;
@@ -868,22 +893,13 @@ uint32_t levels_for_shape(uint32_t total, uint32_t leaf) {  // convert_decimal.c
 
 extern "C" {
 
-// convert_decimal.cu:198-269
-hipError_t convert_decimal_precompute_powers(BatchMPContext *ctx, ConvertDecimalPowerTable *table,
-                                             uint32_t leaf_digits, uint32_t levels) {
-    if (ctx == nullptr || table == nullptr || leaf_digits == 0) return hipErrorInvalidValue;
-    table->powers = nullptr;
-    table->levels = 0;
-    table->leaf_digits = leaf_digits;
-    if (levels == 0) return hipSuccess;
-    BatchMPArray *powers = new (std::nothrow) BatchMPArray[levels];
-    if (powers == nullptr) return hipErrorOutOfMemory;
-    for (uint32_t i = 0; i < levels; ++i) powers[i] = BatchMPArray{};
-    auto fail = [&](hipError_t e) {
-        for (uint32_t i = 0; i < levels; ++i) batch_mp_array_release(powers[i]);
-        delete[] powers;
-        return e;
-    };
+}  // extern "C"
+
+namespace {
+
+// 5^leaf_digits and its repeated squares: powers[l] = 5^(leaf_digits 2^l)
+// for l < levels, row l in the block of bufs[l] (convert_decimal.cu:198-269).
+hipError_t fill_powers(BatchMPContext *ctx, uint32_t leaf_digits, uint32_t levels, BatchMPArray *powers, Buf *bufs) {
     // 5^leaf_digits on the host (at most ~1400 bits), then squarings.
     Limbs five(1, 1);
     for (uint32_t k = 0; k < leaf_digits; ++k) {
@@ -895,18 +911,62 @@ hipError_t convert_decimal_precompute_powers(BatchMPContext *ctx, ConvertDecimal
         }
         if (carry) five.push_back((uint32_t)carry);
     }
-    powers[0] = batch_mp_array_create(1, (uint32_t)five.size());
-    if (powers[0].data == nullptr) return fail(hipErrorOutOfMemory);
-    hipError_t err = hipMemcpyAsync(powers[0].data, five.data(), five.size() * 4, hipMemcpyHostToDevice, ctx->stream);
-    if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
-    if (err != hipSuccess) return fail(err);
+    GPUPI_RETURN_IF(bufs[0].view(1, (uint32_t)five.size(), powers[0]));
+    GPUPI_RETURN_IF(hipMemcpyAsync(powers[0].data, five.data(), five.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
     for (uint32_t l = 1; l < levels; ++l) {
-        powers[l] = batch_mp_array_create(1, powers[l - 1].length * 2);
-        if (powers[l].data == nullptr) return fail(hipErrorOutOfMemory);
-        err = batch_mp_mul(ctx, powers[l - 1], powers[l - 1], powers[l]);
-        if (err == hipSuccess) err = batch_mp_array_compact(ctx, &powers[l]);
-        if (err != hipSuccess) return fail(err);
+        GPUPI_RETURN_IF(bufs[l].view(1, powers[l - 1].length * 2, powers[l]));
+        GPUPI_RETURN_IF(batch_mp_mul(ctx, powers[l - 1], powers[l - 1], powers[l]));
+        GPUPI_RETURN_IF(batch_mp_array_compact(ctx, &powers[l]));
     }
+    return hipSuccess;
+}
+
+// The power table of one pi run, its blocks from the context's cache.
+struct CachedPowers {
+    std::unique_ptr<Buf[]> bufs;
+    std::vector<BatchMPArray> arrays;
+    ConvertDecimalPowerTable table{};
+    hipError_t fill(BatchMPContext *ctx, uint32_t leaf_digits, uint32_t levels) {
+        table.leaf_digits = leaf_digits;
+        if (levels == 0) return hipSuccess;
+        bufs.reset(new (std::nothrow) Buf[levels]);
+        if (!bufs) return hipErrorOutOfMemory;
+        for (uint32_t l = 0; l < levels; ++l) bufs[l].ctx = ctx;
+        arrays.assign(levels, BatchMPArray{});
+        GPUPI_RETURN_IF(fill_powers(ctx, leaf_digits, levels, arrays.data(), bufs.get()));
+        table.powers = arrays.data();
+        table.levels = levels;
+        return hipSuccess;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// convert_decimal.cu:198-269
+hipError_t convert_decimal_precompute_powers(BatchMPContext *ctx, ConvertDecimalPowerTable *table,
+                                             uint32_t leaf_digits, uint32_t levels) {
+    if (ctx == nullptr || table == nullptr || leaf_digits == 0) return hipErrorInvalidValue;
+    table->powers = nullptr;
+    table->levels = 0;
+    table->leaf_digits = leaf_digits;
+    if (levels == 0) return hipSuccess;
+    // the caller releases each row with batch_mp_array_release: plain blocks
+    std::unique_ptr<Buf[]> bufs(new (std::nothrow) Buf[levels]);
+    BatchMPArray *powers = new (std::nothrow) BatchMPArray[levels];
+    if (!bufs || powers == nullptr) {
+        delete[] powers;
+        return hipErrorOutOfMemory;
+    }
+    for (uint32_t i = 0; i < levels; ++i) powers[i] = BatchMPArray{};
+    const hipError_t err = fill_powers(ctx, leaf_digits, levels, powers, bufs.get());
+    if (err != hipSuccess) {
+        delete[] powers;
+        return err;
+    }
+    for (uint32_t i = 0; i < levels; ++i) powers[i].data = bufs[i].release();
     table->powers = powers;
     table->levels = levels;
     return hipSuccess;
@@ -941,7 +1001,7 @@ hipError_t convert_decimal_equal_split(BatchMPContext *ctx, BatchMPArray input, 
         bits[l] = bits_for_digits(dig[l]);
         limbs[l] = ceil_div(bits[l], 32);
     }
-    Buf bcur, bnext, bmant, bprod;
+    Buf bcur(ctx), bnext(ctx), bmant(ctx), bprod(ctx);
     BatchMPArray cur = input;
     if (levels == 0) {
         const uint32_t need = bits_for_digits(total_digits);
@@ -972,7 +1032,7 @@ hipError_t convert_decimal_equal_split(BatchMPContext *ctx, BatchMPArray input, 
             cur = next;
         }
     }
-    Buf baligned;
+    Buf baligned(ctx);
     BatchMPArray aligned;
     GPUPI_RETURN_IF(baligned.view(cur.batch_size, cur.length, aligned));
     const uint32_t leaf_bits = levels == 0 ? bits_for_digits(total_digits) : bits[levels];
@@ -1006,18 +1066,18 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
     uint32_t groups = 0;
     uint64_t inverse_bits = 0;
     GPUPI_RETURN_IF(gpupi_pi_fractional(ctx, (int)prec_digits, &ret, &groups, &inverse_bits));
-    ConvertDecimalPowerTable powers{};
+    CachedPowers powers;
     hipError_t err = stage(ctx, "decimal_powers", "precompute_5pow", convert_digits, levels, leaf, 0, false,
-                           [&] { return convert_decimal_precompute_powers(ctx, &powers, leaf, levels); });
+                           [&] { return powers.fill(ctx, leaf, levels); });
     char *digits = nullptr;
     if (err == hipSuccess) err = hipMalloc(&digits, std::max<uint32_t>(convert_digits, 1));
     if (err == hipSuccess && requested_digits > 0) {
         const BatchMPArray frac = make_subview(ret, 0, std::min(prec_limbs, ret.length));
         err = stage(ctx, "decimal_convert", "equal_split", convert_digits, 1, frac.length, 0, false, [&] {
-            return convert_decimal_equal_split(ctx, frac, prec_limbs * 32u, convert_digits, leaf, &powers, digits);
+            return convert_decimal_equal_split(ctx, frac, prec_limbs * 32u, convert_digits, leaf, &powers.table,
+                                               digits);
         });
     }
-    convert_decimal_release_powers(&powers);
     if (err == hipSuccess) err = hipEventRecord(stop, ctx->stream);
     if (err == hipSuccess) err = hipEventSynchronize(stop);
     float ms = 0.0f;
@@ -1126,6 +1186,18 @@ void gpupi_leaf_stage_destroy(GpupiLeafStage *s) {
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->ws) (void)hipFree(s->ws);
     delete s;
+}
+
+hipError_t gpupi_cache_query(const BatchMPContext *ctx, size_t *cached_bytes, uint64_t *allocations) {
+    if (ctx == nullptr) return hipErrorInvalidValue;
+    if (cached_bytes) *cached_bytes = ctx->cache->cached_bytes();
+    if (allocations) *allocations = ctx->cache->allocations();
+    return hipSuccess;
+}
+
+hipError_t gpupi_cache_trim(BatchMPContext *ctx) {
+    if (ctx == nullptr) return hipErrorInvalidValue;
+    return ctx->cache->trim();
 }
 
 void gpupi_pi_free(GpupiPiResult *result) {

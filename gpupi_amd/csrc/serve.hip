@@ -150,6 +150,9 @@ void GpupiServer::pi_worker() {
                 }
                 gpupi_pi_free(&r);
             }
+            // the job's memory goes back before its budget share does
+            const hipError_t trimmed = gpupi_cache_trim(ctx);
+            if (status == hipSuccess) status = trimmed;
         }
         finish(std::move(j), status);
     }

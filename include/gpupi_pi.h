@@ -135,6 +135,17 @@ typedef struct GpupiStageEntry {
 hipError_t gpupi_stage_profile_enable(BatchMPContext *ctx, int enable);
 int gpupi_stage_profile_query(const BatchMPContext *ctx, GpupiStageEntry *out, int max_entries);
 
+/* Device blocks the context keeps for the pipeline (no reference
+ * counterpart; the reference allocates and frees every buffer, and each
+ * free waits for the whole device).  The pipeline's buffers take blocks from
+ * the context's cache and hand them back, so a repeated run allocates
+ * nothing beyond its two result buffers.  query writes the bytes cached now
+ * and the number of device allocations the cache has made; trim frees the
+ * cached blocks (batch_mp_destroy does too).  Arrays returned by the stage
+ * entry points are ordinary device allocations: release them as before. */
+hipError_t gpupi_cache_query(const BatchMPContext *ctx, size_t *cached_bytes, uint64_t *allocations);
+hipError_t gpupi_cache_trim(BatchMPContext *ctx);
+
 /* Sizing helpers (pi_bs_gpu.cu:924-944) */
 uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits);
 uint32_t gpupi_prec_limbs_for_digits(uint32_t digits);

@@ -92,6 +92,27 @@ def test_pi_matches_oracle(ctx, digits):
     assert dec == odec
 
 
+def test_repeat_run_reuses_cached_blocks():
+    # the second run of one size takes its buffers from the context's block
+    # cache (RET's block leaves the cache with the result; best fit may pair
+    # requests and blocks differently from the first run)
+    with g.Context() as c:
+        ret1, dec1, _ = c.pi(200000)
+        cached1, allocs1 = c.cache_stats()
+        assert cached1 > 0
+        ret2, dec2, _ = c.pi(200000)
+        cached2, allocs2 = c.cache_stats()
+        print(f"allocations: first run {allocs1}, second run {allocs2 - allocs1}; cached {cached2} bytes")
+        assert allocs2 - allocs1 <= 4
+        assert np.array_equal(ret1, ret2) and dec1 == dec2
+        c.cache_trim()
+        assert c.cache_stats()[0] == 0
+        ret3, dec3, _ = c.pi(200000)
+        assert np.array_equal(ret1, ret3) and dec1 == dec3
+    oret, odec, _ = of.pi_digits(200000)
+    assert np.array_equal(ret1, oret) and dec1 == odec
+
+
 @pytest.mark.parametrize("digits", [1, 64, 763, 768, 20000])
 def test_pi_digits_match_golden_fixture(ctx, digits):
     # tests/golden/pi_digits.txt: pi from an independent implementation
