@@ -444,6 +444,59 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     }
 }
 
+// Largest last level the paired kernel takes over from the single-product
+// one.
+constexpr int kPairMaxM = 12;
+
+// Last level of two products sharing B (C_q = A_q B, q = 0, 1; product q
+// has rows0 / rows1 rows, B row r serves row r of both): per prime, B's tile
+// is loaded and transformed once and kept in registers; each product's A
+// tile is then transformed, multiplied, inverted and written back over A.
+// Block b covers row b / tiles, tile b % tiles of both products.
+template <int M>
+__global__ void __launch_bounds__(256) ntt_row_pair_kernel(PassArgs a, uint32_t rows0, uint32_t rows1) {
+    __shared__ double lds[kLdsSlots];
+    constexpr uint32_t kJ = 1u << (M - 4);
+    const uint32_t tiles = tiles_per_product(a.plan);
+    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t row = bid / tiles, t = bid % tiles;
+    const NttOp ob = a.ops[a.prods[0].b];
+    const uint64_t K = 1ull << a.plan.log2K;
+    const size_t off = (size_t)t * kTile + row_step0_base<M>();
+    const double *TB0 = a.T + ob.base + (size_t)(a.prods[0].b_bcast ? 0u : row) * K + off;
+#pragma clang loop unroll(disable)
+    for (int P = 0; P < 2; ++P) {
+        const Mod m = mod_of(P);
+        const double *tab = a.tables + P * kPrimeTable;
+        const double *TB = TB0 + P * ob.pstride;
+        double rb[16], v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) rb[j] = TB[j * kJ];
+        fwd_dft<M, false, kDifToPointB>(rb, lds, tab, P, m);
+#pragma clang loop unroll(disable)
+        for (int q = 0; q < 2; ++q) {
+            if (row >= (q ? rows1 : rows0)) continue;  // uniform over the block
+            const NttOp oa = a.ops[a.prods[q].a];
+            double *TA = a.T + oa.base + (size_t)row * K + off + P * oa.pstride;
+            // the table pointer is opaque per product, so that the twiddle
+            // loads stay inside the loop instead of occupying registers
+            // across it
+            const double *tabq = tab;
+            __asm__ volatile("" : "+s"(tabq));
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = TA[j * kJ];
+            fwd_dft<M, false, kDifToPointA>(v, lds, tabq, P, m);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = mulp(rb[j], v[j], m);
+            inv_dft<M, false>(v, lds, tabq, P, m);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) TA[j * kJ] = v[j];
+        }
+        __syncthreads();
+    }
+}
+
 // Inverse column level of prime P on the thread's elements v (as loaded):
 // inverse twiddle, inverse DFT; leaves the thread's elements tid + 256 u in
 // v.  The caller must make sure no thread still reads `lds`.
@@ -734,10 +787,11 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Optional per-launch timing of the NTT kernels (HIP events on the launch
 // stream) with each launch's algorithmic bytes, for live roofline numbers.
-enum KernelKind { kColFwdLimbs, kColFwd, kRowFused, kColInv, kColInvLimbs, kLocal, kKindCount };
+enum KernelKind { kColFwdLimbs, kColFwd, kRowFused, kRowPair, kColInv, kColInvLimbs, kLocal, kKindCount };
 const char *const kKindNames[kKindCount] = {"ntt_col_fwd_kernel<limbs>", "ntt_col_fwd_kernel",
-                                            "ntt_row_fused_kernel", "ntt_col_inv_kernel",
-                                            "ntt_col_inv_limbs_kernel", "ntt_local_kernel"};
+                                            "ntt_row_fused_kernel", "ntt_row_pair_kernel",
+                                            "ntt_col_inv_kernel", "ntt_col_inv_limbs_kernel",
+                                            "ntt_local_kernel"};
 struct StatRecord {
     int kind, log2K;
     hipEvent_t start, stop;
@@ -1109,12 +1163,25 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
             }
         }
         if (prod_blocks == 0) continue;
-        GPUPI_RETURN_IF(timed(kRowFused, k, 3 * prod_tb, stream, [&] {
-            dispatch_m<kRowLo, kRowHi>(p.log2M[p.levels - 1], [&](auto mc) {
-                hipLaunchKernelGGL((ntt_row_fused_kernel<decltype(mc)::value>), dim3((uint32_t)prod_blocks),
-                                   dim3(256), 0, stream, a);
-            });
-        }));
+        if (j.n_prods == 2 && j.prods[0].b == j.prods[1].b && p.log2M[p.levels - 1] <= kPairMaxM) {
+            // shared B: its last-level transform once per tile for both products
+            const uint32_t pair_rows = std::max(rows[0], rows[1]);
+            const double pair_tb = 16.0 * (double)pair_rows * (double)K + 2 * prod_tb;
+            GPUPI_RETURN_IF(timed(kRowPair, k, pair_tb, stream, [&] {
+                dispatch_m<kRowLo, kRowHi>(p.log2M[p.levels - 1], [&](auto mc) {
+                    hipLaunchKernelGGL((ntt_row_pair_kernel<decltype(mc)::value>),
+                                       dim3((uint32_t)((uint64_t)pair_rows * tiles)), dim3(256), 0, stream, a,
+                                       rows[0], rows[1]);
+                });
+            }));
+        } else {
+            GPUPI_RETURN_IF(timed(kRowFused, k, 3 * prod_tb, stream, [&] {
+                dispatch_m<kRowLo, kRowHi>(p.log2M[p.levels - 1], [&](auto mc) {
+                    hipLaunchKernelGGL((ntt_row_fused_kernel<decltype(mc)::value>), dim3((uint32_t)prod_blocks),
+                                       dim3(256), 0, stream, a);
+                });
+            }));
+        }
         if (p.levels == 3) {
             GPUPI_RETURN_IF(timed(kColInv, k, 2 * prod_tb, stream, [&] {
                 dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {

@@ -187,6 +187,9 @@ SHARED = [  # (N1, L1, N2, L2, rows of B, L_b)
     (3, 300000, 3, 290000, 4, 240000),    # two levels, one row of B unused
     (2, 2500000, 3, 2200000, 3, 2000000),  # three levels
     (1, 40000, 2, 30000, 2, 20000),
+    (2, 100000, 3, 90000, 3, 80000),      # K = 2^18: last level 2^10
+    (2, 200000, 2, 180000, 2, 150000),    # K = 2^19: last level 2^11
+    (1, 10000000, 1, 9000000, 1, 8000000),  # K = 2^25: three levels, last 2^11
     (5, 100, 5, 90, 5, 80),               # schoolbook: two plain products
     (4, 1000, 4, 5000, 4, 700),           # different transform sizes: two plain products
     (6, 600, 6, 700, 6, 900),             # one tile each: two plain products
