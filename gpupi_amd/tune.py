@@ -9,7 +9,7 @@ product, and keeps the fastest; it also times the schoolbook and NTT paths
 around the crossover.  The library's built-in plans are the result of this
 tuner on MI355X.
 
-    python -m gpupi_amd.tune --ks 13-28 [--apply] [--json table.json]
+    python -m gpupi_amd.tune --ks 13-28 [--shared] [--apply] [--json table.json]
 
 Results are exact for every split; only speed changes."""
 import argparse
@@ -34,9 +34,10 @@ def splits(k: int):
 
 
 class Operands:
-    """Random operands of 2^(total-1) limbs each and room for the products."""
+    """Random operands of 2^(total-1) limbs each and room for the products
+    (a second product buffer with shared=True)."""
 
-    def __init__(self, total_log2: int, seed: int = 1):
+    def __init__(self, total_log2: int, seed: int = 1, shared: bool = False):
         rng = np.random.default_rng(seed)
         half = 1 << (total_log2 - 1)
         a = rng.integers(0, 1 << 32, size=half, dtype=np.uint64).astype(np.uint32)
@@ -44,11 +45,18 @@ class Operands:
         self.ba, _ = g.upload_rows(a[None, :])
         self.bb, _ = g.upload_rows(a[None, ::-1].copy())
         self.bc = g.DeviceBuffer(4 * 2 * half)
+        self.bc2 = g.DeviceBuffer(4 * 2 * half) if shared else None
 
     def arrays(self, L: int):
         N = self.half // L
         return (g.BatchMPArray(self.ba.ptr, L, N, L), g.BatchMPArray(self.bb.ptr, L, N, L),
                 g.BatchMPArray(self.bc.ptr, 2 * L, N, 2 * L))
+
+    def shared_arrays(self, L: int):
+        """A1, A2, B, C1, C2 of the binary-splitting tree's pattern (C1 = A1 B,
+        C2 = A2 B): two products of N rows sharing B."""
+        A, B, C = self.arrays(L)
+        return A, B, A, C, g.BatchMPArray(self.bc2.ptr, 2 * L, A.batch_size, 2 * L)
 
 
 def time_mul(ctx, A, B, C, reps: int) -> float:
@@ -61,16 +69,29 @@ def time_mul(ctx, A, B, C, reps: int) -> float:
     return (time.perf_counter() - t0) * 1e3 / reps
 
 
-def tune_splits(ctx, ops: Operands, k: int, reps: int = 3, log=None):
+def time_mul_shared(ctx, arrays, reps: int) -> float:
+    g.check(ctx.mul_shared(*arrays), "mul_shared")
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.check(ctx.mul_shared(*arrays), "mul_shared")
+    g.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def tune_splits(ctx, ops: Operands, k: int, reps: int = 3, log=None, shared: bool = False):
     """[(levels, ms)] for every split of 2^k, fastest first; raises if two
-    splits disagree on the product."""
+    splits disagree on the product.  shared: time pairs of products sharing
+    an operand (the binary-splitting tree's launches) instead of single
+    products."""
     A, B, C = ops.arrays(1 << (k - 1))
+    sh = ops.shared_arrays(1 << (k - 1)) if shared else None
     ref = None
     res = []
     try:
         for m0, m1 in splits(k):
             g.ntt_set_split(k, m0, m1)
-            ms = time_mul(ctx, A, B, C, reps)
+            ms = time_mul_shared(ctx, sh, reps) if shared else time_mul(ctx, A, B, C, reps)
             head = ops.bc.download(np.uint32, count=min(1 << 16, 2 * A.length * A.batch_size))
             if ref is None:
                 ref = head
@@ -114,6 +135,8 @@ def main(argv=None) -> int:
     ap.add_argument("--ks", default="13-28", help="transform sizes, e.g. 13-28 or 20,24")
     ap.add_argument("--total-log2", type=int, default=28, help="coefficients per timed call (log2)")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--shared", action="store_true",
+                    help="time pairs of products sharing an operand (the tree's launches)")
     ap.add_argument("--crossover", action="store_true", help="also time the schoolbook/NTT crossover")
     ap.add_argument("--apply", action="store_true", help="keep the fastest splits for this process")
     ap.add_argument("--json", help="write the table here")
@@ -125,10 +148,11 @@ def main(argv=None) -> int:
         ks = [int(x) for x in args.ks.split(",")]
     ks = [k for k in ks if 13 <= k <= min(30, args.total_log2)]
     ctx = g.Context()
-    ops = Operands(args.total_log2)
+    ops = Operands(args.total_log2, shared=args.shared)
     table, report = {}, {}
     for k in ks:
-        res = tune_splits(ctx, ops, k, args.reps, log=lambda s: print(s, file=sys.stderr, flush=True))
+        res = tune_splits(ctx, ops, k, args.reps, log=lambda s: print(s, file=sys.stderr, flush=True),
+                          shared=args.shared)
         table[k] = list(res[0][0])
         report[k] = {"best": list(res[0][0]), "best_ms": round(res[0][1], 3),
                      "builtin": list(g.ntt_get_split(k)),

@@ -369,3 +369,9 @@ def test_tuner_times_every_split_and_restores_the_plan(ctx):
     assert sorted(lv for lv, _ in res) == sorted((m0, 15 - m0) for m0, _ in tune.splits(15))
     assert all(ms > 0 for _, ms in res) and res[0][1] <= res[-1][1]
     assert g.ntt_get_split(15) == before
+    # the tree's pattern: two products sharing an operand, every split exact
+    ops = tune.Operands(20, seed=6, shared=True)
+    before = g.ntt_get_split(16)
+    res = tune.tune_splits(ctx, ops, 16, reps=1, shared=True)
+    assert sorted(lv for lv, _ in res) == sorted((m0, 16 - m0) for m0, _ in tune.splits(16))
+    assert g.ntt_get_split(16) == before
