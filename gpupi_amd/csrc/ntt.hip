@@ -409,7 +409,8 @@ __device__ __forceinline__ uint32_t row_step0_base() {
 }
 
 // Last level: per prime, forward on A's tile (kept in registers), forward on
-// B's tile, pointwise product, inverse; written back over A's plane.
+// B's tile, pointwise product, inverse; written back over A's plane.  A
+// square (B is A) multiplies A's transformed tile by itself.
 template <int M>
 __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     __shared__ double lds[kLdsSlots];
@@ -432,11 +433,20 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
         for (int j = 0; j < 16; ++j) ra[j] = TA[j * kJ];
         fwd_dft<M, false, kDifToPointA>(ra, lds, tab, P, m);
         __syncthreads();
+        if (pr.a == pr.b) {  // uniform over the launch
 #pragma unroll
-        for (int j = 0; j < 16; ++j) v[j] = TB[j * kJ];
-        fwd_dft<M, false, kDifToPointB>(v, lds, tab, P, m);
+            for (int j = 0; j < 16; ++j) v[j] = mulp(ra[j], ra[j], m);
+        } else {
+            // an opaque copy of the table pointer keeps B's twiddle loads
+            // after A's transform (fewer registers live)
+            const double *tabb = tab;
+            __asm__ volatile("" : "+s"(tabb));
 #pragma unroll
-        for (int j = 0; j < 16; ++j) v[j] = mulp(v[j], ra[j], m);
+            for (int j = 0; j < 16; ++j) v[j] = TB[j * kJ];
+            fwd_dft<M, false, kDifToPointB>(v, lds, tabb, P, m);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = mulp(v[j], ra[j], m);
+        }
         inv_dft<M, false>(v, lds, tab, P, m);
 #pragma unroll
         for (int j = 0; j < 16; ++j) TA[j * kJ] = v[j];
@@ -585,6 +595,11 @@ __global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
     col_inv_prime<M, 0>(v0, lds, ct, a.tables, tw0);
     __syncthreads();
     col_inv_prime<M, 1>(v1, lds, ct, a.tables, tw1);
+    // prime 1's elements wait in the thread's own LDS slots while the limbs
+    // are formed (registers for the carry resolution)
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lds[ct.lds0 + 272u * u] = v1[u];
 
     const double kinv0 = c_ntt.kinv[0][log2K], kinv1 = c_ntt.kinv[1][log2K];
     const uint32_t lane = threadIdx.x & (kWave - 1u);
@@ -599,7 +614,7 @@ __global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         const unsigned __int128 c =
-            recombine(mulp(v0[u], kinv0, mod_of(0)), mulp(v1[u], kinv1, mod_of(1)));
+            recombine(mulp(v0[u], kinv0, mod_of(0)), mulp(lds[ct.lds0 + 272u * u], kinv1, mod_of(1)));
         const uint32_t w0 = (uint32_t)c, w1 = (uint32_t)(c >> 32), w2 = (uint32_t)(c >> 64);
         const uint32_t w1a = __shfl_up(w1, 1), w2a = __shfl_up(w2, 1), w2b = __shfl_up(w2, 2);
         const uint64_t s = (uint64_t)w0 + (jr >= 1 ? w1a : 0u) + (jr >= 2 ? w2b : 0u);
@@ -985,11 +1000,12 @@ JobShape shape_of(const NttJob &j) {
 
 // Whether the operands and products describe a job run_ntt can do.
 bool job_ok(const NttJob &j) {
-    if (j.n_ops < 2 || j.n_ops > 3 || j.n_prods < 1 || j.n_prods > 2) return false;
+    if (j.n_ops < 1 || j.n_ops > 3 || j.n_prods < 1 || j.n_prods > 2) return false;
     if (j.ragged && (j.items == nullptr || j.host_items == nullptr || j.n_ops != 2 || j.n_prods != 1)) return false;
     for (uint32_t q = 0; q < j.n_prods; ++q) {
         const NttJob::Prod &pr = j.prods[q];
-        if (pr.a >= j.n_ops || pr.b >= j.n_ops || pr.a == pr.b) return false;
+        if (pr.a >= j.n_ops || pr.b >= j.n_ops) return false;
+        if (pr.a == pr.b && (j.ragged || j.n_prods != 1)) return false;  // squares: one product
         for (uint32_t r = 0; r < j.n_prods; ++r)
             if (r != q && (j.prods[r].a == pr.a || j.prods[r].b == pr.a)) return false;
         if (!j.ragged && transform_log2(j.ops[pr.a].L, j.ops[pr.b].L) != j.k) return false;
@@ -1214,10 +1230,23 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
 }
 
 // A uniform product C = A B of N rows; a broadcast A (stride 0) is swapped
-// into the B position so that its transform is computed once.
+// into the B position so that its transform is computed once, and a square
+// (B is A: same rows) transforms its one operand once.
 NttJob uniform_job(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t N, uint32_t L_a, uint32_t L_b,
                    uint32_t stride_A, uint32_t stride_B, uint32_t stride_C) {
     NttJob j;
+    if (A == B && A != nullptr && L_a == L_b && stride_A == stride_B) {
+        j.n_ops = 1;
+        j.n_prods = 1;
+        j.ops[0] = NttJob::Op{A, L_a, stride_A};
+        j.prods[0].a = 0;
+        j.prods[0].b = 0;
+        j.prods[0].C = C;
+        j.prods[0].sC = stride_C;
+        j.prods[0].N = N;
+        j.k = transform_log2(L_a, L_b);
+        return j;
+    }
     if (stride_A == 0 && stride_B != 0) {
         std::swap(A, B);
         std::swap(L_a, L_b);

@@ -73,6 +73,34 @@ def test_ntt(ctx, n, la, lb):
         g.check(ctx.set_ntt_threshold(384))
 
 
+SQUARES = [(5, 900, None), (3, 20000, None), (2, 20000, 20003), (1, 3000000, None), (4, 3000, 0)]
+
+
+@pytest.mark.parametrize("n,la,stride", SQUARES, ids=[f"N{n}_{a}_s{s}" for n, a, s in SQUARES])
+def test_ntt_square(ctx, n, la, stride):
+    # A * A through one operand (the library transforms it once): one tile,
+    # two and three levels, rows at a stride, and one row at stride 0
+    g.check(ctx.set_ntt_threshold(2))
+    try:
+        rng = np.random.default_rng(la + n)
+        for mode in ("random", "ones"):
+            A = rand_rows(rng, 1 if stride == 0 else n, la, mode)
+            buf, Aa = g.upload_rows(A, None if stride in (None, 0) else stride)
+            if stride == 0:
+                Aa = Aa.view(batch_size=n, stride=0)
+                A = np.repeat(A, n, axis=0)
+            lc = 2 * la
+            bc, Ca = g.upload_rows(np.full((n, lc + 1), PAD, dtype=np.uint32), lc + 1)
+            Ca.length = lc
+            g.check(ctx.mul(Aa, Aa, Ca), "mul")
+            g.synchronize()
+            full = bc.download(np.uint32, count=n * (lc + 1)).reshape(n, lc + 1)
+            assert np.all(full[:, lc:] == PAD), "write past the product"
+            assert np.array_equal(full[:, :lc], oracle_mul(A, A)), mode
+    finally:
+        g.check(ctx.set_ntt_threshold(384))
+
+
 ALIGNED = [(4, 3000, 2900), (3, 40000, 30000), (2, 300000, 200000), (1, 1 << 20, 1 << 20)]
 
 
