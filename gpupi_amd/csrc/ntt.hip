@@ -1191,7 +1191,9 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
                 });
             }));
         } else {
-            GPUPI_RETURN_IF(timed(kRowFused, k, 3 * prod_tb, stream, [&] {
+            // a square reads its one operand's planes once
+            const double row_tb = (j.prods[0].a == j.prods[0].b ? 2 : 3) * prod_tb;
+            GPUPI_RETURN_IF(timed(kRowFused, k, row_tb, stream, [&] {
                 dispatch_m<kRowLo, kRowHi>(p.log2M[p.levels - 1], [&](auto mc) {
                     hipLaunchKernelGGL((ntt_row_fused_kernel<decltype(mc)::value>), dim3((uint32_t)prod_blocks),
                                        dim3(256), 0, stream, a);
