@@ -136,14 +136,27 @@ __device__ __forceinline__ void group_level(uint32_t lane, const Nodes &ev, cons
 
 // Exact division of a[0..n) by the block divisor, in place (Hensel, from the
 // low limb up: q_i = (a_i - c) d^-1 mod 2^32, c' = (q_i d + c - a_i) / 2^32).
+// Lanes divide different rows in one loop of NMAX >= n steps (a lane idles
+// past its n), and the next limb is read from LDS while the current one is
+// divided.
+// With d = d_lo + 2^32 d_hi and c = c_lo + 2^32 c_hi: q d_lo + c_lo - a_i is
+// 2^32 [a_i < c_lo] (it is 0 mod 2^32 and above -2^32), so
+// c' = q d_hi + mulhi(q, d_lo) + c_hi + [a_i < c_lo] (< d).
+template <int NMAX>
 __device__ __forceinline__ void exact_div_block(uint32_t *a, int n) {
+    constexpr uint32_t kDlo = (uint32_t)kBlockDivisor, kDhi = (uint32_t)(kBlockDivisor >> 32);
     uint64_t c = 0;
-    for (int i = 0; i < n; ++i) {
-        const uint32_t ai = a[i];
-        const uint32_t q = (ai - (uint32_t)c) * kBlockDivisorInv;
-        const unsigned __int128 t = (unsigned __int128)q * kBlockDivisor + c - ai;
-        c = (uint64_t)(t >> 32);
-        a[i] = q;
+    uint32_t next = a[0];
+    for (int i = 0; i < NMAX; ++i) {
+        const uint32_t ai = next;
+        if (i + 1 < n) next = a[i + 1];
+        const uint32_t clo = (uint32_t)c, chi = (uint32_t)(c >> 32);
+        const uint32_t q = (ai - clo) * kBlockDivisorInv;
+        const uint64_t s = (uint64_t)__umulhi(q, kDlo) + chi + (ai < clo ? 1u : 0u);
+        if (i < n) {
+            c = (uint64_t)q * kDhi + s;
+            a[i] = q;
+        }
     }
 }
 
@@ -177,10 +190,12 @@ __global__ void __launch_bounds__(256) group_stage_kernel(uint32_t N, uint32_t *
     const Nodes n4{sm + kT1, sm + kT1 + 48, sm + kR1, 0, 0, 0};
     // head merge: P P0 | Q Q0 | R Q0 | P R0, R = (R Q0 << 15) + P R0
     group_level<1, 48, 64, 77, 3, 4, 6, 15, false, 83, false>(lane, n4, head, sm + kT0, sm + kR0);
-    // exact division of P (51), Q (68), R (83)
-    if (lane == 0) exact_div_block(sm + kT0, 51);
-    if (lane == 1) exact_div_block(sm + kT0 + 51, 68);
-    if (lane == 2) exact_div_block(sm + kR0, 83);
+    // exact division of P (51), Q (68), R (83): one lane each, side by side
+    if (lane < 3) {
+        const int off = lane == 0 ? kT0 : lane == 1 ? kT0 + 51 : kR0;
+        const int n = lane == 0 ? 51 : lane == 1 ? 68 : 83;
+        exact_div_block<83>(sm + off, n);
+    }
     __syncthreads();
     if (live) {
         uint32_t *po = P + (size_t)g * 51, *qo = Q + (size_t)g * 68, *ro = R + (size_t)g * 83;
