@@ -146,33 +146,44 @@ __global__ void __launch_bounds__(256) length_wave_kernel(const uint32_t *A, uin
     block_max_atomic(best, result);
 }
 
-// Long rows: blocks scan 4096-limb chunks counted from the top of each row.
-// A block whose chunk lies wholly below the best length found so far leaves
-// at once, so the work is about one chunk per row plus the zero tail above
-// the top limb of the longest rows (rows of a batch differ in length).
+// Long rows: 4096-limb chunks counted from the top of each row.  Block g of
+// a row scans chunks g, g + G, g + 2G, ... (G blocks per row) and stops at
+// the first that lies wholly below the best length found so far, so the work
+// is about one chunk per row plus the zero tail above the top limb of the
+// longest rows.  Rows are taken last first: in the binary-splitting tree the
+// last rows (the largest terms) are the longest, so the best length is known
+// before the shorter rows' zero tails are reached.  G grows for few rows, so
+// that a lone row's zero tail (a Newton residual) is scanned by many blocks.
 constexpr uint32_t kLenChunk = 4096;
+constexpr uint32_t kLenMinBlocksPerRow = 1;
+constexpr uint32_t kLenBlocks = 4096;  // blocks per launch when rows are few
 
-__global__ void __launch_bounds__(256) length_chunk_kernel(const uint32_t *A, uint32_t L, uint32_t stride_A,
-                                                           uint32_t chunks_per_row, int bits, uint32_t *result) {
+__global__ void __launch_bounds__(256) length_chunk_kernel(const uint32_t *A, uint32_t N, uint32_t L,
+                                                           uint32_t stride_A, uint32_t chunks_per_row,
+                                                           uint32_t blocks_per_row, int bits, uint32_t *result) {
     __shared__ uint32_t skip;
-    const uint32_t row = blockIdx.x / chunks_per_row;
-    const uint32_t c = blockIdx.x - row * chunks_per_row;
-    const int64_t hi = (int64_t)L - (int64_t)c * kLenChunk;  // chunk covers [hi - 4096, hi)
-    const int64_t lo = hi - (int64_t)kLenChunk > 0 ? hi - (int64_t)kLenChunk : 0;
-    if (threadIdx.x == 0) {
-        const uint32_t cur = atomicAdd(result, 0u);
-        const uint64_t best_possible = bits ? 32ull * (uint64_t)hi : (uint64_t)hi;
-        skip = (uint64_t)cur >= best_possible ? 1u : 0u;
-    }
-    __syncthreads();
-    if (skip) return;
+    const uint32_t row = N - 1u - blockIdx.x / blocks_per_row;
+    const uint32_t g = blockIdx.x % blocks_per_row;
     const uint32_t *a = A + (size_t)row * stride_A;
-    uint32_t best = 0;
-    for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
-        const uint32_t v = a[j];
-        if (v != 0u) best = max(best, length_of((uint32_t)j + 1u, v, bits != 0));
+    for (uint32_t c = g; c < chunks_per_row; c += blocks_per_row) {
+        const int64_t hi = (int64_t)L - (int64_t)c * kLenChunk;  // chunk covers [hi - 4096, hi)
+        const int64_t lo = hi - (int64_t)kLenChunk > 0 ? hi - (int64_t)kLenChunk : 0;
+        if (threadIdx.x == 0) {
+            // a relaxed load: a stale value only skips less
+            const uint32_t cur = __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t best_possible = bits ? 32ull * (uint64_t)hi : (uint64_t)hi;
+            skip = (uint64_t)cur >= best_possible ? 1u : 0u;
+        }
+        __syncthreads();
+        if (skip) return;  // lower chunks lie lower still
+        uint32_t best = 0;
+        for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+            const uint32_t v = a[j];
+            if (v != 0u) best = max(best, length_of((uint32_t)j + 1u, v, bits != 0));
+        }
+        // its barrier also orders this chunk's reads of `skip` before the next write
+        block_max_atomic(best, result);
     }
-    block_max_atomic(best, result);
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, uint64_t count, uint32_t value) {
@@ -219,7 +230,9 @@ hipError_t launch_length_max(const uint32_t *A, uint32_t N, uint32_t L, uint32_t
     if (N == 0 || L == 0) return hipSuccess;
     if (L > 4 * kLenChunk) {
         const uint32_t cpr = ceil_div(L, kLenChunk);
-        hipLaunchKernelGGL(length_chunk_kernel, dim3(N * cpr), dim3(256), 0, stream, A, L, stride_A, cpr,
+        const uint32_t bpr = std::min(cpr, std::max(kLenMinBlocksPerRow, ceil_div(kLenBlocks, N)));
+        if ((uint64_t)N * bpr > 0x7fffffffull) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(length_chunk_kernel, dim3(N * bpr), dim3(256), 0, stream, A, N, L, stride_A, cpr, bpr,
                            bits ? 1 : 0, d_result);
     } else if (L <= 64) {
         hipLaunchKernelGGL(length_thread_kernel, dim3(std::min<uint32_t>(ceil_div(N, 256), 2048)), dim3(256), 0,

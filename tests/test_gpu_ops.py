@@ -168,6 +168,25 @@ def test_length_max(ctx, n, length):
         assert r.value == of.lib().o_bitlength_max(h.oarray())
 
 
+def test_length_max_long_rows_of_different_lengths(ctx):
+    # the chunked scan (rows over 16384 limbs): rows end at different
+    # heights, so blocks must walk down past several all-zero chunks
+    import ctypes
+    rng = np.random.default_rng(22)
+    n, length = 6, 200000
+    for longest in (5, 2, 0):
+        A = rand_rows(rng, n, length)
+        for i in range(n):
+            top = length - 30000 * (i if i != longest else 0) - 7 * i - (0 if i == longest else 50000)
+            A[i, max(top, 0):] = 0
+        b, X = g.upload_rows(A, length + 3)
+        h = of.HostArray(A, length + 3)
+        assert ctx.limblength_max_value(X) == of.lib().o_limblength_max(h.oarray())
+        r = ctypes.c_uint32(0)
+        g.check(ctx.bitlength_max(X, ctypes.byref(r)))
+        assert r.value == of.lib().o_bitlength_max(h.oarray())
+
+
 @pytest.mark.parametrize("L_limb,L_dec,n", [(1, 9, 3), (2, 18, 5), (51, 477, 300), (52, 489, 70),
                                              (61, 576, 10), (65, 600, 4), (2047, 19000, 2), (300, 20000, 3)])
 def test_decimal_naive(ctx, L_limb, L_dec, n):
