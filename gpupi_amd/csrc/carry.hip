@@ -265,6 +265,111 @@ __global__ void __launch_bounds__(256) spill_add_vec_kernel(uint32_t *C, uint32_
     if (flags != nullptr && lane == 0) flags[wave] = (uint8_t)(cin | (all_prop ? 2u : 0u));
 }
 
+// Row-aligned 16-byte access: the base is 16-byte aligned and, with more
+// than one row, so is every row.
+bool rows_aligned(const void *p, uint32_t N, uint32_t stride) {
+    return p != nullptr && (((uintptr_t)p) & 15u) == 0 && (N <= 1 || (stride & 3u) == 0);
+}
+
+// Four limbs x[0..4) of row `row` from limb j0 on (zero outside [0, L)).
+__device__ __forceinline__ void load4(const uint32_t *p, uint32_t row, uint32_t stride, int64_t j0, uint32_t L,
+                                      bool vec, uint32_t *x) {
+    const uint32_t *r = p + (size_t)row * stride;
+    if (vec && j0 >= 0 && j0 + 4 <= (int64_t)L) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(r + j0);
+        x[0] = v.x;
+        x[1] = v.y;
+        x[2] = v.z;
+        x[3] = v.w;
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) x[t] = (j0 + t >= 0 && j0 + t < (int64_t)L) ? r[j0 + t] : 0u;
+    }
+}
+
+// Add, Sub, ShiftAdd and ShiftSub with four limbs per lane: one wave per
+// chunk of 1024 limbs of one row, the chunk and flag layout of
+// addsub_chunk_kernel<K, 16> (so carry_fixup_kernel completes long rows).
+// C's rows (and B's and, for Add/Sub, A's when a_vec/b_vec) start 16-byte
+// aligned; the shifted A of ShiftAdd/ShiftSub is read a limb at a time.
+template <AddSubKind K>
+__global__ void __launch_bounds__(256) addsub_vec_kernel(AddSubArgs a, bool a_vec, bool b_vec, uint32_t chunks_per_row,
+                                                         uint64_t total_chunks, uint8_t *flags) {
+    constexpr bool kSub = OpTraits<K>::kSub;
+    constexpr bool kShift = K == AddSubKind::ShiftAdd || K == AddSubKind::ShiftSub;
+    constexpr int kSeg = 4;  // 4 x 256 limbs per wave
+    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    if (wave >= total_chunks) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t row = (uint32_t)(wave / chunks_per_row);
+    const uint32_t chunk = (uint32_t)(wave - (uint64_t)row * chunks_per_row);
+    uint32_t *out = a.C + (size_t)row * a.stride_C;
+    const int64_t ws = kShift ? (int64_t)(a.shift >> 5) : 0;
+    const uint32_t bs = kShift ? (a.shift & 31u) : 0u;
+    uint32_t d[kSeg][4], g[kSeg][4];
+#pragma unroll
+    for (int s = 0; s < kSeg; ++s) {
+        const int64_t j0 = (int64_t)chunk * (kSeg * 4 * kWave) + s * (4 * kWave) + 4 * (int64_t)lane;
+        uint32_t x[4], y[4];
+        if constexpr (kShift) {
+            // x_t = (A << shift) at limb j0 + t: A[i] << bs | A[i - 1] >> (32 - bs), i = j0 + t - ws
+            uint32_t w[5];
+            const uint32_t *ra = a.A + (size_t)row * a.stride_A;
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                const int64_t i = j0 - ws - 1 + t;
+                w[t] = (i >= 0 && i < (int64_t)a.L_a) ? ra[i] : 0u;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x[t] = bs ? (w[t + 1] << bs) | (w[t] >> (32u - bs)) : w[t + 1];
+        } else {
+            load4(a.A, row, a.stride_A, j0, a.L_a, a_vec, x);
+        }
+        load4(a.B, row, a.stride_B, j0, a.L_b, b_vec, y);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            d[s][t] = kSub ? x[t] - y[t] : x[t] + y[t];
+            g[s][t] = kSub ? (x[t] < y[t] ? 1u : 0u) : (d[s][t] < x[t] ? 1u : 0u);
+        }
+    }
+    uint32_t cin = 0;
+    bool all_prop = true;
+#pragma unroll
+    for (int s = 0; s < kSeg; ++s) {
+        const int64_t j0 = (int64_t)chunk * (kSeg * 4 * kWave) + s * (4 * kWave) + 4 * (int64_t)lane;
+        // the lane's four limbs for a carry-in of zero
+        uint32_t r[4], c = 0;
+        bool prop = true;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            r[t] = kSub ? d[s][t] - c : d[s][t] + c;
+            c = g[s][t] | (c & (kSub ? (d[s][t] == 0u ? 1u : 0u) : (d[s][t] == 0xffffffffu ? 1u : 0u)));
+            prop = prop && (kSub ? r[t] == 0u : r[t] == 0xffffffffu) && j0 + t < (int64_t)a.L_c;
+        }
+        const bool valid = j0 < (int64_t)a.L_c;
+        const uint64_t G = ballot64(valid && c != 0), P = ballot64(valid && prop);
+        uint32_t cout;
+        const uint64_t cm = resolve_carries(G, P, cin, &cout);
+        uint32_t ci = (uint32_t)(cm >> lane) & 1u;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t v = kSub ? r[t] - ci : r[t] + ci;
+            ci = ci & (kSub ? (r[t] == 0u ? 1u : 0u) : (r[t] == 0xffffffffu ? 1u : 0u));
+            r[t] = v;
+        }
+        if (j0 + 4 <= (int64_t)a.L_c) {
+            *reinterpret_cast<uint4 *>(out + j0) = make_uint4(r[0], r[1], r[2], r[3]);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (j0 + t < (int64_t)a.L_c) out[j0 + t] = r[t];
+        }
+        all_prop = all_prop && (P == ~0ull);
+        cin = cout;
+    }
+    if (flags != nullptr && lane == 0) flags[wave] = (uint8_t)(cin | (all_prop ? 2u : 0u));
+}
+
 struct ChunkPlan {
     bool thread_rows;
     int seg;
@@ -305,12 +410,23 @@ hipError_t run_addsub(const AddSubArgs &a, const SpillArgs &sp, uint8_t *flags, 
     const uint64_t blocks = ceil_div64(total, 4);
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
     uint8_t *fl = need_fix ? flags : nullptr;
+    constexpr bool kVecKind = K == AddSubKind::Add || K == AddSubKind::Sub || K == AddSubKind::ShiftAdd ||
+                              K == AddSubKind::ShiftSub;
+    if (kVecKind && p.chunk_len == 4u * 4u * kWave && rows_aligned(a.C, a.N, a.stride_C)) {
+        // four limbs per lane (in place is fine: a lane reads the limbs it
+        // writes before writing them); 16-byte B (and A) loads where their
+        // rows allow
+        const bool a_vec = rows_aligned(a.A, a.N, a.stride_A), b_vec = rows_aligned(a.B, a.N, a.stride_B);
+        hipLaunchKernelGGL(addsub_vec_kernel<K>, dim3((uint32_t)blocks), dim3(256), 0, stream, a, a_vec, b_vec,
+                           p.chunks_per_row, total, fl);
+    } else {
     switch (p.seg) {
         case 1: hipLaunchKernelGGL((addsub_chunk_kernel<K, 1>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, sp, p.chunks_per_row, total, fl); break;
         case 2: hipLaunchKernelGGL((addsub_chunk_kernel<K, 2>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, sp, p.chunks_per_row, total, fl); break;
         case 4: hipLaunchKernelGGL((addsub_chunk_kernel<K, 4>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, sp, p.chunks_per_row, total, fl); break;
         case 8: hipLaunchKernelGGL((addsub_chunk_kernel<K, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, sp, p.chunks_per_row, total, fl); break;
         default: hipLaunchKernelGGL((addsub_chunk_kernel<K, 16>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, sp, p.chunks_per_row, total, fl); break;
+    }
     }
     GPUPI_RETURN_IF(hipGetLastError());
     if (need_fix) {

@@ -423,9 +423,9 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     const BatchMPArray Ro = view_of(R.data + R.stride, R.length, half, R.stride * 2);
     // P = P_e P_o (the last group's P is never used once rows are whole
     // groups) and P*R = P_e R_o share P_e's transform
-    // product rows 16-byte aligned, except in the batched leaf forms, whose
-    // buffers keep the reference's preallocated sizes (and may be frozen for
-    // graph capture)
+    // product and R rows 16-byte aligned, except in the batched leaf forms,
+    // whose buffers keep the reference's preallocated sizes (and may be
+    // frozen for graph capture)
     auto out_view = [&](Buf &b, uint32_t batch, uint32_t length, BatchMPArray &v) {
         return compacting ? b.view_aligned(batch, length, v) : b.view(batch, length, v);
     };
@@ -447,7 +447,7 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     const uint32_t shift = 15u * terms_right;
     const uint32_t shifted_len = R1.length + (shift >> 5) + ((shift & 31u) != 0);
     const uint32_t rlen = std::max(shifted_len, R2.length) + (subtract ? 0u : 1u);
-    GPUPI_RETURN_IF(t.bRn.view(half, rlen, Rn));
+    GPUPI_RETURN_IF(out_view(t.bRn, half, rlen, Rn));
     GPUPI_RETURN_IF(p_shift_addsub(ctx, R1, R2, shift, subtract, n1, Rn));
     if (compacting) GPUPI_RETURN_IF(compact(ctx, Rn, "R", n1));
     t.bP.swap(t.bPn);
