@@ -106,6 +106,24 @@ size_t round_block(size_t bytes) {
 }
 }  // namespace
 
+hipError_t compact_arrays(BatchMPContext *ctx, BatchMPArray *const *arrays, int n) {
+    if (ctx == nullptr || n < 0 || n > kScalarWords) return hipErrorInvalidValue;
+    for (int i = 0; i < n; ++i)
+        if (arrays[i] == nullptr || !valid_array(*arrays[i])) return hipErrorInvalidValue;
+    GPUPI_RETURN_IF(launch_fill_u32(ctx->d_scalar, (uint64_t)n, 0u, ctx->stream));
+    for (int i = 0; i < n; ++i) {
+        const BatchMPArray &A = *arrays[i];
+        if (A.batch_size == 0 || A.length == 0) continue;
+        GPUPI_RETURN_IF(launch_length_max(A.data, A.batch_size, A.length, A.stride, false, ctx->d_scalar + i,
+                                          ctx->stream));
+    }
+    GPUPI_RETURN_IF(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) arrays[i]->length = ctx->h_scalar[i];
+    return hipSuccess;
+}
+
 hipError_t BlockCache::acquire(hipStream_t stream, size_t bytes, void **ptr, size_t *got) {
     *ptr = nullptr;
     *got = 0;
@@ -189,8 +207,8 @@ BatchMPContext *batch_mp_init(void) {
     ctx->ntt_min_limbs = kDefaultNttMinLimbs;
     ctx->cache = new (std::nothrow) gpupi::BlockCache();
     if (ctx->cache == nullptr || gpupi::ntt_tables_init(&ctx->ntt_tables) != hipSuccess ||
-        hipMalloc(&ctx->d_scalar, sizeof(uint32_t)) != hipSuccess ||
-        hipHostMalloc(&ctx->h_scalar, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        hipMalloc(&ctx->d_scalar, gpupi::kScalarWords * sizeof(uint32_t)) != hipSuccess ||
+        hipHostMalloc(&ctx->h_scalar, gpupi::kScalarWords * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         batch_mp_destroy(ctx);
         return nullptr;
     }

@@ -148,6 +148,12 @@ class BlockCache {
     uint64_t allocations_ = 0;
 };
 
+constexpr int kScalarWords = 4;  // device words of BatchMPContext::d_scalar
+
+// Compacts up to kScalarWords arrays with one host round trip: each
+// array's length becomes its batch's limb length (batch_mp_array_compact).
+hipError_t compact_arrays(BatchMPContext *ctx, BatchMPArray *const *arrays, int n);
+
 }  // namespace gpupi
 
 // Context internals shared between arith.hip and the pipeline sources.
@@ -159,7 +165,7 @@ struct BatchMPContext {
     NTTPrecomputedTables ntt_tables;
     uint32_t *workspace;
     size_t workspace_size_bytes;
-    uint32_t *d_scalar;       // device word for length reductions
+    uint32_t *d_scalar;       // kScalarWords device words for length reductions
     uint32_t *h_scalar;       // pinned host mirror
     hipStream_t stream;
     int ntt_min_limbs;        // schoolbook -> NTT crossover on L_a + L_b

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <new>
 #include <vector>
@@ -213,6 +214,21 @@ hipError_t p_mul_pair(BatchMPContext *ctx, const char *label, const char *label1
 hipError_t compact(BatchMPContext *ctx, BatchMPArray &a, const char *label = "", uint32_t n1 = 0) {
     return stage(ctx, "compact", label, n1, a.batch_size, a.length, 0, false,
                  [&] { return batch_mp_array_compact(ctx, &a); });
+}
+
+// Compactions sharing one host round trip, for arrays whose lengths nothing
+// launched in between reads (entry sizes are the first array's).
+hipError_t compact_many(BatchMPContext *ctx, std::initializer_list<BatchMPArray *> arrays, const char *label,
+                        uint32_t n1) {
+    BatchMPArray *list[gpupi::kScalarWords];
+    int n = 0;
+    for (BatchMPArray *a : arrays) {
+        if (n == gpupi::kScalarWords) return hipErrorInvalidValue;
+        list[n++] = a;
+    }
+    if (n == 0) return hipSuccess;
+    return stage(ctx, "compact", label, n1, list[0]->batch_size, list[0]->length, 0, false,
+                 [&] { return gpupi::compact_arrays(ctx, list, n); });
 }
 
 hipError_t p_shift_addsub(BatchMPContext *ctx, const BatchMPArray &A, const BatchMPArray &B, uint32_t shift,
@@ -433,7 +449,6 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     GPUPI_RETURN_IF(out_view(t.bR2, half, R.length + P.length, R2));
     if (Po.batch_size > 0) {
         GPUPI_RETURN_IF(p_mul_pair(ctx, "P,P*R", "P", "P*R", n1, true, Po, Ro, Pe, Pn, R2));
-        if (compacting) GPUPI_RETURN_IF(compact(ctx, Pn, "P", n1));
     } else {
         GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, Pe, Ro, R2));
     }
@@ -441,7 +456,6 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     GPUPI_RETURN_IF(out_view(t.bQn, half, Q.length * 2, Qn));
     GPUPI_RETURN_IF(out_view(t.bR1, half, R.length + Q.length, R1));
     GPUPI_RETURN_IF(p_mul_pair(ctx, "Q,R*Q", "Q", "R*Q", n1, false, Qe, Re, Qo, Qn, R1));
-    if (compacting) GPUPI_RETURN_IF(compact(ctx, Qn, "Q", n1));
     // R(a,c) = R(a,b) Q(b,c) 2^(15 terms(b,c)) +- P(a,b) R(b,c)
     const uint32_t terms_right = (n / n1) + ((n / n1) >> 4);
     const uint32_t shift = 15u * terms_right;
@@ -449,7 +463,9 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     const uint32_t rlen = std::max(shifted_len, R2.length) + (subtract ? 0u : 1u);
     GPUPI_RETURN_IF(out_view(t.bRn, half, rlen, Rn));
     GPUPI_RETURN_IF(p_shift_addsub(ctx, R1, R2, shift, subtract, n1, Rn));
-    if (compacting) GPUPI_RETURN_IF(compact(ctx, Rn, "R", n1));
+    // the level's three compactions in one round trip (R is formed from the
+    // products' bound lengths, as it would be after compacting P and Q)
+    if (compacting) GPUPI_RETURN_IF(compact_many(ctx, {&Pn, &Qn, &Rn}, "P,Q,R", n1));
     t.bP.swap(t.bPn);
     t.bQ.swap(t.bQn);
     t.bR.swap(t.bRn);
@@ -623,9 +639,7 @@ hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t N, BatchMPArray *Q_o
     // the 17-term group stage with bound lengths (no synchronization inside),
     // then one compaction of each array
     GPUPI_RETURN_IF(leaf_stage_fused(ctx, t, N, P, Q, R));
-    GPUPI_RETURN_IF(compact(ctx, P, "P", 2 * N));
-    GPUPI_RETURN_IF(compact(ctx, Q, "Q", 2 * N));
-    GPUPI_RETURN_IF(compact(ctx, R, "R", 2 * N));
+    GPUPI_RETURN_IF(compact_many(ctx, {&P, &Q, &R}, "P,Q,R", 2 * N));
     for (uint32_t n1 = N; n1 > 1; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, true, P, Q, R));
     // Q and R now hold one row each; hand their buffers to the caller.
     *Q_out = view_of(t.bQ.release(), Q.length, 1, Q.stride);
@@ -649,24 +663,21 @@ hipError_t gpupi_sqrt_10005(BatchMPContext *ctx, int target_digits, BatchMPArray
         valid_digits *= 2;
         const uint32_t sd = (uint32_t)valid_digits;
         BatchMPArray PQ, Qn, PP, QQ, QQ5, Pn;
+        // intermediate results keep their bound lengths (their top limbs may
+        // be zero); only the new P and Q are compacted, in one round trip
         GPUPI_RETURN_IF(bPQ.view(1, P.length + Q.length, PQ));
         GPUPI_RETURN_IF(p_mul(ctx, "P*Q", sd, P, Q, PQ));
-        GPUPI_RETURN_IF(compact(ctx, PQ, "P*Q", sd));
         GPUPI_RETURN_IF(bQn.view(1, PQ.length + 1, Qn));
         GPUPI_RETURN_IF(p_shift_bits(ctx, "2*P*Q", sd, PQ, Qn, 1));
-        GPUPI_RETURN_IF(compact(ctx, Qn, "Q", sd));
         GPUPI_RETURN_IF(bPP.view(1, P.length * 2, PP));
         GPUPI_RETURN_IF(p_mul(ctx, "P*P", sd, P, P, PP));
-        GPUPI_RETURN_IF(compact(ctx, PP, "P*P", sd));
         GPUPI_RETURN_IF(bQQ.view(1, Q.length * 2, QQ));
         GPUPI_RETURN_IF(p_mul(ctx, "Q*Q", sd, Q, Q, QQ));
-        GPUPI_RETURN_IF(compact(ctx, QQ, "Q*Q", sd));
         GPUPI_RETURN_IF(bQQ5.view(1, QQ.length + 1, QQ5));
         GPUPI_RETURN_IF(p_mul_small(ctx, "10005*Q*Q", sd, QQ, 10005u, QQ5));
-        GPUPI_RETURN_IF(compact(ctx, QQ5, "10005*Q*Q", sd));
         GPUPI_RETURN_IF(bPn.view(1, std::max(PP.length, QQ5.length) + 1, Pn));
         GPUPI_RETURN_IF(p_add(ctx, "P*P+10005*Q*Q", sd, PP, QQ5, Pn));
-        GPUPI_RETURN_IF(compact(ctx, Pn, "P", sd));
+        GPUPI_RETURN_IF(compact_many(ctx, {&Pn, &Qn}, "P,Q", sd));
         bP.swap(bPn);
         bQ.swap(bQn);
         P = Pn;
