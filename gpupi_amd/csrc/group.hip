@@ -17,14 +17,13 @@
 // Carries never cross from one product into the next because every product
 // fits its slot; the shifted add/sub of R works the same way.
 #include "common.hpp"
+#include "exact_div.hpp"
 #include "leaf_terms.hpp"
 
 namespace gpupi {
 namespace {
 
 constexpr uint32_t kGroupsPerBlock = 4;  // one wave each
-constexpr uint64_t kBlockDivisor = 130016887243243ull;  // 830297 * 156590819 (47 bits)
-constexpr uint32_t kBlockDivisorInv = 0x33261ec3u;      // its inverse mod 2^32
 
 // LDS words per group: leaf terms (16 x 13) | head term (13) | two product
 // areas | two R areas.
@@ -134,27 +133,21 @@ __device__ __forceinline__ void group_level(uint32_t lane, const Nodes &ev, cons
     __syncthreads();
 }
 
-// Exact division of a[0..n) by the block divisor, in place (Hensel, from the
-// low limb up: q_i = (a_i - c) d^-1 mod 2^32, c' = (q_i d + c - a_i) / 2^32).
-// Lanes divide different rows in one loop of NMAX >= n steps (a lane idles
-// past its n), and the next limb is read from LDS while the current one is
-// divided.
-// With d = d_lo + 2^32 d_hi and c = c_lo + 2^32 c_hi: q d_lo + c_lo - a_i is
-// 2^32 [a_i < c_lo] (it is 0 mod 2^32 and above -2^32), so
-// c' = q d_hi + mulhi(q, d_lo) + c_hi + [a_i < c_lo] (< d).
+// Exact division of a[0..n) by the block divisor, in place (hensel_step
+// from the low limb up).  Lanes divide different rows in one loop of
+// NMAX >= n steps (a lane idles past its n), and the next limb is read from
+// LDS while the current one is divided.
 template <int NMAX>
 __device__ __forceinline__ void exact_div_block(uint32_t *a, int n) {
-    constexpr uint32_t kDlo = (uint32_t)kBlockDivisor, kDhi = (uint32_t)(kBlockDivisor >> 32);
     uint64_t c = 0;
     uint32_t next = a[0];
     for (int i = 0; i < NMAX; ++i) {
         const uint32_t ai = next;
         if (i + 1 < n) next = a[i + 1];
-        const uint32_t clo = (uint32_t)c, chi = (uint32_t)(c >> 32);
-        const uint32_t q = (ai - clo) * kBlockDivisorInv;
-        const uint64_t s = (uint64_t)__umulhi(q, kDlo) + chi + (ai < clo ? 1u : 0u);
+        uint64_t c1 = c;
+        const uint32_t q = hensel_step(ai, c1);
         if (i < n) {
-            c = (uint64_t)q * kDhi + s;
+            c = c1;
             a[i] = q;
         }
     }

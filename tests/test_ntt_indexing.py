@@ -28,3 +28,16 @@ def test_ntt_arith_bounds(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "bounds check: ok" in out.stdout
+
+
+def test_group_exact_division_step(tmp_path):
+    """The 17-term group stage's exact division by 13^2 17^3 7^6 11^3
+    (gpupi_amd/csrc/exact_div.hpp): the short carry step equals the 128-bit
+    step on random and edge inputs, and whole divisions of multiples of the
+    divisor return the multiplier."""
+    exe = tmp_path / "hensel_step_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "gpupi_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "native", "hensel_step_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "hensel check: ok" in out.stdout
