@@ -141,12 +141,12 @@ __global__ void __launch_bounds__(256) extract_shifted_rows_kernel(const uint32_
                                                                    uint32_t dst_stride, uint32_t dst_length,
                                                                    uint32_t rows, uint32_t shift_bits,
                                                                    uint32_t dst_row_offset, uint32_t dst_row_step) {
-    const uint64_t total = (uint64_t)rows * dst_length;
+    // rows * dst_length < 2^32 (checked at launch): 32-bit index arithmetic
+    const uint32_t total = rows * dst_length;
     const uint32_t ws = shift_bits >> 5, bs = shift_bits & 31u;
-    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (uint64_t)blockDim.x * gridDim.x) {
-        const uint32_t r = (uint32_t)(idx / dst_length);
-        const uint32_t j = (uint32_t)(idx - (uint64_t)r * dst_length);
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += blockDim.x * gridDim.x) {
+        const uint32_t r = idx / dst_length;
+        const uint32_t j = idx - r * dst_length;
         const uint64_t k = (uint64_t)j + ws;
         const uint32_t *s = src + (size_t)r * src_stride;
         uint64_t v = 0;
@@ -163,15 +163,22 @@ __global__ void __launch_bounds__(256) extract_low_bits_rows_kernel(const uint32
                                                                     uint32_t *dst, uint32_t dst_stride,
                                                                     uint32_t dst_length, uint32_t rows,
                                                                     uint32_t top_mask) {
-    const uint64_t total = (uint64_t)rows * dst_length;
-    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (uint64_t)blockDim.x * gridDim.x) {
-        const uint32_t r = (uint32_t)(idx / dst_length);
-        const uint32_t j = (uint32_t)(idx - (uint64_t)r * dst_length);
+    // rows * dst_length < 2^32 (checked at launch): 32-bit index arithmetic
+    const uint32_t total = rows * dst_length;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += blockDim.x * gridDim.x) {
+        const uint32_t r = idx / dst_length;
+        const uint32_t j = idx - r * dst_length;
         uint32_t v = src[(size_t)r * src_stride + j];
         if (j + 1u == dst_length) v &= top_mask;
         dst[(size_t)r * dst_stride + j] = v;
     }
+}
+
+// Limb `index` of each row ANDed with `mask`, in place.
+__global__ void __launch_bounds__(256) mask_limb_rows_kernel(uint32_t *a, uint32_t stride, uint32_t rows,
+                                                             uint32_t index, uint32_t mask) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows) a[(size_t)r * stride + index] &= mask;
 }
 
 uint32_t grid_for(uint64_t total) {
@@ -202,15 +209,27 @@ hipError_t launch_decimal_leaf(const uint32_t *A, char *B, uint32_t N, uint32_t 
 hipError_t launch_extract_shifted(const BatchMPArray &src, BatchMPArray &dst, uint32_t rows, uint32_t shift_bits,
                                   uint32_t dst_row_offset, uint32_t dst_row_step, hipStream_t stream) {
     if (dst.length == 0 || rows == 0) return hipSuccess;
+    if ((uint64_t)rows * dst.length > 0xffffffffull) return hipErrorInvalidValue;
     hipLaunchKernelGGL(extract_shifted_rows_kernel, dim3(grid_for((uint64_t)rows * dst.length)), dim3(256), 0,
                        stream, src.data, src.stride, src.length, dst.data, dst.stride, dst.length, rows,
                        shift_bits, dst_row_offset, dst_row_step);
     return hipGetLastError();
 }
 
+hipError_t launch_keep_low_bits(BatchMPArray &a, uint32_t rows, uint32_t valid_bits, hipStream_t stream) {
+    const uint32_t len = (uint32_t)ceil_div64(valid_bits, 32);
+    if (len > a.length) return hipErrorInvalidValue;
+    a.length = len;
+    if (len == 0 || rows == 0 || (valid_bits & 31u) == 0) return hipSuccess;
+    hipLaunchKernelGGL(mask_limb_rows_kernel, dim3(ceil_div(rows, 256)), dim3(256), 0, stream, a.data, a.stride, rows,
+                       len - 1, (1u << (valid_bits & 31u)) - 1u);
+    return hipGetLastError();
+}
+
 hipError_t launch_extract_low(const BatchMPArray &src, BatchMPArray &dst, uint32_t rows, uint32_t valid_bits,
                               hipStream_t stream) {
     if (dst.length == 0 || rows == 0) return hipSuccess;
+    if ((uint64_t)rows * dst.length > 0xffffffffull) return hipErrorInvalidValue;
     uint32_t mask = 0xffffffffu;
     if (valid_bits & 31u) mask = (1u << (valid_bits & 31u)) - 1u;
     hipLaunchKernelGGL(extract_low_bits_rows_kernel, dim3(grid_for((uint64_t)rows * dst.length)), dim3(256), 0,

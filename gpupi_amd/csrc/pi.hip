@@ -28,6 +28,7 @@ hipError_t launch_extract_shifted(const BatchMPArray &src, BatchMPArray &dst, ui
                                   uint32_t dst_row_offset, uint32_t dst_row_step, hipStream_t stream);
 hipError_t launch_extract_low(const BatchMPArray &src, BatchMPArray &dst, uint32_t rows, uint32_t valid_bits,
                               hipStream_t stream);
+hipError_t launch_keep_low_bits(BatchMPArray &a, uint32_t rows, uint32_t valid_bits, hipStream_t stream);
 }  // namespace gpupi
 
 namespace {
@@ -1031,8 +1032,16 @@ hipError_t convert_decimal_equal_split(BatchMPContext *ctx, BatchMPArray input, 
             GPUPI_RETURN_IF(gpupi::launch_extract_shifted(cur, next, cur.batch_size, bits[l] - child_bits, 0, 2,
                                                           ctx->stream));
             const uint32_t mant_bits = bits[l] - child_digits;
-            GPUPI_RETURN_IF(bmant.view(cur.batch_size, ceil_div(mant_bits, 32), mant));
-            GPUPI_RETURN_IF(gpupi::launch_extract_low(cur, mant, cur.batch_size, mant_bits, ctx->stream));
+            if (l == 0) {
+                // the caller's rows: a copy of their low bits
+                GPUPI_RETURN_IF(bmant.view(cur.batch_size, ceil_div(mant_bits, 32), mant));
+                GPUPI_RETURN_IF(gpupi::launch_extract_low(cur, mant, cur.batch_size, mant_bits, ctx->stream));
+            } else {
+                // this conversion's own rows, not needed again once their
+                // left halves are out: their low bits kept in place
+                mant = cur;
+                GPUPI_RETURN_IF(gpupi::launch_keep_low_bits(mant, cur.batch_size, mant_bits, ctx->stream));
+            }
             const BatchMPArray &pw = table->powers[levels - 1 - l];
             const BatchMPArray power = view_of(pw.data, pw.length, cur.batch_size, 0);
             GPUPI_RETURN_IF(bprod.view_aligned(cur.batch_size, mant.length + power.length, prod));
