@@ -222,14 +222,20 @@ GPUPI_HD SmallDft small_dft(int m, int s, uint32_t q, bool col) {
 }
 
 // Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin:
-// block b runs on XCD b mod 8.  xcd_tile(b, n) renumbers blocks so that
-// each XCD gets one contiguous range of tiles, and neighbouring tiles (which
-// share 128-byte lines of limbs) meet in one L2.  A bijection on [0, n).
+// block b runs on XCD b mod 8.  xcd_tile(b, n) renumbers blocks in groups
+// of 8 * kXcdChunk: within a group each XCD gets kXcdChunk consecutive
+// tiles, so neighbouring tiles (which share 128-byte lines of limbs) meet
+// in one L2 while the blocks resident at one time still cover one compact
+// range of memory.  Blocks past the last whole group keep their number.  A
+// bijection on [0, n).  Measured at 10^9 digits: one contiguous range per
+// XCD 961.8 ms of kernels per step, chunks of 16 / 32 / 128 / 256 tiles
+// 940.7 / 940.5 / 941.0 / 944.0 ms, the plain round-robin order 983.2 ms.
 constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdChunk = 32;
 GPUPI_HD uint32_t xcd_tile(uint32_t b, uint32_t n) {
-    const uint32_t x = b % kXcds, i = b / kXcds;
-    const uint32_t q = n / kXcds, r = n % kXcds;
-    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    constexpr uint32_t kGroup = kXcds * kXcdChunk;
+    if (b >= (n / kGroup) * kGroup) return b;
+    return (b / kGroup) * kGroup + (b % kXcds) * kXcdChunk + (b / kXcds) % kXcdChunk;
 }
 
 // Run length of the final inverse column level (limbs per run): the C

@@ -155,6 +155,10 @@ static void check_xcd_tile() {
             if (t < n) seen[t]++;
         }
         for (uint32_t t = 0; t < n; ++t) EXPECT(seen[t] == 1, "xcd_tile bijection n=%u", n);
+        // inside whole groups, each run of kXcdChunk tiles runs on one XCD
+        const uint32_t whole = (n / (kXcds * kXcdChunk)) * (kXcds * kXcdChunk);
+        for (uint32_t b = 0; b < whole; ++b)
+            EXPECT((xcd_tile(b, n) / kXcdChunk) % kXcds == b % kXcds, "xcd_tile chunk on one XCD n=%u", n);
     }
 }
 
