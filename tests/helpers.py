@@ -56,3 +56,25 @@ def closed_form(kind: str, a: int, b: int, scalar: int, shift: int, lc: int) -> 
     if kind == "mul":
         return a * b
     raise ValueError(kind)
+
+
+def run_jobs(srv, pis, muls):
+    """Submits the jobs, waits for every result and returns them keyed by
+    (kind, size); the keys must be distinct (jobs of one shape would share a
+    key, so they are refused here rather than counted once)."""
+    ids = {}
+    for d in pis:
+        ids[srv.submit_pi(d, keep_digits=True)] = ("pi", d)
+    for A, B in muls:
+        ids[srv.submit_mul(A, B)] = ("mul", (A.size, B.size))
+    assert all(i >= 0 for i in ids)
+    assert len(set(ids.values())) == len(ids), "jobs of one shape in one call"
+    out, received = {}, 0
+    while received < len(ids):
+        got = srv.results(wait=True)
+        assert got, "the server returned no result while jobs were outstanding"
+        for r, data in got:
+            assert r.status == 0, (ids[r.id], r.status)
+            out[ids[r.id]] = (r, data)
+            received += 1
+    return out

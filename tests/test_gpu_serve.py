@@ -6,7 +6,7 @@ import pytest
 
 import gpupi_amd as g
 import oracle_ffi as of
-from helpers import rand_rows
+from helpers import rand_rows, run_jobs
 
 pytestmark = pytest.mark.gpu
 
@@ -19,21 +19,6 @@ def oracle_mul(A, B):
     assert of.lib().o_mul(of.HostArray(A.reshape(1, -1)).oarray(), of.HostArray(B.reshape(1, -1)).oarray(),
                           hc.oarray()) == 0
     return hc.rows()[0]
-
-
-def run_jobs(srv, pis, muls):
-    ids = {}
-    for d in pis:
-        ids[srv.submit_pi(d, keep_digits=True)] = ("pi", d)
-    for A, B in muls:
-        ids[srv.submit_mul(A, B)] = ("mul", (A.size, B.size))
-    assert all(i >= 0 for i in ids)
-    out = {}
-    while len(out) < len(ids):
-        for r, data in srv.results(wait=True):
-            assert r.status == 0, (ids[r.id], r.status)
-            out[ids[r.id]] = (r, data)
-    return out
 
 
 def test_mix_equals_alone_and_oracle():
@@ -64,16 +49,14 @@ def test_mix_equals_alone_and_oracle():
     assert max(r.batch_size for k, (r, _) in mix.items() if k[0] == "mul") > 1
 
 
-@pytest.mark.skip(reason="stalled for over 180 s on the MI355X box (no output, GPU still responsive); the "
-                         "serialized-budget scenario passes in the CPU simulation of the server "
-                         "(tests/native/serve_sim.cpp, also under ThreadSanitizer) and no cause has been "
-                         "found in the host code, so this case stays off the GPU until one is")
 def test_small_budget_serializes_but_stays_exact():
     rng = np.random.default_rng(3)
-    muls = [(rand_rows(rng, 1, 1 << 19)[0], rand_rows(rng, 1, 1 << 19)[0]) for _ in range(3)]
-    est_mul = g.lib().gpupi_serve_job_bytes(g.JOB_MUL, 1 << 19, 1 << 19)
+    # three shapes (results are keyed by shape), each over half the budget
+    muls = [(rand_rows(rng, 1, (1 << 19) - d)[0], rand_rows(rng, 1, 1 << 19)[0]) for d in range(3)]
     need = g.lib().gpupi_serve_job_bytes(g.JOB_PI, 5000, 0)
-    assert need / 2 < est_mul <= need  # any two jobs exceed the budget
+    for A, B in muls:
+        est_mul = g.lib().gpupi_serve_job_bytes(g.JOB_MUL, A.size, B.size)
+        assert need / 2 < est_mul <= need  # any two jobs exceed the budget
     srv = g.Server(budget_bytes=need, pi_workers=2)
     try:
         out = run_jobs(srv, [5000], muls)

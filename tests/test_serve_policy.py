@@ -1,6 +1,10 @@
 """CPU: the serving loop's admission rule and footprint estimate
 (gpupi_serve.h), called through the C ABI with no device work."""
+import numpy as np
+import pytest
+
 import gpupi_amd as g
+from helpers import run_jobs
 
 GB = 10 ** 9
 PI, MUL = g.JOB_PI, g.JOB_MUL
@@ -57,3 +61,50 @@ def test_server_threading_simulation(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "serve sim: ok" in out.stdout
+
+
+class _FakeServer:
+    """Returns every submitted job's result at the first results() call."""
+
+    class _Result:
+        def __init__(self, jid):
+            self.id, self.status = jid, 0
+
+    def __init__(self, lose: int = 0):
+        self.next, self.queued, self.lose = 0, [], lose
+
+    def _submit(self):
+        self.queued.append(self.next)
+        self.next += 1
+        return self.next - 1
+
+    def submit_pi(self, digits, keep_digits=False):
+        return self._submit()
+
+    def submit_mul(self, A, B):
+        return self._submit()
+
+    def results(self, wait=True):
+        out = [(self._Result(i), None) for i in self.queued[self.lose:]]
+        self.queued = []
+        return out
+
+
+def test_run_jobs_returns_once_every_result_is_in():
+    a, b = np.zeros(5, np.uint32), np.zeros(7, np.uint32)
+    out = run_jobs(_FakeServer(), [1000, 2000], [(a, b), (b, a)])
+    assert sorted(out) == [("mul", (5, 7)), ("mul", (7, 5)), ("pi", 1000), ("pi", 2000)]
+
+
+def test_run_jobs_refuses_jobs_of_one_shape():
+    # jobs of one shape would share one key: the helper refuses them instead
+    # of waiting for a count of keys that can never be reached
+    a = np.zeros(5, np.uint32)
+    with pytest.raises(AssertionError, match="one shape"):
+        run_jobs(_FakeServer(), [], [(a, a), (a, a)])
+
+
+def test_run_jobs_stops_when_the_server_has_nothing_left():
+    a, b = np.zeros(5, np.uint32), np.zeros(7, np.uint32)
+    with pytest.raises(AssertionError, match="no result"):
+        run_jobs(_FakeServer(lose=1), [], [(a, b), (b, a)])
