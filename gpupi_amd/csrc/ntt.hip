@@ -639,9 +639,11 @@ __global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
 }
 
 // K = 2^M <= 4096, one prime: transforms of A (kept in registers) and B,
-// pointwise, inverse; the result (times 1/K) is left in `out` in tile order.
+// pointwise, inverse; res[j] is the result (times 1/K) at tile element
+// row_step0_base<M>() + j 2^(M-4), the element the last inverse step read
+// from `lds` (so it may be written back there without a barrier).
 template <int M>
-__device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, double *out, uint32_t prod0, int P) {
+__device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, uint32_t prod0, int P, double *res) {
     constexpr uint32_t kJ = 1u << (M - 4);
     const Mod m = mod_of(P);
     const double *tab = a.tables + P * kPrimeTable;
@@ -672,21 +674,45 @@ __device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, doub
     inv_dft<M, false>(v, lds, tab, P, m);
     const double kinv = c_ntt.kinv[P][M];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) out[lds_index(i0 + j * kJ)] = mulp(v[j], kinv, m);
+    for (int j = 0; j < 16; ++j) res[j] = mulp(v[j], kinv, m);
 }
 
 // K = 2^M <= 4096: whole products inside one tile (4096 / K per block).
+// One tile buffer: both primes transform in `work` in turn, prime 0's
+// results wait in registers, and the recombined coefficients (below 2^75:
+// at most 2^11 products of two limbs each) replace the thread's own
+// elements as a low 64-bit word in `work` and a high word in `hi_w`.
+// 52.2 KB of LDS, three blocks per CU.
 template <int M>
 __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
-    __shared__ double l0[kLdsSlots];
-    __shared__ double l1[kLdsSlots];
-    __shared__ uint64_t spill_s[256];
+    __shared__ double work[kLdsSlots];
+    __shared__ uint64_t hi_w[kLdsSlots / 2];  // 32-bit high words, then the spill words
     __shared__ uint32_t wave_flags[4];
+    constexpr uint32_t kJ = 1u << (M - 4);
+    uint64_t *const lo_s = reinterpret_cast<uint64_t *>(work);
+    uint32_t *const hi_s = reinterpret_cast<uint32_t *>(hi_w);
     const uint32_t prod0 = xcd_tile(blockIdx.x, gridDim.x) * (kTile >> M);
-    // prime 0 transforms in l1 and leaves its result in l0; prime 1 the reverse
-    local_prime<M>(a, l1, l0, prod0, 0);
+    // prime 0's results: the first half stays in registers, the second
+    // half waits in hi_w (8 x 256 words, lane-consecutive)
+    double res0[16], res1[16];
+    local_prime<M>(a, work, prod0, 0, res0);
+    double *const park = reinterpret_cast<double *>(hi_w);
+#pragma unroll
+    for (int j = 8; j < 16; ++j) park[(j - 8) * 256 + threadIdx.x] = res0[j];
     __syncthreads();
-    local_prime<M>(a, l1, l1, prod0, 1);
+    local_prime<M>(a, work, prod0, 1, res1);
+#pragma unroll
+    for (int j = 8; j < 16; ++j) res0[j] = park[(j - 8) * 256 + threadIdx.x];
+    // every parked word is read before the high words take their place
+    __syncthreads();
+    const uint32_t is0 = row_step0_base<M>();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t e = lds_index(is0 + j * kJ);
+        const unsigned __int128 c = recombine(res0[j], res1[j]);
+        lo_s[e] = (uint64_t)c;
+        hi_s[e] = (uint32_t)(c >> 64);
+    }
     __syncthreads();
 
     // each thread: 16 consecutive coefficients of one product
@@ -698,10 +724,13 @@ __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
     unsigned __int128 pending = 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        pending += recombine(l0[e0 + j], l1[e0 + j]);
+        pending += ((unsigned __int128)hi_s[e0 + j] << 64) + lo_s[e0 + j];
         limb[j] = (uint32_t)pending;
         pending >>= 32;
     }
+    // every high word is read before the spill words take their place
+    __syncthreads();
+    uint64_t *const spill_s = hi_w;
     spill_s[threadIdx.x] = (uint64_t)pending;
     __syncthreads();
     const uint64_t sin = (r0 == 0) ? 0ull : spill_s[threadIdx.x - 1];
