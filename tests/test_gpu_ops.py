@@ -39,6 +39,9 @@ CASES = [
     (2, 200, 150, 201, "random"), (2, 200, 150, 120, "random"), (5, 1030, 900, 1031, "top"),
     (3, 5000, 4000, 5001, "ones"), (2, 70000, 3, 70001, "top"), (1, 300000, 299999, 300001, "random"),
     (1000, 40, 33, 41, "sparse"),
+    # rows of 129..512 limbs: one wave of four or eight 64-limb segments per row
+    (3, 150, 120, 151, "ones"), (2, 300, 280, 301, "random"), (2, 380, 10, 381, "top"),
+    (2, 440, 400, 441, "ones"), (2, 500, 500, 512, "random"),
     # rows a multiple of 4 limbs: 16-byte aligned rows, four limbs per lane
     (3, 1030, 900, 1032, "top"), (4, 5000, 4000, 5004, "ones"), (3, 2048, 1500, 2052, "random"),
 ]
