@@ -679,7 +679,8 @@ __device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, uint
 
 // K = 2^M <= 4096: whole products inside one tile (4096 / K per block).
 // One tile buffer: both primes transform in `work` in turn, prime 0's
-// results wait in registers, and the recombined coefficients (below 2^75:
+// results wait half in registers and half in `hi_w` (16 KB), and the
+// recombined coefficients (below 2^75:
 // at most 2^11 products of two limbs each) replace the thread's own
 // elements as a low 64-bit word in `work` and a high word in `hi_w`.
 // 52.2 KB of LDS, three blocks per CU.
@@ -693,7 +694,7 @@ __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
     uint32_t *const hi_s = reinterpret_cast<uint32_t *>(hi_w);
     const uint32_t prod0 = xcd_tile(blockIdx.x, gridDim.x) * (kTile >> M);
     // prime 0's results: the first half stays in registers, the second
-    // half waits in hi_w (8 x 256 words, lane-consecutive)
+    // half waits in hi_w (8 x 256 doubles, lane-consecutive)
     double res0[16], res1[16];
     local_prime<M>(a, work, prod0, 0, res0);
     double *const park = reinterpret_cast<double *>(hi_w);
