@@ -147,10 +147,10 @@ __global__ void __launch_bounds__(256) length_wave_kernel(const uint32_t *A, uin
 }
 
 // Long rows: 4096-limb chunks counted from the top of each row.  Block g of
-// a row scans chunks g, g + G, g + 2G, ... (G blocks per row) and stops at
-// the first that lies wholly below the best length found so far, so the work
-// is about one chunk per row plus the zero tail above the top limb of the
-// longest rows.  Rows are taken last first: in the binary-splitting tree the
+// a row scans chunks g, g + G, g + 2G, ... (G blocks per row), each only
+// above the best length found so far, and stops at the first chunk that lies
+// wholly below it, so the work is about the limbs above the best length in
+// each row plus the zero tail above the top limb of the longest rows.  Rows are taken last first: in the binary-splitting tree the
 // last rows (the largest terms) are the longest, so the best length is known
 // before the shorter rows' zero tails are reached.  G grows for few rows, so
 // that a lone row's zero tail (a Newton residual) is scanned by many blocks.
@@ -161,7 +161,7 @@ constexpr uint32_t kLenBlocks = 4096;  // blocks per launch when rows are few
 __global__ void __launch_bounds__(256) length_chunk_kernel(const uint32_t *A, uint32_t N, uint32_t L,
                                                            uint32_t stride_A, uint32_t chunks_per_row,
                                                            uint32_t blocks_per_row, int bits, uint32_t *result) {
-    __shared__ uint32_t skip;
+    __shared__ int64_t start;
     const uint32_t row = N - 1u - blockIdx.x / blocks_per_row;
     const uint32_t g = blockIdx.x % blocks_per_row;
     const uint32_t *a = A + (size_t)row * stride_A;
@@ -169,19 +169,21 @@ __global__ void __launch_bounds__(256) length_chunk_kernel(const uint32_t *A, ui
         const int64_t hi = (int64_t)L - (int64_t)c * kLenChunk;  // chunk covers [hi - 4096, hi)
         const int64_t lo = hi - (int64_t)kLenChunk > 0 ? hi - (int64_t)kLenChunk : 0;
         if (threadIdx.x == 0) {
-            // a relaxed load: a stale value only skips less
+            // a relaxed load: a stale value only scans more.  A limb below
+            // index cur (limbs) or cur / 32 (bits) cannot beat the best so far.
             const uint32_t cur = __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint64_t best_possible = bits ? 32ull * (uint64_t)hi : (uint64_t)hi;
-            skip = (uint64_t)cur >= best_possible ? 1u : 0u;
+            const int64_t first = bits ? (int64_t)(cur >> 5) : (int64_t)cur;
+            start = first > lo ? first : lo;
         }
         __syncthreads();
-        if (skip) return;  // lower chunks lie lower still
+        const int64_t from = start;
+        if (from >= hi) return;  // lower chunks lie lower still
         uint32_t best = 0;
-        for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+        for (int64_t j = from + threadIdx.x; j < hi; j += blockDim.x) {
             const uint32_t v = a[j];
             if (v != 0u) best = max(best, length_of((uint32_t)j + 1u, v, bits != 0));
         }
-        // its barrier also orders this chunk's reads of `skip` before the next write
+        // its barrier also orders this chunk's reads of `start` before the next write
         block_max_atomic(best, result);
     }
 }
