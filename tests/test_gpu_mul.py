@@ -337,13 +337,13 @@ def test_ntt_three_level(ctx):
         g.check(ctx.set_ntt_threshold(384))
 
 
-def ragged_case(ctx, lengths, seed):
-    """Upload random pairs, run one ragged call, return (device C rows, host A, B)."""
+def ragged_case(ctx, lengths, seed, mode="random"):
+    """Upload pairs (rand_rows mode), run one ragged call, return (device C rows, host A, B)."""
     rng = np.random.default_rng(seed)
     bufs, items, As, Bs, Cs = [], [], [], [], []
     for la, lb in lengths:
-        A = rand_rows(rng, 1, la) if la else np.zeros((1, 0), np.uint32)
-        B = rand_rows(rng, 1, lb) if lb else np.zeros((1, 0), np.uint32)
+        A = rand_rows(rng, 1, la, mode) if la else np.zeros((1, 0), np.uint32)
+        B = rand_rows(rng, 1, lb, mode) if lb else np.zeros((1, 0), np.uint32)
         ba = g.DeviceBuffer.from_array(A.reshape(-1)) if la else None
         bb = g.DeviceBuffer.from_array(B.reshape(-1)) if lb else None
         bc = g.DeviceBuffer.from_array(np.full(la + lb + 1, PAD, dtype=np.uint32))
@@ -372,6 +372,17 @@ def test_ragged_products(ctx):
             assert np.array_equal(C[:-1], oracle_mul(A, B)[0]), (la, lb)
         else:
             assert not np.any(C[:-1]), (la, lb)
+
+
+def test_ragged_one_tile_products_of_all_ones(ctx):
+    # the largest coefficients a one-tile product meets (2^11 products of
+    # two all-ones limbs: above 2^74), through the per-item operand path
+    lengths = [(2048, 2048), (2047, 2049), (1024, 1000), (300, 200), (2000, 96), (129, 127)]
+    err, out, As, Bs, _ = ragged_case(ctx, lengths, 9, "ones")
+    assert err == 0
+    for (la, lb), C, A, B in zip(lengths, out, As, Bs):
+        assert C[-1] == PAD, (la, lb)
+        assert np.array_equal(C[:-1], oracle_mul(A, B)[0]), (la, lb)
 
 
 def test_ragged_matches_uniform_call(ctx):
