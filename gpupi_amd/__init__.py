@@ -23,6 +23,8 @@ HIP_SUCCESS = 0
 HIP_ERROR_INVALID_VALUE = 1
 PREC_REQUESTED = 0
 PREC_ROUNDED = 1
+FORM_BATCHED = 0
+FORM_PLAIN = 1
 
 _H2D, _D2H, _D2D = 1, 2, 3
 
@@ -173,6 +175,7 @@ def _declare(lib):
         "gpupi_ntt_set_split": (err, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "gpupi_ntt_get_split": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "gpupi_stage_profile_enable": (err, [c_ctx, ctypes.c_int]),
+        "gpupi_set_pipeline_form": (err, [c_ctx, ctypes.c_int]),
         "gpupi_cache_query": (err, [c_ctx, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(u64)]),
         "gpupi_cache_trim": (err, [c_ctx]),
         "gpupi_mul_ragged": (err, [c_ctx, ctypes.POINTER(GpupiMulItem), u32]),
@@ -338,6 +341,10 @@ class Context:
         finally:
             self._lib.gpupi_pi_free(ctypes.byref(res))
         return ret, dec, res
+
+    def set_form(self, form: int) -> None:
+        """FORM_BATCHED (default) or FORM_PLAIN, the reference's per-op form."""
+        check(self._lib.gpupi_set_pipeline_form(self.ptr, form), "gpupi_set_pipeline_form")
 
     def cache_stats(self) -> tuple:
         """(bytes of device blocks cached now, device allocations made so far)."""

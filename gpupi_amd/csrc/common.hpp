@@ -171,4 +171,5 @@ struct BatchMPContext {
     int ntt_min_limbs;        // schoolbook -> NTT crossover on L_a + L_b
     StageProfiler *profiler;  // null unless stage profiling is on
     gpupi::BlockCache *cache; // blocks of the pipeline's buffers
+    int pipeline_form;        // GPUPI_FORM_BATCHED or GPUPI_FORM_PLAIN
 };

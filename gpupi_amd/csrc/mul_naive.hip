@@ -10,6 +10,8 @@
 //    handling in the inner loop: t = a*b + lo fits 64 bits).  Columns are
 //    turned into limbs with two lane shifts (y_{k-1}, z_{k-2}) and one
 //    ballot carry resolution per 256-limb group.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace gpupi {
@@ -203,6 +205,26 @@ const ThreadLauncher kThreadLaunchers[kThreadMaxShort + 1] = {
 
 }  // namespace
 
+namespace {
+// mul_wave_kernel<4> may take up to 160 KB of dynamic LDS; the attribute is
+// a property of the current device, set once per device (serving threads
+// launch concurrently).
+std::mutex g_lds_attr_mu;
+uint64_t g_lds_attr_done = 0;  // bit d: set on device d
+
+hipError_t allow_large_lds() {
+    int dev = 0;
+    GPUPI_RETURN_IF(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_lds_attr_mu);
+    const uint64_t bit = dev >= 0 && dev < 64 ? 1ull << dev : 0ull;
+    if (bit != 0 && (g_lds_attr_done & bit) != 0) return hipSuccess;
+    GPUPI_RETURN_IF(hipFuncSetAttribute((const void *)mul_wave_kernel<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        160 * 1024));
+    g_lds_attr_done |= bit;
+    return hipSuccess;
+}
+}  // namespace
+
 bool mul_naive_supports(uint32_t L_a, uint32_t L_b) {
     return (uint64_t)L_a + L_b <= kWaveMaxTotal || (L_a <= kThreadMaxShort || L_b <= kThreadMaxShort);
 }
@@ -238,12 +260,7 @@ hipError_t launch_mul_naive(const uint32_t *A, const uint32_t *B, uint32_t *C, u
     if (!wave_ok) return hipErrorInvalidValue;
     if (N > 0x7fffffffu) return hipErrorInvalidValue;
     const size_t lds = (size_t)(L_a + kPad + L_b + kPad) * sizeof(uint32_t);
-    static bool attr_set = false;
-    if (lds > 65536 && !attr_set) {
-        GPUPI_RETURN_IF(hipFuncSetAttribute((const void *)mul_wave_kernel<4>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
-    }
+    if (lds > 65536) GPUPI_RETURN_IF(allow_large_lds());
     if (total <= 2u * kWave) {
         hipLaunchKernelGGL(mul_wave_kernel<2>, dim3(N), dim3(kWave), lds, stream, A, B, C, L_a, L_b, stride_A,
                            stride_B, stride_C);

@@ -32,6 +32,7 @@
 
 #include <atomic>
 #include <cstdio>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -842,12 +843,16 @@ struct StatRecord {
     hipEvent_t start, stop;
     double bytes;
 };
+// Process-wide; several threads (the serving loop's workers) may launch at
+// once, so the records and the event pool are guarded by `mu`.
 struct Stats {
-    bool enabled = false;
+    std::atomic<bool> enabled{false};
+    std::mutex mu;
     std::vector<StatRecord> records;
     std::vector<hipEvent_t> pool;
 } g_stats;
 
+// Called with g_stats.mu held.
 hipEvent_t stats_event() {
     if (!g_stats.pool.empty()) {
         hipEvent_t e = g_stats.pool.back();
@@ -862,16 +867,22 @@ hipEvent_t stats_event() {
 // Records `launch` between two events when stats are enabled.
 template <typename F>
 hipError_t timed(int kind, int log2K, double bytes, hipStream_t stream, F &&launch) {
-    if (!g_stats.enabled) {
+    if (!g_stats.enabled.load(std::memory_order_relaxed)) {
         launch();
         return hipGetLastError();
     }
-    StatRecord r{kind, log2K, stats_event(), stats_event(), bytes};
+    StatRecord r{kind, log2K, nullptr, nullptr, bytes};
+    {
+        std::lock_guard<std::mutex> lk(g_stats.mu);
+        r.start = stats_event();
+        r.stop = stats_event();
+    }
     if (r.start == nullptr || r.stop == nullptr) return hipErrorOutOfMemory;
     GPUPI_RETURN_IF(hipEventRecord(r.start, stream));
     launch();
     GPUPI_RETURN_IF(hipGetLastError());
     GPUPI_RETURN_IF(hipEventRecord(r.stop, stream));
+    std::lock_guard<std::mutex> lk(g_stats.mu);
     g_stats.records.push_back(r);
     return hipSuccess;
 }
@@ -1422,6 +1433,7 @@ size_t batch_mul_ntt_workspace_size(uint32_t N, uint32_t L_a, uint32_t L_b) {
 
 void gpupi_kernel_stats_reset(int enable) {
     using namespace gpupi;
+    std::lock_guard<std::mutex> lk(g_stats.mu);
     for (auto &r : g_stats.records) {
         g_stats.pool.push_back(r.start);
         g_stats.pool.push_back(r.stop);
@@ -1436,6 +1448,7 @@ int stats_query(GpupiKernelStat *out, int max_entries, bool by_size) {
     using namespace gpupi;
     constexpr int kSizes = ntt::kMaxLog2 + 1;
     std::vector<GpupiKernelStat> agg((size_t)kKindCount * kSizes);
+    std::lock_guard<std::mutex> lk(g_stats.mu);
     for (auto &r : g_stats.records) {
         if (hipEventSynchronize(r.stop) != hipSuccess) return -1;
         float ms = 0.0f;

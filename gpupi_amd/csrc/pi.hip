@@ -47,33 +47,43 @@ __global__ void __launch_bounds__(256) leaf17_kernel(int is_head, uint32_t *P, u
 
 // ------------------------------------------------------- buffer helpers
 
-// A grow-only device buffer holding one batch array.  Attached to a context,
-// it takes its blocks from the context's block cache and hands them back
-// (work on the context stream only); otherwise it allocates and frees them.
+// A grow-only device buffer holding one batch array.  Attached to a context
+// in the batched form, it takes its blocks from the context's block cache
+// and hands them back (work on the context stream only); otherwise (no
+// context, or the plain form) it allocates and frees them.
 struct Buf {
     uint32_t *ptr = nullptr;
     size_t cap = 0;       // words
     bool frozen = false;  // set while a graph is being captured: no reallocation
+    bool cached = false;  // ptr is a block of ctx's cache
     BatchMPContext *ctx = nullptr;
     Buf() = default;
     explicit Buf(BatchMPContext *c) : ctx(c) {}
     Buf(const Buf &) = delete;
     Buf &operator=(const Buf &) = delete;
     ~Buf() { drop(); }
+    bool use_cache() const { return ctx != nullptr && ctx->pipeline_form != GPUPI_FORM_PLAIN; }
     void drop() {
         if (ptr == nullptr) return;
-        if (ctx) ctx->cache->give_back(ptr, cap * sizeof(uint32_t));
+        if (cached) ctx->cache->give_back(ptr, cap * sizeof(uint32_t));
         else (void)hipFree(ptr);
         ptr = nullptr;
         cap = 0;
+        cached = false;
+    }
+    // Takes ownership of a block of `words` words handed over by a stage
+    // entry point (a block of this context's cache in the batched form).
+    void adopt(uint32_t *p, size_t words) {
+        drop();
+        ptr = p;
+        cap = words;
+        cached = p != nullptr && use_cache();
     }
     hipError_t reserve(size_t words) {
         if (words <= cap) return hipSuccess;
         if (frozen) return hipErrorInvalidValue;
-        if (ctx == nullptr) {
-            if (ptr) GPUPI_RETURN_IF(hipFree(ptr));
-            ptr = nullptr;
-            cap = 0;
+        if (!use_cache()) {
+            drop();
             GPUPI_RETURN_IF(hipMalloc(&ptr, std::max<size_t>(words, 1) * sizeof(uint32_t)));
             cap = words;
             return hipSuccess;
@@ -87,6 +97,7 @@ struct Buf {
         GPUPI_RETURN_IF(ctx->cache->acquire(ctx->stream, std::max<size_t>(want, 1) * sizeof(uint32_t), &p, &got));
         ptr = static_cast<uint32_t *>(p);
         cap = got / sizeof(uint32_t);
+        cached = true;
         return hipSuccess;
     }
     // A contiguous array view (stride = length) of `batch` rows.
@@ -114,11 +125,13 @@ struct Buf {
     void swap(Buf &o) {
         std::swap(ptr, o.ptr);
         std::swap(cap, o.cap);
+        std::swap(cached, o.cached);
     }
     uint32_t *release() {
         uint32_t *p = ptr;
         ptr = nullptr;
         cap = 0;
+        cached = false;
         return p;
     }
 };
@@ -422,10 +435,20 @@ struct TreeBufs {
     }
 };
 
+// How a tree level fixes the lengths of its outputs.
+//   kBound: no compaction, the allocation bounds stay the lengths (the
+//     batched and graph forms of the group stage; nothing synchronizes);
+//   kShared: the P, Q, R compactions share one host round trip, products
+//     sharing an operand transform it once, rows 16-byte aligned (the
+//     batched pipeline);
+//   kPerOp: the reference's sequence (pi_bs_gpu.cu:637-723): P, compact P,
+//     Q, compact Q, R*Q, P*R, R, compact R, each product on its own.
+// All three leave the same integers.
+enum class Lengths { kBound, kShared, kPerOp };
+
 // One pairwise level n1 of the tree (pi_bs_gpu.cu:625-723) over the rows of
-// P, Q, R; with compacting = false the output lengths are the allocation
-// bounds and nothing synchronizes (the values are the same integers).
-hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1, bool compacting, BatchMPArray &P,
+// P, Q, R.
+hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1, Lengths mode, BatchMPArray &P,
                       BatchMPArray &Q, BatchMPArray &R) {
     const uint32_t n = N * 16u;
     const bool subtract = (n1 == n) || (n1 == (n >> 4));
@@ -438,35 +461,57 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     const BatchMPArray Qo = view_of(Q.data + Q.stride, Q.length, half, Q.stride * 2);
     const BatchMPArray Re = view_of(R.data, R.length, half, R.stride * 2);
     const BatchMPArray Ro = view_of(R.data + R.stride, R.length, half, R.stride * 2);
-    // P = P_e P_o (the last group's P is never used once rows are whole
-    // groups) and P*R = P_e R_o share P_e's transform
-    // product and R rows 16-byte aligned, except in the batched leaf forms,
-    // whose buffers keep the reference's preallocated sizes (and may be
-    // frozen for graph capture)
+    // product and R rows 16-byte aligned in the batched pipeline (the
+    // NTT's carry pass and the shift-add take four limbs per lane); the
+    // other forms keep the reference's stride = length (the group stage's
+    // buffers keep its preallocated sizes and may be frozen for capture)
     auto out_view = [&](Buf &b, uint32_t batch, uint32_t length, BatchMPArray &v) {
-        return compacting ? b.view_aligned(batch, length, v) : b.view(batch, length, v);
+        return mode == Lengths::kShared ? b.view_aligned(batch, length, v) : b.view(batch, length, v);
     };
-    GPUPI_RETURN_IF(out_view(t.bPn, half - ignore_last, P.length * 2, Pn));
-    GPUPI_RETURN_IF(out_view(t.bR2, half, R.length + P.length, R2));
-    if (Po.batch_size > 0) {
-        GPUPI_RETURN_IF(p_mul_pair(ctx, "P,P*R", "P", "P*R", n1, true, Po, Ro, Pe, Pn, R2));
-    } else {
-        GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, Pe, Ro, R2));
-    }
-    // Q = Q_e Q_o and R*Q = R_e Q_o share Q_o's transform
-    GPUPI_RETURN_IF(out_view(t.bQn, half, Q.length * 2, Qn));
-    GPUPI_RETURN_IF(out_view(t.bR1, half, R.length + Q.length, R1));
-    GPUPI_RETURN_IF(p_mul_pair(ctx, "Q,R*Q", "Q", "R*Q", n1, false, Qe, Re, Qo, Qn, R1));
-    // R(a,c) = R(a,b) Q(b,c) 2^(15 terms(b,c)) +- P(a,b) R(b,c)
     const uint32_t terms_right = (n / n1) + ((n / n1) >> 4);
     const uint32_t shift = 15u * terms_right;
+    if (mode == Lengths::kPerOp) {
+        GPUPI_RETURN_IF(out_view(t.bPn, half - ignore_last, P.length * 2, Pn));
+        if (Po.batch_size > 0) {
+            GPUPI_RETURN_IF(p_mul(ctx, "P", n1, Pe, Po, Pn));
+            GPUPI_RETURN_IF(compact(ctx, Pn, "P", n1));
+        } else {
+            Pn.length = 0;  // the top level: P is not used again
+        }
+        GPUPI_RETURN_IF(out_view(t.bQn, half, Q.length * 2, Qn));
+        GPUPI_RETURN_IF(p_mul(ctx, "Q", n1, Qe, Qo, Qn));
+        GPUPI_RETURN_IF(compact(ctx, Qn, "Q", n1));
+        GPUPI_RETURN_IF(out_view(t.bR1, half, R.length + Q.length, R1));
+        GPUPI_RETURN_IF(p_mul(ctx, "R*Q", n1, Re, Qo, R1));
+        GPUPI_RETURN_IF(out_view(t.bR2, half, R.length + P.length, R2));
+        GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, Pe, Ro, R2));
+    } else {
+        // P = P_e P_o (the last group's P is never used once rows are
+        // whole groups) and P*R = P_e R_o share P_e's transform
+        GPUPI_RETURN_IF(out_view(t.bPn, half - ignore_last, P.length * 2, Pn));
+        GPUPI_RETURN_IF(out_view(t.bR2, half, R.length + P.length, R2));
+        if (Po.batch_size > 0) {
+            GPUPI_RETURN_IF(p_mul_pair(ctx, "P,P*R", "P", "P*R", n1, true, Po, Ro, Pe, Pn, R2));
+        } else {
+            GPUPI_RETURN_IF(p_mul(ctx, "P*R", n1, Pe, Ro, R2));
+        }
+        // Q = Q_e Q_o and R*Q = R_e Q_o share Q_o's transform
+        GPUPI_RETURN_IF(out_view(t.bQn, half, Q.length * 2, Qn));
+        GPUPI_RETURN_IF(out_view(t.bR1, half, R.length + Q.length, R1));
+        GPUPI_RETURN_IF(p_mul_pair(ctx, "Q,R*Q", "Q", "R*Q", n1, false, Qe, Re, Qo, Qn, R1));
+    }
+    // R(a,c) = R(a,b) Q(b,c) 2^(15 terms(b,c)) +- P(a,b) R(b,c)
     const uint32_t shifted_len = R1.length + (shift >> 5) + ((shift & 31u) != 0);
     const uint32_t rlen = std::max(shifted_len, R2.length) + (subtract ? 0u : 1u);
     GPUPI_RETURN_IF(out_view(t.bRn, half, rlen, Rn));
     GPUPI_RETURN_IF(p_shift_addsub(ctx, R1, R2, shift, subtract, n1, Rn));
-    // the level's three compactions in one round trip (R is formed from the
-    // products' bound lengths, as it would be after compacting P and Q)
-    if (compacting) GPUPI_RETURN_IF(compact_many(ctx, {&Pn, &Qn, &Rn}, "P,Q,R", n1));
+    if (mode == Lengths::kPerOp) {
+        GPUPI_RETURN_IF(compact(ctx, Rn, "R", n1));
+    } else if (mode == Lengths::kShared) {
+        // the level's three compactions in one round trip (R is formed from
+        // the products' bound lengths, as it would be after compacting P, Q)
+        GPUPI_RETURN_IF(compact_many(ctx, {&Pn, &Qn, &Rn}, "P,Q,R", n1));
+    }
     t.bP.swap(t.bPn);
     t.bQ.swap(t.bQn);
     t.bR.swap(t.bRn);
@@ -478,11 +523,12 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
 
 // The head term 17g+16 of each group appended to its 16-term body, then the
 // block's common factor divided out (pi_bs_gpu.cu:725-779).
-hipError_t head_merge(BatchMPContext *ctx, TreeBufs &t, uint32_t n1, bool compacting, const BatchMPArray &P0,
+hipError_t head_merge(BatchMPContext *ctx, TreeBufs &t, uint32_t n1, Lengths mode, const BatchMPArray &P0,
                       const BatchMPArray &Q0, const BatchMPArray &R0, BatchMPArray &P, BatchMPArray &Q,
                       BatchMPArray &R) {
     const uint32_t div_lo = 830297u, div_hi = 156590819u;  // 13^2 17^3, 7^6 11^3 (pi_bs_gpu.cu:151-152)
     const uint32_t half = P.batch_size;
+    const bool compacting = mode != Lengths::kBound;
     BatchMPArray Ph, Qh, Rh, Rp1, Rp2;
     GPUPI_RETURN_IF(t.bPn.view(half, P.length + P0.length, Ph));
     GPUPI_RETURN_IF(p_mul(ctx, "P", n1, P, P0, Ph));
@@ -518,11 +564,11 @@ hipError_t head_merge(BatchMPContext *ctx, TreeBufs &t, uint32_t n1, bool compac
 
 // The 17-term group stage (pi_bs_gpu.cu:555-779): leaf terms, the four
 // pairwise levels of each 16-term body and the head merge; one row of P, Q,
-// R per group.  compacting = true is the reference's per-op form (a
-// synchronizing compaction after each product); false keeps the allocation
-// bounds as lengths (at most 51, 68 and 83 limbs at the end) and launches
-// everything without a host round trip.
-hipError_t leaf_stage(BatchMPContext *ctx, TreeBufs &t, uint32_t N, bool compacting, BatchMPArray &P,
+// R per group.  kPerOp is the reference's form (a synchronizing compaction
+// after each product it compacts); kBound keeps the allocation bounds as
+// lengths (at most 51, 68 and 83 limbs at the end) and launches everything
+// without a host round trip.
+hipError_t leaf_stage(BatchMPContext *ctx, TreeBufs &t, uint32_t N, Lengths mode, BatchMPArray &P,
                       BatchMPArray &Q, BatchMPArray &R) {
     const uint32_t n = N * 16u;
     BatchMPArray P0, Q0, R0;
@@ -542,8 +588,8 @@ hipError_t leaf_stage(BatchMPContext *ctx, TreeBufs &t, uint32_t N, bool compact
                           [&] { return launch_leaf(ctx, 1, P0.data, Q0.data, R0.data, N); }));
     GPUPI_RETURN_IF(stage(ctx, "leaf", "construct_leaf", N * 17u, n, P.length, 0, false,
                           [&] { return launch_leaf(ctx, 0, P.data, Q.data, R.data, n); }));
-    for (uint32_t n1 = n; n1 >= 2 * N; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, compacting, P, Q, R));
-    return head_merge(ctx, t, 2 * N, compacting, P0, Q0, R0, P, Q, R);
+    for (uint32_t n1 = n; n1 >= 2 * N; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, mode, P, Q, R));
+    return head_merge(ctx, t, 2 * N, mode, P0, Q0, R0, P, Q, R);
 }
 
 // The same stage as one fused kernel (rows of the bound lengths 51, 68, 83,
@@ -604,11 +650,11 @@ hipError_t with_stage_context(GpupiLeafStage *s, F &&fn) {
 hipError_t leaf_stage_capture(GpupiLeafStage *s) {
     BatchMPArray P, Q, R;
     // two eager runs: every ping-pong buffer reaches the size of both roles
-    for (int i = 0; i < 2; ++i) GPUPI_RETURN_IF(leaf_stage(s->ctx, s->t, s->N, false, P, Q, R));
+    for (int i = 0; i < 2; ++i) GPUPI_RETURN_IF(leaf_stage(s->ctx, s->t, s->N, Lengths::kBound, P, Q, R));
     GPUPI_RETURN_IF(hipStreamSynchronize(s->stream));
     s->t.freeze(true);
     GPUPI_RETURN_IF(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    const hipError_t body = leaf_stage(s->ctx, s->t, s->N, false, P, Q, R);
+    const hipError_t body = leaf_stage(s->ctx, s->t, s->N, Lengths::kBound, P, Q, R);
     hipGraph_t graph = nullptr;
     const hipError_t end = hipStreamEndCapture(s->stream, &graph);
     s->t.freeze(false);
@@ -637,11 +683,18 @@ hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t N, BatchMPArray *Q_o
     if (ctx == nullptr || Q_out == nullptr || R_out == nullptr || N == 0 || (N & (N - 1)) != 0) return hipErrorInvalidValue;
     TreeBufs t(ctx);
     BatchMPArray P, Q, R;
-    // the 17-term group stage with bound lengths (no synchronization inside),
-    // then one compaction of each array
-    GPUPI_RETURN_IF(leaf_stage_fused(ctx, t, N, P, Q, R));
-    GPUPI_RETURN_IF(compact_many(ctx, {&P, &Q, &R}, "P,Q,R", 2 * N));
-    for (uint32_t n1 = N; n1 > 1; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, true, P, Q, R));
+    if (ctx->pipeline_form == GPUPI_FORM_PLAIN) {
+        // the reference's form: the group stage op by op, every level's
+        // compactions one by one
+        GPUPI_RETURN_IF(leaf_stage(ctx, t, N, Lengths::kPerOp, P, Q, R));
+        for (uint32_t n1 = N; n1 > 1; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, Lengths::kPerOp, P, Q, R));
+    } else {
+        // the 17-term group stage with bound lengths (no synchronization
+        // inside), then one compaction of each array
+        GPUPI_RETURN_IF(leaf_stage_fused(ctx, t, N, P, Q, R));
+        GPUPI_RETURN_IF(compact_many(ctx, {&P, &Q, &R}, "P,Q,R", 2 * N));
+        for (uint32_t n1 = N; n1 > 1; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, Lengths::kShared, P, Q, R));
+    }
     // Q and R now hold one row each; hand their buffers to the caller.
     *Q_out = view_of(t.bQ.release(), Q.length, 1, Q.stride);
     *R_out = view_of(t.bR.release(), R.length, 1, R.stride);
@@ -659,26 +712,35 @@ hipError_t gpupi_sqrt_10005(BatchMPContext *ctx, int target_digits, BatchMPArray
     GPUPI_RETURN_IF(hipMemcpyAsync(P.data, &init[0], 4, hipMemcpyHostToDevice, ctx->stream));
     GPUPI_RETURN_IF(hipMemcpyAsync(Q.data, &init[1], 4, hipMemcpyHostToDevice, ctx->stream));
     GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+    const bool plain = ctx->pipeline_form == GPUPI_FORM_PLAIN;
     double valid_digits = 7.5;
     while (valid_digits < target_digits) {
         valid_digits *= 2;
         const uint32_t sd = (uint32_t)valid_digits;
         BatchMPArray PQ, Qn, PP, QQ, QQ5, Pn;
-        // intermediate results keep their bound lengths (their top limbs may
-        // be zero); only the new P and Q are compacted, in one round trip
+        // batched form: intermediate results keep their bound lengths (their
+        // top limbs may be zero); only the new P and Q are compacted, in one
+        // round trip.  Plain form: every result compacted as the reference
+        // does (pi_bs_gpu.cu:1443-1515).
         GPUPI_RETURN_IF(bPQ.view(1, P.length + Q.length, PQ));
         GPUPI_RETURN_IF(p_mul(ctx, "P*Q", sd, P, Q, PQ));
+        if (plain) GPUPI_RETURN_IF(compact(ctx, PQ, "P*Q", sd));
         GPUPI_RETURN_IF(bQn.view(1, PQ.length + 1, Qn));
         GPUPI_RETURN_IF(p_shift_bits(ctx, "2*P*Q", sd, PQ, Qn, 1));
+        if (plain) GPUPI_RETURN_IF(compact(ctx, Qn, "Q", sd));
         GPUPI_RETURN_IF(bPP.view(1, P.length * 2, PP));
         GPUPI_RETURN_IF(p_mul(ctx, "P*P", sd, P, P, PP));
+        if (plain) GPUPI_RETURN_IF(compact(ctx, PP, "P*P", sd));
         GPUPI_RETURN_IF(bQQ.view(1, Q.length * 2, QQ));
         GPUPI_RETURN_IF(p_mul(ctx, "Q*Q", sd, Q, Q, QQ));
+        if (plain) GPUPI_RETURN_IF(compact(ctx, QQ, "Q*Q", sd));
         GPUPI_RETURN_IF(bQQ5.view(1, QQ.length + 1, QQ5));
         GPUPI_RETURN_IF(p_mul_small(ctx, "10005*Q*Q", sd, QQ, 10005u, QQ5));
+        if (plain) GPUPI_RETURN_IF(compact(ctx, QQ5, "10005*Q*Q", sd));
         GPUPI_RETURN_IF(bPn.view(1, std::max(PP.length, QQ5.length) + 1, Pn));
         GPUPI_RETURN_IF(p_add(ctx, "P*P+10005*Q*Q", sd, PP, QQ5, Pn));
-        GPUPI_RETURN_IF(compact_many(ctx, {&Pn, &Qn}, "P,Q", sd));
+        if (plain) GPUPI_RETURN_IF(compact(ctx, Pn, "P", sd));
+        else GPUPI_RETURN_IF(compact_many(ctx, {&Pn, &Qn}, "P,Q", sd));
         bP.swap(bPn);
         bQ.swap(bQn);
         P = Pn;
@@ -788,10 +850,8 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
     BatchMPArray Q_bs{}, R_bs{};
     GPUPI_RETURN_IF(gpupi_binary_split(ctx, (uint32_t)N, &Q_bs, &R_bs));
     Buf bQbs(ctx), bRbs(ctx);
-    bQbs.ptr = Q_bs.data;
-    bQbs.cap = Q_bs.stride;
-    bRbs.ptr = R_bs.data;
-    bRbs.cap = R_bs.stride;
+    bQbs.adopt(Q_bs.data, Q_bs.stride);
+    bRbs.adopt(R_bs.data, R_bs.stride);
     const unsigned long long r_total_shift = 15ull * (unsigned long long)N * 17ull - 8ull;
     const int rs_bits = (int)(r_total_shift % 32), rs_limbs = (int)(r_total_shift / 32);
     if ((uint32_t)rs_limbs >= R_bs.length) return hipErrorInvalidValue;
@@ -811,10 +871,8 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
     BatchMPArray P15{}, Q15{};
     GPUPI_RETURN_IF(gpupi_sqrt_10005(ctx, target_digits, &P15, &Q15));
     Buf bP15(ctx), bQ15(ctx);
-    bP15.ptr = P15.data;
-    bP15.cap = P15.stride;
-    bQ15.ptr = Q15.data;
-    bQ15.cap = Q15.stride;
+    bP15.adopt(P15.data, P15.stride);
+    bQ15.adopt(Q15.data, Q15.stride);
     Buf bPf(ctx), bQf(ctx);
     BatchMPArray P_final, Q_final;
     GPUPI_RETURN_IF(bPf.view(1, P15.length + Q_tr.length, P_final));
@@ -839,8 +897,7 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
     BatchMPArray inv{};
     GPUPI_RETURN_IF(gpupi_recursive_inverse(ctx, Q_tf, inverse_bits, &inv));
     Buf binv(ctx);
-    binv.ptr = inv.data;
-    binv.cap = inv.stride;
+    binv.adopt(inv.data, inv.stride);
     Buf bprod(ctx);
     BatchMPArray prod;
     GPUPI_RETURN_IF(bprod.view(1, P_tf.length + inv.length, prod));
@@ -1120,6 +1177,12 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
     return hipSuccess;
 }
 
+hipError_t gpupi_set_pipeline_form(BatchMPContext *ctx, int form) {
+    if (ctx == nullptr || (form != GPUPI_FORM_BATCHED && form != GPUPI_FORM_PLAIN)) return hipErrorInvalidValue;
+    ctx->pipeline_form = form;
+    return hipSuccess;
+}
+
 hipError_t gpupi_stage_profile_enable(BatchMPContext *ctx, int enable) {
     if (ctx == nullptr) return hipErrorInvalidValue;
     if (ctx->profiler == nullptr) {
@@ -1184,7 +1247,7 @@ hipError_t gpupi_leaf_stage_run(GpupiLeafStage *s, BatchMPArray *P, BatchMPArray
     } else {
         const bool plain = s->mode == GPUPI_LEAF_PLAIN;
         if (s->mode == GPUPI_LEAF_FUSED) GPUPI_RETURN_IF(leaf_stage_fused(ctx, s->t, s->N, p, q, r));
-        else GPUPI_RETURN_IF(leaf_stage(ctx, s->t, s->N, plain, p, q, r));
+        else GPUPI_RETURN_IF(leaf_stage(ctx, s->t, s->N, plain ? Lengths::kPerOp : Lengths::kBound, p, q, r));
         if (!plain) {
             GPUPI_RETURN_IF(compact(ctx, p, "P", 2 * s->N));
             GPUPI_RETURN_IF(compact(ctx, q, "Q", 2 * s->N));

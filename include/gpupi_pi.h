@@ -46,6 +46,21 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
                         GpupiPiResult *result);
 void gpupi_pi_free(GpupiPiResult *result);
 
+/* How the pipeline issues its work (no reference counterpart).
+ *   GPUPI_FORM_BATCHED (default): the 17-term group stage as one fused
+ *     kernel; products sharing an operand transform it once; the P, Q, R
+ *     compactions of a tree level share one host round trip; buffers come
+ *     from the context's block cache.
+ *   GPUPI_FORM_PLAIN: the reference's form (pi_bs_gpu.cu:503-792,
+ *     :1346-1522, convert_decimal.cu:361-412): one launch sequence per
+ *     operation, a synchronizing compaction after every product and sum the
+ *     reference compacts, the group stage op by op, and buffers allocated
+ *     with hipMalloc and freed with hipFree.
+ * Both forms compute the same integers (RET limbs and digits are
+ * identical); only the time differs. */
+enum { GPUPI_FORM_BATCHED = 0, GPUPI_FORM_PLAIN = 1 };
+hipError_t gpupi_set_pipeline_form(BatchMPContext *ctx, int form);
+
 /* Stage entry points (batch-1 outputs are created arrays; release them). */
 hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t groups, BatchMPArray *Q, BatchMPArray *R);
 hipError_t gpupi_sqrt_10005(BatchMPContext *ctx, int target_digits, BatchMPArray *P, BatchMPArray *Q);

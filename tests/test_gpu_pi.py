@@ -214,3 +214,27 @@ def test_pi_1e7_matches_oracle(ctx):
     assert len(ret) == 1038127
     assert np.array_equal(ret, oret)
     assert dec == odec
+
+
+@pytest.mark.parametrize("digits", [0, 1000, 20000, 1000000])
+def test_plain_form_matches_oracle(digits):
+    # GPUPI_FORM_PLAIN: the reference's per-op form (group stage op by op,
+    # a compaction after every product it compacts, separate products for
+    # shared operands, hipMalloc/hipFree instead of the block cache) computes
+    # the same RET limbs and digits
+    with g.Context() as c:
+        c.set_form(g.FORM_PLAIN)
+        c.stage_profile(digits == 20000)
+        ret, dec, res = c.pi(digits)
+        entries = c.stage_entries()
+        assert c.cache_stats() == (0, 0), "the plain form took blocks from the cache"
+    oret, odec, ocd = of.pi_digits(digits)
+    assert res.convert_digits == ocd
+    assert np.array_equal(ret, oret)
+    assert dec == odec
+    if digits == 20000:
+        kinds = [e["kind"] for e in entries]
+        assert kinds.count("leaf") == 2 and "exactdiv" in kinds
+        # one compaction per P, Q and R of every tree level (group stage
+        # levels included), one per sqrt result
+        assert kinds.count("compact") > 3 * 20
