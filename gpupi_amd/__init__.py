@@ -111,6 +111,14 @@ class GpupiStageEntry(ctypes.Structure):
     ]
 
 
+class NTTPrecomputedTables(ctypes.Structure):
+    """include/batch_kernels.h: the NTT's device root tables."""
+    _fields_ = [
+        ("roots", ctypes.c_void_p),
+        ("max_log2", ctypes.c_uint32),
+    ]
+
+
 class ConvertDecimalPowerTable(ctypes.Structure):
     _fields_ = [
         ("powers", ctypes.POINTER(BatchMPArray)),
@@ -128,6 +136,7 @@ def _declare(lib):
     c_ctx = ctypes.c_void_p
     u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
     err = ctypes.c_int
+    vp = ctypes.c_void_p
     sig = {
         "batch_mp_init": (c_ctx, []),
         "batch_mp_destroy": (None, [c_ctx]),
@@ -169,6 +178,30 @@ def _declare(lib):
         "batch_decimal_naive": (None, [ctypes.c_void_p, ctypes.c_void_p, u32, ctypes.c_int, u32,
                                        ctypes.c_int, u32]),
         "batch_mul_ntt_workspace_size": (ctypes.c_size_t, [u32, u32, u32]),
+        # the per-op L1 layer (include/batch_kernels.h); pointers are device addresses
+        "batch_add_simple": (None, [vp, vp, vp, vp] + [u32] * 7),
+        "batch_sub_simple": (None, [vp, vp, vp, vp] + [u32] * 7),
+        "batch_add_simple_workspace_size": (ctypes.c_size_t, [u32] * 4),
+        "batch_sub_simple_workspace_size": (ctypes.c_size_t, [u32] * 4),
+        "batch_add_small": (None, [vp, u32, vp, vp] + [u32] * 5),
+        "batch_sub_small": (None, [vp, u32, vp, vp] + [u32] * 5),
+        "batch_mul_small": (None, [vp, u32, vp, vp] + [u32] * 5),
+        "batch_add_small_workspace_size": (ctypes.c_size_t, [u32] * 3),
+        "batch_sub_small_workspace_size": (ctypes.c_size_t, [u32] * 3),
+        "batch_mul_small_workspace_size": (ctypes.c_size_t, [u32] * 3),
+        "batch_shift_add_simple": (None, [vp, vp, vp, vp] + [u32] * 8),
+        "batch_shift_sub_simple": (None, [vp, vp, vp, vp] + [u32] * 8),
+        "batch_shift_add_simple_workspace_size": (ctypes.c_size_t, [u32] * 5),
+        "batch_shift_sub_simple_workspace_size": (ctypes.c_size_t, [u32] * 5),
+        "batch_shift_bits": (None, [vp, vp] + [u32] * 5 + [i32]),
+        "batch_exactdiv_small": (None, [vp, u32, u32, u32, u32]),
+        "batch_bitlength_max": (u32, [vp, vp, u32, u32, u32]),
+        "batch_limblength_max": (u32, [vp, vp, u32, u32, u32]),
+        "batch_bitlength_workspace_size": (ctypes.c_size_t, [u32, u32]),
+        "batch_mul_naive": (None, [vp, vp, vp] + [ctypes.c_int] * 6),
+        "init_ntt_precomputed_tables": (None, [ctypes.POINTER(NTTPrecomputedTables)]),
+        "release_ntt_precomputed_tables": (None, [ctypes.POINTER(NTTPrecomputedTables)]),
+        "batch_mul_ntt": (None, [vp, vp, vp, vp, NTTPrecomputedTables] + [u32] * 6),
         "gpupi_kernel_stats_reset": (None, [ctypes.c_int]),
         "gpupi_kernel_stats_query": (ctypes.c_int, [ctypes.POINTER(GpupiKernelStat), ctypes.c_int]),
         "gpupi_kernel_stats_query_sizes": (ctypes.c_int, [ctypes.POINTER(GpupiKernelStat), ctypes.c_int]),
@@ -462,6 +495,24 @@ class LeafStage:
         if self.ptr:
             self._lib.gpupi_leaf_stage_destroy(self.ptr)
             self.ptr = ctypes.c_void_p()
+
+
+def ret_mod_hash(limbs: np.ndarray) -> int:
+    """RET hash of the reference CLI (pi_bs_gpu.cu:852-858): the limbs read as
+    one base-2^32 number, modulo 1e9+7.  Evaluated as a pairwise tree (each
+    round folds limb pairs a + b*B and squares B), vectorised in uint64:
+    every operand stays below 2^30, so no product overflows."""
+    mod = 1000000007
+    v = np.asarray(limbs, dtype=np.uint64) % mod
+    base = (1 << 32) % mod
+    if v.size == 0:
+        return 0
+    while v.size > 1:
+        if v.size & 1:
+            v = np.append(v, np.uint64(0))
+        v = (v[0::2] + (v[1::2] * np.uint64(base)) % mod) % mod
+        base = base * base % mod
+    return int(v[0])
 
 
 def rows_of(arr: BatchMPArray) -> np.ndarray:

@@ -473,7 +473,11 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     if (mode == Lengths::kPerOp) {
         GPUPI_RETURN_IF(out_view(t.bPn, half - ignore_last, P.length * 2, Pn));
         if (Po.batch_size > 0) {
-            GPUPI_RETURN_IF(p_mul(ctx, "P", n1, Pe, Po, Pn));
+            // P_e without the last row when the last group's P is unused
+            // (pi_bs_gpu.cu:645-650)
+            BatchMPArray Pe_p = Pe;
+            Pe_p.batch_size = Po.batch_size;
+            GPUPI_RETURN_IF(p_mul(ctx, "P", n1, Pe_p, Po, Pn));
             GPUPI_RETURN_IF(compact(ctx, Pn, "P", n1));
         } else {
             Pn.length = 0;  // the top level: P is not used again
