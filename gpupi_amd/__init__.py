@@ -146,6 +146,7 @@ def _declare(lib):
         "batch_mp_ensure_workspace": (err, [c_ctx, ctypes.c_size_t]),
         "batch_mp_mul": (err, [c_ctx, A, A, A]),
         "batch_mp_mul_shared": (err, [c_ctx, A, A, A, A, A]),
+        "batch_mp_mul_noalloc": (err, [c_ctx, A, A, A]),
         "batch_mp_mul_small": (err, [c_ctx, A, u32, A]),
         "batch_mp_exactdiv_small": (err, [c_ctx, A, u32]),
         "batch_mp_add": (err, [c_ctx, A, A, A]),
@@ -255,6 +256,7 @@ def hip():
         _hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
         _hip.hipGetErrorString.restype = ctypes.c_char_p
         _hip.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        _hip.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     return _hip
 
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include <new>
@@ -87,6 +88,35 @@ hipError_t length_max(BatchMPContext *ctx, const BatchMPArray &A, bool bits, uin
 
 // Default schoolbook -> NTT crossover on L_a + L_b (tuned by measurement).
 constexpr int kDefaultNttMinLimbs = 384;
+
+// Scratch of the allocation-free multiply (batch_mp_mul_noalloc): the NTT's
+// residue planes, spill words and carry flags.  Each device holds its own
+// copy from module load to process end.  Host code never passes `scratch`
+// itself to a kernel (on the host it is a stand-in, not a device address):
+// it uses the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) uint8_t scratch[GPUPI_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+
+// This device's scratch address, looked up once per device (with that
+// device current) and kept.
+hipError_t scratch_of_current_device(void **out) {
+    int dev = 0;
+    GPUPI_RETURN_IF(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess) return err;
+        if (p == nullptr) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
 
 }  // namespace
 
@@ -212,7 +242,8 @@ BatchMPContext *batch_mp_init(void) {
         batch_mp_destroy(ctx);
         return nullptr;
     }
-    if (hipDeviceSynchronize() != hipSuccess) {
+    void *scratch_ptr = nullptr;
+    if (hipDeviceSynchronize() != hipSuccess || scratch_of_current_device(&scratch_ptr) != hipSuccess) {
         batch_mp_destroy(ctx);
         return nullptr;
     }
@@ -270,6 +301,30 @@ hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, Bat
     return gpupi::launch_mul_ntt(A.data, B.data, C.data, ctx->workspace, ctx->workspace_size_bytes,
                                  ctx->ntt_tables, C.batch_size, A.length, B.length, A.stride, B.stride,
                                  C.stride, ctx->stream);
+}
+
+// batch_mp_mul without allocating: the NTT's scratch is this device's static
+// array, rows run in chunks that fit it; the context's workspace and cache
+// are not touched.
+hipError_t batch_mp_mul_noalloc(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, BatchMPArray C) {
+    if (ctx == nullptr || !valid_array(A) || !valid_array(B) || !valid_output(C)) return hipErrorInvalidValue;
+    if (!same_batch(A, B) || !same_batch(A, C)) return hipErrorInvalidValue;
+    if ((uint64_t)C.length != (uint64_t)A.length + B.length) return hipErrorInvalidValue;
+    if (C.batch_size > 0 && (C.data == A.data || C.data == B.data)) return hipErrorInvalidValue;
+    if (C.batch_size == 0 || C.length == 0) return hipSuccess;
+    const uint32_t total = A.length + B.length;
+    const bool small = A.length == 0 || B.length == 0 ||
+                       ((int64_t)total < ctx->ntt_min_limbs && gpupi::mul_naive_supports(A.length, B.length));
+    if (small || !gpupi::ntt_supports(A.length, B.length)) {
+        if (!gpupi::mul_naive_supports(A.length, B.length)) return hipErrorInvalidValue;
+        return gpupi::launch_mul_naive(A.data, B.data, C.data, C.batch_size, A.length, B.length, A.stride,
+                                       B.stride, C.stride, ctx->stream);
+    }
+    void *ws = nullptr;
+    GPUPI_RETURN_IF(scratch_of_current_device(&ws));
+    return gpupi::launch_mul_ntt_bounded(A.data, B.data, C.data, ws, GPUPI_NOALLOC_SCRATCH_BYTES, ctx->ntt_tables,
+                                         C.batch_size, A.length, B.length, A.stride, B.stride, C.stride,
+                                         ctx->stream);
 }
 
 hipError_t batch_mp_mul_shared(BatchMPContext *ctx, BatchMPArray A1, BatchMPArray A2, BatchMPArray B,

@@ -91,6 +91,13 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
                           uint32_t L_a, uint32_t L_b, uint32_t stride_A, uint32_t stride_B,
                           uint32_t stride_C, hipStream_t stream);
 size_t ntt_workspace_bytes(uint32_t N, uint32_t L_a, uint32_t L_b);
+// The same products with at most `workspace_bytes` of workspace: rows run in
+// chunks that fit it (hipErrorInvalidValue when one row does not); records
+// no kernel statistics (creates no events).
+hipError_t launch_mul_ntt_bounded(const uint32_t *A, const uint32_t *B, uint32_t *C, void *workspace,
+                                  size_t workspace_bytes, const NTTPrecomputedTables &tables, uint32_t N,
+                                  uint32_t L_a, uint32_t L_b, uint32_t stride_A, uint32_t stride_B,
+                                  uint32_t stride_C, hipStream_t stream);
 int ntt_transform_log2(uint32_t L_a, uint32_t L_b);
 // The fused 17-term group stage: N rows each of P (51 limbs), Q (68), R (83).
 hipError_t launch_group_stage(uint32_t N, uint32_t *P, uint32_t *Q, uint32_t *R, hipStream_t stream);

@@ -864,10 +864,14 @@ hipEvent_t stats_event() {
     return e;
 }
 
+// Set on a thread while it runs a launch sequence that must not create
+// events (the allocation-free multiply).
+thread_local bool t_stats_off = false;
+
 // Records `launch` between two events when stats are enabled.
 template <typename F>
 hipError_t timed(int kind, int log2K, double bytes, hipStream_t stream, F &&launch) {
-    if (!g_stats.enabled.load(std::memory_order_relaxed)) {
+    if (t_stats_off || !g_stats.enabled.load(std::memory_order_relaxed)) {
         launch();
         return hipGetLastError();
     }
@@ -1009,6 +1013,7 @@ struct NttJob {
     const GpupiMulItem *items = nullptr;  // ragged: device items
     const GpupiMulItem *host_items = nullptr;
     int k = 0;
+    size_t max_chunk = 0;  // rows per launch chunk at most (0: the default)
 };
 
 // Rows each operand needs (the most of any product using it; one for a
@@ -1066,6 +1071,7 @@ JobLayout layout_of(const NttJob &j, const JobShape &sh) {
     JobLayout l;
     const size_t K = size_t(1) << j.k;
     l.nc = chunk_products(sh.rows, j.k);
+    if (j.max_chunk != 0 && l.nc > j.max_chunk) l.nc = j.max_chunk;
     size_t off = 0;
     for (uint32_t o = 0; o < j.n_ops; ++o) {
         const size_t cap = sh.op_bcast[o] ? 1 : std::min<size_t>(l.nc, sh.op_rows[o]);
@@ -1348,6 +1354,30 @@ hipError_t launch_mul_ntt(const uint32_t *A, const uint32_t *B, uint32_t *C, voi
     if (!ntt_supports(L_a, L_b) || tables.roots == nullptr) return hipErrorInvalidValue;
     return run_ntt(uniform_job(A, B, C, N, L_a, L_b, stride_A, stride_B, stride_C), workspace, workspace_bytes,
                    tables, stream);
+}
+
+hipError_t launch_mul_ntt_bounded(const uint32_t *A, const uint32_t *B, uint32_t *C, void *workspace,
+                                  size_t workspace_bytes, const NTTPrecomputedTables &tables, uint32_t N,
+                                  uint32_t L_a, uint32_t L_b, uint32_t stride_A, uint32_t stride_B,
+                                  uint32_t stride_C, hipStream_t stream) {
+    if (N == 0) return hipSuccess;
+    if (!ntt_supports(L_a, L_b) || tables.roots == nullptr) return hipErrorInvalidValue;
+    NttJob j = uniform_job(A, B, C, N, L_a, L_b, stride_A, stride_B, stride_C);
+    if (make_plan(j.k).levels > 1) {
+        // the most rows per chunk whose planes, spills and flags fit
+        const JobShape sh = shape_of(j);
+        size_t nc = chunk_products(sh.rows, j.k);
+        for (;;) {
+            j.max_chunk = nc;
+            if (layout_of(j, sh).total <= workspace_bytes) break;
+            if (nc == 1) return hipErrorInvalidValue;  // one row does not fit
+            nc = (nc + 1) / 2;
+        }
+    }
+    t_stats_off = true;
+    const hipError_t err = run_ntt(j, workspace, workspace_bytes, tables, stream);
+    t_stats_off = false;
+    return err;
 }
 
 bool ntt_shared_supports(uint32_t L1, uint32_t L2, uint32_t Lb) {

@@ -127,6 +127,23 @@ hipError_t batch_mp_set_ntt_threshold(BatchMPContext *ctx, uint32_t min_total_li
 hipError_t batch_mp_mul_shared(BatchMPContext *ctx, BatchMPArray A1, BatchMPArray A2, BatchMPArray B,
                                BatchMPArray C1, BatchMPArray C2);
 
+/* Bytes of the static per-device scratch of batch_mp_mul_noalloc. */
+#define GPUPI_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+
+/* Extension with no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops): batch_mp_mul with the same arguments, validation and results, but
+ * no hipMalloc, hipFree or allocating library call.  NTT products use a
+ * static per-device scratch array of GPUPI_NOALLOC_SCRATCH_BYTES (looked up
+ * once per device by batch_mp_init or the first call) instead of the
+ * context's workspace, running their rows in chunks that fit it; a product
+ * whose single row needs more (two distinct operands: L_a + L_b above 2^21
+ * limbs) is rejected with hipErrorInvalidValue.  Every call on
+ * one device shares that device's scratch: calls that overlap on one device
+ * (two streams, two threads) race and must be serialized by the caller.
+ * Kernel statistics are not recorded for these calls. */
+hipError_t batch_mp_mul_noalloc(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, BatchMPArray C);
+
 #ifdef __cplusplus
 }
 
