@@ -172,6 +172,10 @@ def _declare(lib):
                                           ctypes.c_void_p, u32]),
         "gpupi_round_up_decimal_digits": (u32, [u32, ctypes.POINTER(u32)]),
         "gpupi_prec_limbs_for_digits": (u32, [u32]),
+        "gpupi_calib_mac_peak": (err, [ctypes.POINTER(ctypes.c_double)]),
+        "gpupi_calib_copy": (err, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+        "gpupi_device_info": (err, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
+                                    ctypes.c_int]),
         "convert_decimal_precompute_powers": (err, [c_ctx, ctypes.POINTER(ConvertDecimalPowerTable), u32, u32]),
         "convert_decimal_release_powers": (None, [ctypes.POINTER(ConvertDecimalPowerTable)]),
         "convert_decimal_equal_split": (err, [c_ctx, A, u32, u32, u32,
@@ -548,6 +552,21 @@ def download_rows(arr: BatchMPArray, buf: DeviceBuffer) -> np.ndarray:
     for i in range(arr.batch_size):
         out[i] = flat[i * stride: i * stride + arr.length]
     return out
+
+
+def device_info() -> dict:
+    """Compute units, peak engine clock (kHz) and architecture of the current device."""
+    cus, clk = ctypes.c_int(0), ctypes.c_int(0)
+    arch = ctypes.create_string_buffer(64)
+    check(lib().gpupi_device_info(ctypes.byref(cus), ctypes.byref(clk), arch, 64), "gpupi_device_info")
+    return {"compute_units": cus.value, "clock_khz": clk.value, "arch": arch.value.decode()}
+
+
+def mac_peak() -> float:
+    """Measured device peak of v_mad_u64_u32 multiply-adds per second."""
+    r = ctypes.c_double(0.0)
+    check(lib().gpupi_calib_mac_peak(ctypes.byref(r)), "gpupi_calib_mac_peak")
+    return r.value
 
 
 def kernel_stats_reset(enable: bool) -> None:

@@ -943,6 +943,18 @@ uint32_t gpupi_prec_limbs_for_digits(uint32_t digits) {
     return prec_bits / 32u + 1u;
 }
 
+hipError_t gpupi_device_info(int *compute_units, int *clock_khz, char *arch, int arch_len) {
+    if (compute_units == nullptr || clock_khz == nullptr) return hipErrorInvalidValue;
+    int dev = 0;
+    GPUPI_RETURN_IF(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    GPUPI_RETURN_IF(hipGetDeviceProperties(&prop, dev));
+    *compute_units = prop.multiProcessorCount;
+    *clock_khz = prop.clockRate;
+    if (arch != nullptr && arch_len > 0) std::snprintf(arch, (size_t)arch_len, "%s", prop.gcnArchName);
+    return hipSuccess;
+}
+
 }  // extern "C"
 
 // ========================================================= radix conversion

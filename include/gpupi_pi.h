@@ -165,6 +165,19 @@ hipError_t gpupi_cache_trim(BatchMPContext *ctx);
 uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits);
 uint32_t gpupi_prec_limbs_for_digits(uint32_t digits);
 
+/* Measurement helper (no reference counterpart): the current device's
+ * compute units, peak engine clock (kHz) and architecture name, for the
+ * integer-VALU roofline of the benchmark. */
+hipError_t gpupi_device_info(int *compute_units, int *clock_khz, char *arch, int arch_len);
+
+/* Calibration (no reference counterpart): the measured peak of back-to-back
+ * v_mad_u64_u32 (32x32->64 multiply-adds) over the whole device, the
+ * integer roofline of the schoolbook and group-stage kernels; and a plain
+ * 16-byte copy kernel of `bytes` (a multiple of 16) whose HBM counters scale
+ * the NTT kernels' FETCH_SIZE / WRITE_SIZE to bytes. */
+hipError_t gpupi_calib_mac_peak(double *macs_per_second);
+hipError_t gpupi_calib_copy(void *dst, const void *src, size_t bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

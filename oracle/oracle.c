@@ -25,6 +25,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -96,15 +97,18 @@ static void shl_words(uint32_t *dst, size_t n, const uint32_t *src, uint32_t len
 /* batch ops (batch_arith.cu:170-424 and the per-op headers)           */
 /* ------------------------------------------------------------------ */
 
-/* batch_arith.cu:170-202: C = A * B, C.length == A.length + B.length */
-int o_mul(OArray A, OArray B, OArray C) {
-    if (!valid_array(A) || !valid_array(B) || !valid_array(C)) return -1;
-    if (A.batch_size != B.batch_size || A.batch_size != C.batch_size) return -1;
-    if (C.length != A.length + B.length) return -1;
+/* Worker threads for the batched products (CPU-baseline runs only; the
+ * tests keep one thread).  Rows are independent, so the result does not
+ * depend on the thread count. */
+static int g_threads = 1;
+
+void o_set_threads(int threads) { g_threads = threads < 1 ? 1 : threads; }
+
+static void mul_rows(OArray A, OArray B, OArray C, uint32_t r0, uint32_t r1) {
     const size_t na = (A.length + 1) / 2, nb = (B.length + 1) / 2;
     mp_limb_t *a = xmalloc((na + 1) * 8), *b = xmalloc((nb + 1) * 8);
     mp_limb_t *r = xmalloc((na + nb + 1) * 8);
-    for (uint32_t i = 0; i < A.batch_size; ++i) {
+    for (uint32_t i = r0; i < r1; ++i) {
         uint32_t *c = row_ptr(C, i);
         if (na == 0 || nb == 0) {
             memset(c, 0, (size_t)C.length * 4);
@@ -117,6 +121,40 @@ int o_mul(OArray A, OArray B, OArray C) {
         store_words(c, C.length, r, na + nb);
     }
     free(a); free(b); free(r);
+}
+
+typedef struct {
+    OArray A, B, C;
+    uint32_t r0, r1;
+} MulTask;
+
+static void *mul_worker(void *p) {
+    MulTask *t = p;
+    mul_rows(t->A, t->B, t->C, t->r0, t->r1);
+    return NULL;
+}
+
+/* batch_arith.cu:170-202: C = A * B, C.length == A.length + B.length */
+int o_mul(OArray A, OArray B, OArray C) {
+    if (!valid_array(A) || !valid_array(B) || !valid_array(C)) return -1;
+    if (A.batch_size != B.batch_size || A.batch_size != C.batch_size) return -1;
+    if (C.length != A.length + B.length) return -1;
+    uint32_t T = (uint32_t)g_threads;
+    if (T > A.batch_size) T = A.batch_size;
+    if (T <= 1) {
+        mul_rows(A, B, C, 0, A.batch_size);
+        return 0;
+    }
+    pthread_t th[256];
+    MulTask task[256];
+    if (T > 256) T = 256;
+    for (uint32_t t = 0; t < T; ++t) {
+        task[t] = (MulTask){A, B, C, (uint32_t)((uint64_t)A.batch_size * t / T),
+                            (uint32_t)((uint64_t)A.batch_size * (t + 1) / T)};
+        if (t + 1 < T && pthread_create(&th[t], NULL, mul_worker, &task[t]) != 0) abort();
+    }
+    mul_rows(A, B, C, task[T - 1].r0, task[T - 1].r1);
+    for (uint32_t t = 0; t + 1 < T; ++t) pthread_join(th[t], NULL);
     return 0;
 }
 

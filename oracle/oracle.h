@@ -30,6 +30,8 @@ typedef struct {
  * Return 0 on success, -1 on invalid arguments (the reference's
  * cudaErrorInvalidValue cases), -2 if an exact division was not exact. */
 int o_mul(OArray A, OArray B, OArray C);
+/* threads for the rows of o_mul (default 1; CPU-baseline runs only) */
+void o_set_threads(int threads);
 int o_mul_small(OArray A, uint32_t b, OArray C);
 int o_exactdiv_small(OArray A, uint32_t b);
 int o_add(OArray A, OArray B, OArray C);

@@ -84,6 +84,8 @@ def lib():
         L.o_decimal_hash.argtypes = [ctypes.c_char_p, u32]
         L.o_decimal_hash.restype = u32
         L.o_free.argtypes = [ctypes.c_void_p]
+        L.o_set_threads.argtypes = [ctypes.c_int]
+        L.o_set_threads.restype = None
         _lib = L
     return _lib
 
@@ -160,6 +162,20 @@ def pi_digits(digits: int, readme_prec: bool = False):
     L.o_free(ctypes.cast(ret, ctypes.c_void_p))
     L.o_free(ctypes.cast(dig, ctypes.c_void_p))
     return limbs, raw.decode(), cd.value
+
+
+def set_threads(threads: int) -> None:
+    """Threads for the rows of the oracle's batched products (results do not
+    depend on it)."""
+    lib().o_set_threads(threads)
+
+
+def mul(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """One product A * B of two limb vectors (GMP mpn_mul; releases the GIL
+    while it runs, so several may run on threads)."""
+    hc = HostArray.empty(1, A.size + B.size)
+    assert lib().o_mul(HostArray(A.reshape(1, -1)).oarray(), HostArray(B.reshape(1, -1)).oarray(), hc.oarray()) == 0
+    return hc.buf[:A.size + B.size].copy()
 
 
 def mod_hash(limbs: np.ndarray) -> int:

@@ -29,7 +29,7 @@ def stage_rows(stage):
 
 
 @pytest.mark.parametrize("mode", list(MODES))
-@pytest.mark.parametrize("groups", [1, 4, 64, 4096])
+@pytest.mark.parametrize("groups", [1, 4, 64, 4096, pytest.param(65536, marks=pytest.mark.slow)])
 def test_leaf_stage_matches_oracle(ctx, groups, mode):
     want = oracle_stage(groups)
     st = g.LeafStage(ctx, groups, MODES[mode])

@@ -385,6 +385,27 @@ def test_ragged_one_tile_products_of_all_ones(ctx):
         assert np.array_equal(C[:-1], oracle_mul(A, B)[0]), (la, lb)
 
 
+@pytest.mark.parametrize("k", [4, 5, 6, 7, 8, 9, 10, 11, 12])
+def test_one_tile_products_of_all_ones_every_size(ctx, k):
+    # the one-tile kernel at every transform size 2^4..2^12 (with the
+    # crossover lowered, so that the small sizes reach it) on all-ones
+    # operands, whose coefficients are the largest a product of that size
+    # meets; uniform and ragged calls
+    g.check(ctx.set_ntt_threshold(2))
+    try:
+        K = 1 << k
+        lengths = [(K // 2, K // 2), (K - 1, 1), (K // 2 + 1, K // 2 - 1), (3 * K // 8, 3 * K // 8 + 1)]
+        err, out, As, Bs, _ = ragged_case(ctx, lengths, 9, "ones")
+        assert err == 0
+        for (la, lb), C, A, B in zip(lengths, out, As, Bs):
+            assert C[-1] == PAD, (la, lb)
+            assert np.array_equal(C[:-1], oracle_mul(A, B)[0]), (la, lb)
+        A = np.full((5, K // 2), 0xFFFFFFFF, dtype=np.uint32)
+        assert np.array_equal(device_mul(ctx, A, A.copy()), oracle_mul(A, A))
+    finally:
+        g.check(ctx.set_ntt_threshold(384))
+
+
 def test_ragged_matches_uniform_call(ctx):
     # the same pairs through batch_mp_mul one by one
     lengths = [(3000, 2000), (777, 4000), (20000, 1), (9000, 9000)]

@@ -9,6 +9,7 @@ own known answers.
 - convert_decimal: the reference's four known-answer cases;
 - sizing formulas: the reference's published lengths."""
 import ctypes
+import os
 import math
 from fractions import Fraction
 
@@ -170,3 +171,37 @@ def test_sizing_matches_reference_published_lengths():
     # README.md:72 (rounded convention) and convert_decimal_test.cu:30 (requested)
     assert L.o_prec_limbs_for_digits(refcases.README_1E9["convert_digits"]) + 1 == refcases.README_1E9["ret_len"]
     assert L.o_prec_limbs_for_digits(10 ** 9) + 1 == 103812502
+
+
+def test_threaded_oracle_gives_the_same_results():
+    # the CPU baseline splits the rows of every batched product over
+    # threads; results must not depend on it
+    ret1, dec1, _ = of.pi_digits(30000)
+    of.set_threads(4)
+    try:
+        ret4, dec4, _ = of.pi_digits(30000)
+        rng = np.random.default_rng(5)
+        A = rng.integers(0, 1 << 32, size=(37, 300), dtype=np.uint64).astype(np.uint32)
+        B = rng.integers(0, 1 << 32, size=(37, 200), dtype=np.uint64).astype(np.uint32)
+        hc = of.HostArray.empty(37, 500)
+        assert of.lib().o_mul(of.HostArray(A).oarray(), of.HostArray(B).oarray(), hc.oarray()) == 0
+        got = hc.rows()
+    finally:
+        of.set_threads(1)
+    assert np.array_equal(ret1, ret4) and dec1 == dec4
+    for i in range(37):
+        assert int.from_bytes(got[i].tobytes(), "little") == (int.from_bytes(A[i].tobytes(), "little") *
+                                                              int.from_bytes(B[i].tobytes(), "little"))
+
+
+def test_classic_cpu_baseline_digits_match_the_oracle():
+    # oracle/pi_oracle --classic (the textbook GMP Chudnovsky of the CPU
+    # baseline) prints the same digits as the restated pipeline
+    import re
+    import subprocess
+    cli = os.path.join(os.path.dirname(of.__file__), "pi_oracle")
+    out = subprocess.run([cli, "20000", "--classic", "--threads", "3"], capture_output=True, text=True,
+                         check=True).stdout
+    _, dec, _ = of.pi_digits(20000)
+    assert f"DEC_hash = {of.lib().o_decimal_hash(dec.encode(), 20000)}" in out
+    assert re.search(r"DEC = 31415926535897932384\.\.\.", out)
