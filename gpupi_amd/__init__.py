@@ -224,6 +224,8 @@ def _declare(lib):
         "gpupi_server_results": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GpupiJobResult), ctypes.c_int,
                                                 ctypes.c_int]),
         "gpupi_server_drain": (None, [ctypes.c_void_p]),
+        "gpupi_server_memory": (err, [ctypes.c_void_p, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+        "gpupi_set_memory_pool": (err, [c_ctx, ctypes.c_void_p]),
         "gpupi_server_destroy": (None, [ctypes.c_void_p]),
         "gpupi_serve_job_bytes": (u64, [ctypes.c_int, u32, u32]),
         "gpupi_serve_admit": (ctypes.c_int, [ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int32), ctypes.c_int, u64,
@@ -471,6 +473,13 @@ class Server:
             else:
                 out.append((r, k.raw if k is not None else None))
         return out
+
+    def memory(self) -> tuple:
+        """(peak bytes handed out, peak bytes held) by the server's memory pool."""
+        used, reserved = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        check(self._lib.gpupi_server_memory(self.ptr, ctypes.byref(used), ctypes.byref(reserved)),
+              "gpupi_server_memory")
+        return used.value, reserved.value
 
     def drain(self) -> None:
         self._lib.gpupi_server_drain(self.ptr)

@@ -35,6 +35,18 @@ bool alias_ok(const BatchMPArray &x, const BatchMPArray &c) {
 
 hipError_t ensure(BatchMPContext *ctx, size_t bytes) {
     if (bytes <= ctx->workspace_size_bytes) return hipSuccess;
+    if (ctx->cache->pool() != nullptr) {
+        // stream order: the old workspace returns to the pool after the work
+        // queued on it, the new one is ready for the work queued next
+        if (ctx->workspace != nullptr) GPUPI_RETURN_IF(hipFreeAsync(ctx->workspace, ctx->stream));
+        ctx->workspace = nullptr;
+        ctx->workspace_size_bytes = 0;
+        void *fresh = nullptr;
+        GPUPI_RETURN_IF(hipMallocFromPoolAsync(&fresh, bytes, ctx->cache->pool(), ctx->stream));
+        ctx->workspace = static_cast<uint32_t *>(fresh);
+        ctx->workspace_size_bytes = bytes;
+        return hipSuccess;
+    }
     uint32_t *fresh = nullptr;
     GPUPI_RETURN_IF(hipMalloc(&fresh, bytes));
     if (ctx->workspace != nullptr) {
@@ -157,6 +169,13 @@ hipError_t compact_arrays(BatchMPContext *ctx, BatchMPArray *const *arrays, int 
 hipError_t BlockCache::acquire(hipStream_t stream, size_t bytes, void **ptr, size_t *got) {
     *ptr = nullptr;
     *got = 0;
+    if (pool_ != nullptr) {
+        const size_t want = round_block(bytes);
+        GPUPI_RETURN_IF(hipMallocFromPoolAsync(ptr, want, pool_, stream));
+        ++allocations_;
+        *got = want;
+        return hipSuccess;
+    }
     if (!bound_ || stream != stream_) {
         // blocks last used on another stream: let the device drain first
         if (!free_.empty()) GPUPI_RETURN_IF(hipDeviceSynchronize());
@@ -188,10 +207,25 @@ hipError_t BlockCache::acquire(hipStream_t stream, size_t bytes, void **ptr, siz
     return hipSuccess;
 }
 
-void BlockCache::give_back(void *ptr, size_t bytes) {
+void BlockCache::give_back(void *ptr, size_t bytes, hipStream_t stream) {
     if (ptr == nullptr) return;
+    if (pool_ != nullptr) {
+        (void)hipFreeAsync(ptr, stream);
+        return;
+    }
     free_.emplace(bytes, ptr);
     cached_ += bytes;
+}
+
+hipError_t ctx_alloc(BatchMPContext *ctx, void **ptr, size_t bytes) {
+    if (ctx->cache->pool() != nullptr) return hipMallocFromPoolAsync(ptr, bytes, ctx->cache->pool(), ctx->stream);
+    return hipMalloc(ptr, bytes);
+}
+
+void ctx_free(BatchMPContext *ctx, void *ptr) {
+    if (ptr == nullptr) return;
+    if (ctx->cache->pool() != nullptr) (void)hipFreeAsync(ptr, ctx->stream);
+    else (void)hipFree(ptr);
 }
 
 hipError_t BlockCache::trim() {
@@ -252,7 +286,14 @@ BatchMPContext *batch_mp_init(void) {
 
 void batch_mp_destroy(BatchMPContext *ctx) {
     if (ctx == nullptr) return;
-    if (ctx->workspace) (void)hipFree(ctx->workspace);
+    if (ctx->workspace) {
+        if (ctx->cache != nullptr && ctx->cache->pool() != nullptr) {
+            (void)hipFreeAsync(ctx->workspace, ctx->stream);
+            (void)hipStreamSynchronize(ctx->stream);
+        } else {
+            (void)hipFree(ctx->workspace);
+        }
+    }
     if (ctx->d_scalar) (void)hipFree(ctx->d_scalar);
     if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
     gpupi::ntt_tables_release(&ctx->ntt_tables);

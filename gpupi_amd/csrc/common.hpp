@@ -136,13 +136,21 @@ namespace gpupi {
 // the old user's.  A request from another stream first waits for the device.
 // Only trim() frees blocks (hipFree waits for the whole device, which would
 // stall the host between stages).
+//
+// With a memory pool attached (set_pool: the serving loop's one pool for all
+// its workers) blocks come from and go back to that pool in stream order
+// (hipMallocFromPoolAsync / hipFreeAsync): nothing is kept here and nothing
+// waits for the device; the pool reuses freed memory across streams.
 class BlockCache {
   public:
     // A block of at least `bytes` for work on `stream`; *got is its size.
     hipError_t acquire(hipStream_t stream, size_t bytes, void **ptr, size_t *got);
-    // Hands a block back; `bytes` may be less than its size, never more.
-    void give_back(void *ptr, size_t bytes);
+    // Hands a block back after the work queued on `stream`; `bytes` may be
+    // less than its size, never more.
+    void give_back(void *ptr, size_t bytes, hipStream_t stream);
     hipError_t trim();
+    void set_pool(hipMemPool_t pool) { pool_ = pool; }
+    hipMemPool_t pool() const { return pool_; }
     size_t cached_bytes() const { return cached_; }
     uint64_t allocations() const { return allocations_; }
     ~BlockCache() { (void)trim(); }
@@ -153,7 +161,13 @@ class BlockCache {
     bool bound_ = false;
     size_t cached_ = 0;
     uint64_t allocations_ = 0;
+    hipMemPool_t pool_ = nullptr;
 };
+
+// Device memory of a context: from its memory pool in stream order when one
+// is attached, else hipMalloc / hipFree.
+hipError_t ctx_alloc(BatchMPContext *ctx, void **ptr, size_t bytes);
+void ctx_free(BatchMPContext *ctx, void *ptr);
 
 constexpr int kScalarWords = 4;  // device words of BatchMPContext::d_scalar
 

@@ -88,10 +88,12 @@ struct StepMap {
     static constexpr uint32_t kStride = COL ? 1u << (LSS + kTileLog2 - M) : 1u << LSS;
     uint32_t base[PER];  // tile index of element j = 0 of each small DFT
     uint32_t n2[PER];
-    __device__ __forceinline__ StepMap() {
+    // tid: the thread's index within its tile's 256 (threadIdx.x, or its
+    // virtual index in a block of paired column tiles)
+    __device__ __forceinline__ explicit StepMap(uint32_t tid) {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const SmallDft d = small_dft(M, S, threadIdx.x + 256u * u, COL);
+            const SmallDft d = small_dft(M, S, tid + 256u * u, COL);
             base[u] = d.base;
             n2[u] = d.n2;
         }
@@ -154,15 +156,15 @@ __device__ __forceinline__ void lds_store(double *lds, const double *v, const St
 // Steps S..LAST (forward, increasing) or S..LAST (inverse, decreasing) as
 // LDS -> registers -> LDS passes, each followed by a barrier.
 template <int M, bool INV, bool COL, int S, int LAST>
-__device__ __forceinline__ void lds_steps(double *lds, const double *tab, int P, Mod m) {
+__device__ __forceinline__ void lds_steps(double *lds, const double *tab, int P, Mod m, uint32_t tid) {
     if constexpr (INV ? S >= LAST : S <= LAST) {
-        const StepMap<M, S, COL> sm;
+        const StepMap<M, S, COL> sm(tid);
         double v[16];
         lds_load(v, lds, sm);
         step_regs<M, S, INV, COL, INV ? inv_kind<M, S>() : fwd_kind<M, S, kDifMid>()>(v, sm, tab, P, m);
         lds_store(lds, v, sm);
         __syncthreads();
-        lds_steps<M, INV, COL, INV ? S - 1 : S + 1, LAST>(lds, tab, P, m);
+        lds_steps<M, INV, COL, INV ? S - 1 : S + 1, LAST>(lds, tab, P, m, tid);
     }
 }
 
@@ -170,14 +172,15 @@ __device__ __forceinline__ void lds_steps(double *lds, const double *tab, int P,
 // elements; on exit its last-step elements (bit-reversed DFT order).  LDS is
 // used when M > 4: the caller must make sure no thread still reads `lds`.
 template <int M, bool COL, int FINAL>
-__device__ __forceinline__ void fwd_dft(double *v, double *lds, const double *tab, int P, Mod m) {
+__device__ __forceinline__ void fwd_dft(double *v, double *lds, const double *tab, int P, Mod m,
+                                        uint32_t tid = threadIdx.x) {
     constexpr int kLast = step_count(M) - 1;
-    step_regs<M, 0, false, COL, fwd_kind<M, 0, FINAL>()>(v, StepMap<M, 0, COL>(), tab, P, m);
+    step_regs<M, 0, false, COL, fwd_kind<M, 0, FINAL>()>(v, StepMap<M, 0, COL>(tid), tab, P, m);
     if constexpr (kLast > 0) {
-        lds_store(lds, v, StepMap<M, 0, COL>());
+        lds_store(lds, v, StepMap<M, 0, COL>(tid));
         __syncthreads();
-        lds_steps<M, false, COL, 1, kLast - 1>(lds, tab, P, m);
-        const StepMap<M, kLast, COL> sl;
+        lds_steps<M, false, COL, 1, kLast - 1>(lds, tab, P, m, tid);
+        const StepMap<M, kLast, COL> sl(tid);
         lds_load(v, lds, sl);
         step_regs<M, kLast, false, COL, FINAL>(v, sl, tab, P, m);
     }
@@ -187,15 +190,16 @@ __device__ __forceinline__ void fwd_dft(double *v, double *lds, const double *ta
 // elements (as left by fwd_dft, so the first LDS store touches only slots
 // this thread read); on exit its step-0 elements.
 template <int M, bool COL>
-__device__ __forceinline__ void inv_dft(double *v, double *lds, const double *tab, int P, Mod m) {
+__device__ __forceinline__ void inv_dft(double *v, double *lds, const double *tab, int P, Mod m,
+                                        uint32_t tid = threadIdx.x) {
     constexpr int kLast = step_count(M) - 1;
-    const StepMap<M, kLast, COL> sl;
+    const StepMap<M, kLast, COL> sl(tid);
     step_regs<M, kLast, true, COL, inv_kind<M, kLast>()>(v, sl, tab, P, m);
     if constexpr (kLast > 0) {
         lds_store(lds, v, sl);
         __syncthreads();
-        lds_steps<M, true, COL, kLast - 1, 1>(lds, tab, P, m);
-        const StepMap<M, 0, COL> s0;
+        lds_steps<M, true, COL, kLast - 1, 1>(lds, tab, P, m, tid);
+        const StepMap<M, 0, COL> s0(tid);
         lds_load(v, lds, s0);
         step_regs<M, 0, true, COL, kDitLast>(v, s0, tab, P, m);
     }
@@ -303,8 +307,40 @@ __device__ __forceinline__ BlockPos block_pos(const uint32_t *start, uint32_t ns
     return BlockPos{seg, local / tiles, local % tiles};
 }
 
+// Column kernels of DFT size 2^M >= 2^7 (tiles of 32 or fewer adjacent
+// columns) can run two neighbouring tiles per block of 512 threads, so that
+// a wave's global accesses cover the 2C adjacent columns of both tiles (runs
+// of 2C elements instead of C): thread t works for tile half(t) as virtual
+// thread col_vtid(t), and each half has its own LDS tile.  On the first
+// level's access pattern at K = 2^28 a bare copy kernel moves limbs -> two
+// binary64 planes at 3.1 -> 4.1 TB/s and binary64 -> binary64 at 3.9 -> 5.2
+// TB/s with runs doubled, but in the NTT kernels (per 10^9 step) only the
+// first forward level gains (195 -> 191 ms) and the second inverse level a
+// little (64.4 -> 63.0 ms); the second forward level loses 2 ms and the
+// first inverse level 13 ms (barriers over eight waves instead of four), so
+// only those two pair their tiles.  Bit 0: first forward level, bit 1:
+// second forward level, bit 2: second inverse level, bit 3: first inverse
+// level.
+#ifndef GPUPI_COL_PAIR_KINDS
+#define GPUPI_COL_PAIR_KINDS 5
+#endif
+constexpr bool pair_kind(int bit) { return ((GPUPI_COL_PAIR_KINDS) >> bit) & 1; }
+constexpr int kPairFwdLimbs = 0, kPairFwd = 1, kPairInv = 2, kPairInvLimbs = 3;
+constexpr int pair_log2_for(int kind, int lm) { return pair_kind(kind) ? col_pair_log2(lm) : 0; }
+
+template <int M, int KIND>
+struct ColPair {
+    static constexpr int kLog2 = pair_log2_for(KIND, M);
+    static constexpr int kTiles = 1 << kLog2;
+    static constexpr int kThreads = 256 * kTiles;
+    static constexpr int LC = kTileLog2 - M;
+    __device__ __forceinline__ static uint32_t half() { return kLog2 ? col_pair_half(threadIdx.x, M) : 0u; }
+    __device__ __forceinline__ static uint32_t vtid() { return kLog2 ? col_pair_vtid(threadIdx.x, M) : threadIdx.x; }
+};
+
 // Per-thread geometry of a column tile (level `level`, DFT size 2^M): the
-// thread's elements tid + 256 u are exactly its step-0 elements.
+// thread (tile-local index tid) holds elements tid + 256 u, exactly its
+// step-0 elements.
 template <int M>
 struct ColThread {
     uint64_t off0;     // element offset of u = 0 within the product
@@ -312,17 +348,19 @@ struct ColThread {
     uint32_t cg;       // (col0 + g) << (30 - lk): twiddle exponent per bitrev(r)
     uint32_t b0;       // bitrev(r0, M)
     uint32_t lds0;     // LDS slot of u = 0
+    uint32_t tid;      // index within the tile's 256 threads
 
-    __device__ __forceinline__ ColThread(const Plan &p, int level, uint32_t t) {
+    __device__ __forceinline__ ColThread(const Plan &p, int level, uint32_t t, uint32_t tid_) {
         constexpr int LC = kTileLog2 - M;
+        tid = tid_;
         const ColTile ct = col_tile(p, level, t);
-        const uint32_t g = threadIdx.x & ((1u << LC) - 1u);
-        const uint32_t r0 = threadIdx.x >> LC;
-        off0 = col_thread_offset(p, level, ct, threadIdx.x);
+        const uint32_t g = tid & ((1u << LC) - 1u);
+        const uint32_t r0 = tid >> LC;
+        off0 = col_thread_offset(p, level, ct, tid);
         ustep = (uint32_t)col_row_step_log2(p, level);
         cg = (ct.col0 + g) << (kMaxLog2 - slab_log2(p, level));
         b0 = __brev(r0) >> (32 - M);
-        lds0 = lds_index(threadIdx.x);
+        lds0 = lds_index(tid);
     }
     __device__ __forceinline__ uint64_t off(int u) const { return off0 + ((uint64_t)u << ustep); }
 };
@@ -336,8 +374,8 @@ __device__ __forceinline__ void col_fwd_prime(double *v, double *lds, const ColT
                                               const ColTw &tw) {
     const Mod m = mod_of(P);
     const double *tab = tables + P * kPrimeTable;
-    fwd_dft<M, true, kDifToTwiddle>(v, lds, tab, P, m);
-    lds_store(lds, v, StepMap<M, step_count(M) - 1, true>());
+    fwd_dft<M, true, kDifToTwiddle>(v, lds, tab, P, m, ct.tid);
+    lds_store(lds, v, StepMap<M, step_count(M) - 1, true>(ct.tid));
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < 16; ++u) v[u] = lds[ct.lds0 + 272u * u];
@@ -354,12 +392,15 @@ __device__ __forceinline__ void col_fwd_prime(double *v, double *lds, const ColT
 // store.  Above level 0 the second prime's data is loaded after the first
 // prime's store, which keeps the kernel within 128 VGPRs.
 template <bool FROM_LIMBS, int M>
-__global__ void __launch_bounds__(256) ntt_col_fwd_kernel(PassArgs a, int level) {
-    __shared__ double lds[kLdsSlots];
-    const BlockPos bp = block_pos(a.op_start, a.n_ops, tiles_per_product(a.plan));
+__global__ void __launch_bounds__((ColPair<M, FROM_LIMBS ? kPairFwdLimbs : kPairFwd>::kThreads))
+    ntt_col_fwd_kernel(PassArgs a, int level) {
+    using CP = ColPair<M, FROM_LIMBS ? kPairFwdLimbs : kPairFwd>;
+    __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
+    double *const lds = lds_tiles + CP::half() * kLdsSlots;
+    const BlockPos bp = block_pos(a.op_start, a.n_ops, tiles_per_product(a.plan) >> CP::kLog2);
     const NttOp o = a.ops[bp.seg];
     const uint64_t K = 1ull << a.plan.log2K;
-    const ColThread<M> ct(a.plan, level, bp.t);
+    const ColThread<M> ct(a.plan, level, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
     double *T0 = a.T + o.base + (size_t)bp.row * K;
     double *T1 = T0 + o.pstride;
     double v0[16], v1[16];
@@ -521,8 +562,8 @@ __device__ __forceinline__ void col_inv_prime(double *v, double *lds, const ColT
 #pragma unroll
     for (int u = 0; u < 16; ++u) lds[ct.lds0 + 272u * u] = v[u];
     __syncthreads();
-    lds_steps<M, true, true, kLast, 1>(lds, tab, P, m);
-    const StepMap<M, 0, true> s0;
+    lds_steps<M, true, true, kLast, 1>(lds, tab, P, m, ct.tid);
+    const StepMap<M, 0, true> s0(ct.tid);
     lds_load(v, lds, s0);
     step_regs<M, 0, true, true, kDitLast>(v, s0, tab, P, m);
 }
@@ -543,11 +584,13 @@ __device__ __forceinline__ void col_inv_load(double *v0, double *v1, ColTw &tw0,
 // Inverse column level above the first: both primes through one tile,
 // stored together at the end.
 template <int M>
-__global__ void __launch_bounds__(256) ntt_col_inv_kernel(PassArgs a, int level) {
-    __shared__ double lds[kLdsSlots];
-    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan));
+__global__ void __launch_bounds__((ColPair<M, kPairInv>::kThreads)) ntt_col_inv_kernel(PassArgs a, int level) {
+    using CP = ColPair<M, kPairInv>;
+    __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
+    double *const lds = lds_tiles + CP::half() * kLdsSlots;
+    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan) >> CP::kLog2);
     const NttOp oa = a.ops[a.prods[bp.seg].a];
-    const ColThread<M> ct(a.plan, level, bp.t);
+    const ColThread<M> ct(a.plan, level, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
     double *T0 = a.T + oa.base + (size_t)bp.row * (1ull << a.plan.log2K);
     double *T1 = T0 + oa.pstride;
     // register budget (occupancy 4 needs <= 128 VGPRs): one prime's
@@ -577,18 +620,22 @@ __global__ void __launch_bounds__(256) ntt_col_inv_kernel(PassArgs a, int level)
 // one neighbour shift and one ballot carry resolution per run; what passes
 // the run's top goes to the spill word (the carry engine adds it).
 template <int M>
-__global__ void __launch_bounds__(256) ntt_col_inv_limbs_kernel(PassArgs a) {
-    __shared__ double lds[kLdsSlots];
+__global__ void __launch_bounds__((ColPair<M, kPairInvLimbs>::kThreads)) ntt_col_inv_limbs_kernel(PassArgs a) {
+    using CP = ColPair<M, kPairInvLimbs>;
+    __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
+    double *const lds = lds_tiles + CP::half() * kLdsSlots;
     constexpr int LC = kTileLog2 - M;
     constexpr int RL = run_log2_for(M);
     constexpr uint32_t kRun = 1u << RL;
+    // a run (2^RL lanes aligned to 2^RL) lies inside one group of 2^LC
+    // adjacent lanes, which hold one row of one tile also in paired blocks
     static_assert(kRun <= (1u << LC) && kRun <= kWave, "a run is one tile row inside one wave");
-    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan));
+    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan) >> CP::kLog2);
     const uint32_t prod = bp.row;
     const NttProd pr = a.prods[bp.seg];
     const NttOp oa = a.ops[pr.a];
     const int log2K = a.plan.log2K;
-    const ColThread<M> ct(a.plan, 0, bp.t);
+    const ColThread<M> ct(a.plan, 0, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
     const double *T0 = a.T + oa.base + (size_t)prod * (1ull << log2K);
     double v0[16], v1[16];
     ColTw tw0, tw1;
@@ -1209,18 +1256,31 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
         // 4 B per limb
         const double fwd_tb = 16.0 * (double)(fwd_blocks / tiles) * (double)K;
         const double prod_tb = 16.0 * (double)prod_rows * (double)K;
+        // a column level of 2^lm points runs 2^col_pair_log2(lm) tiles per
+        // block: block starts and grid in units of such blocks
+        const auto col_args = [&](int kind, int lm) {
+            PassArgs c = a;
+            const int pl = pair_log2_for(kind, lm);
+            for (int o = 0; o < 3; ++o) c.op_start[o] >>= pl;
+            for (int q = 0; q < 2; ++q) c.prod_start[q] >>= pl;
+            return c;
+        };
         if (fwd_blocks > 0) {
             GPUPI_RETURN_IF(timed(kColFwdLimbs, k, 4.0 * (double)limb_reads + fwd_tb, stream, [&] {
                 dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
-                    hipLaunchKernelGGL((ntt_col_fwd_kernel<true, decltype(mc)::value>), dim3((uint32_t)fwd_blocks),
-                                       dim3(256), 0, stream, a, 0);
+                    constexpr int pl = pair_log2_for(kPairFwdLimbs, decltype(mc)::value);
+                    hipLaunchKernelGGL((ntt_col_fwd_kernel<true, decltype(mc)::value>),
+                                       dim3((uint32_t)(fwd_blocks >> pl)), dim3(256 << pl), 0, stream,
+                                       col_args(kPairFwdLimbs, p.log2M[0]), 0);
                 });
             }));
             if (p.levels == 3) {
                 GPUPI_RETURN_IF(timed(kColFwd, k, 2 * fwd_tb, stream, [&] {
                     dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
+                        constexpr int pl = pair_log2_for(kPairFwd, decltype(mc)::value);
                         hipLaunchKernelGGL((ntt_col_fwd_kernel<false, decltype(mc)::value>),
-                                           dim3((uint32_t)fwd_blocks), dim3(256), 0, stream, a, 1);
+                                           dim3((uint32_t)(fwd_blocks >> pl)), dim3(256 << pl), 0, stream,
+                                           col_args(kPairFwd, p.log2M[1]), 1);
                     });
                 }));
             }
@@ -1250,16 +1310,19 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
         if (p.levels == 3) {
             GPUPI_RETURN_IF(timed(kColInv, k, 2 * prod_tb, stream, [&] {
                 dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
-                    hipLaunchKernelGGL((ntt_col_inv_kernel<decltype(mc)::value>), dim3((uint32_t)prod_blocks),
-                                       dim3(256), 0, stream, a, 1);
+                    constexpr int pl = pair_log2_for(kPairInv, decltype(mc)::value);
+                    hipLaunchKernelGGL((ntt_col_inv_kernel<decltype(mc)::value>), dim3((uint32_t)(prod_blocks >> pl)),
+                                       dim3(256 << pl), 0, stream, col_args(kPairInv, p.log2M[1]), 1);
                 });
             }));
         }
         GPUPI_RETURN_IF(timed(kColInvLimbs, k, prod_tb + 4.0 * (double)limb_writes + 8.0 * prod_rows * runs, stream,
                               [&] {
                                   dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
+                                      constexpr int pl = pair_log2_for(kPairInvLimbs, decltype(mc)::value);
                                       hipLaunchKernelGGL((ntt_col_inv_limbs_kernel<decltype(mc)::value>),
-                                                         dim3((uint32_t)prod_blocks), dim3(256), 0, stream, a);
+                                                         dim3((uint32_t)(prod_blocks >> pl)), dim3(256 << pl), 0,
+                                                         stream, col_args(kPairInvLimbs, p.log2M[0]));
                                   });
                               }));
         for (uint32_t q = 0; q < j.n_prods; ++q) {

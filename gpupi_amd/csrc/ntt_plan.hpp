@@ -151,6 +151,21 @@ GPUPI_HD uint64_t col_thread_offset(const Plan &p, int level, ColTile ct, uint32
 }
 GPUPI_HD int col_row_step_log2(const Plan &p, int level) { return slab_log2(p, level) - 4; }
 
+// Column levels of 2^7 or more points (32 or fewer adjacent columns per
+// tile) run 2^col_pair_log2 neighbouring tiles per block of 256 *
+// 2^col_pair_log2 threads.  Thread t works for tile col_pair_half(t) of the
+// block as that tile's thread col_pair_vtid(t): within a group of 2^lc * 2^pl
+// consecutive threads, the first 2^lc serve the first tile's row, the next
+// 2^lc the second tile's (adjacent columns), and so on, so a wave's accesses
+// cover 2^pl * C adjacent columns per row.
+GPUPI_HD constexpr int col_pair_log2(int lm) { return lm >= 7 ? 1 : 0; }
+GPUPI_HD constexpr uint32_t col_pair_half(uint32_t t, int lm) {
+    return (t >> (kTileLog2 - lm)) & ((1u << col_pair_log2(lm)) - 1u);
+}
+GPUPI_HD constexpr uint32_t col_pair_vtid(uint32_t t, int lm) {
+    return (t & ((1u << (kTileLog2 - lm)) - 1u)) | ((t >> (kTileLog2 - lm + col_pair_log2(lm))) << (kTileLog2 - lm));
+}
+
 GPUPI_HD uint32_t tiles_per_product(const Plan &p) { return 1u << (p.log2K - kTileLog2 > 0 ? p.log2K - kTileLog2 : 0); }
 
 // Row (last) level: tile t holds G = 4096 / M consecutive slabs.

@@ -65,7 +65,7 @@ struct Buf {
     bool use_cache() const { return ctx != nullptr && ctx->pipeline_form != GPUPI_FORM_PLAIN; }
     void drop() {
         if (ptr == nullptr) return;
-        if (cached) ctx->cache->give_back(ptr, cap * sizeof(uint32_t));
+        if (cached) ctx->cache->give_back(ptr, cap * sizeof(uint32_t), ctx->stream);
         else (void)hipFree(ptr);
         ptr = nullptr;
         cap = 0;
@@ -1163,7 +1163,7 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
     hipError_t err = stage(ctx, "decimal_powers", "precompute_5pow", convert_digits, levels, leaf, 0, false,
                            [&] { return powers.fill(ctx, leaf, levels); });
     char *digits = nullptr;
-    if (err == hipSuccess) err = hipMalloc(&digits, std::max<uint32_t>(convert_digits, 1));
+    if (err == hipSuccess) err = gpupi::ctx_alloc(ctx, reinterpret_cast<void **>(&digits), std::max<uint32_t>(convert_digits, 1));
     if (err == hipSuccess && requested_digits > 0) {
         const BatchMPArray frac = make_subview(ret, 0, std::min(prec_limbs, ret.length));
         err = stage(ctx, "decimal_convert", "equal_split", convert_digits, 1, frac.length, 0, false, [&] {
@@ -1176,8 +1176,8 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
     float ms = 0.0f;
     if (err == hipSuccess) err = hipEventElapsedTime(&ms, start, stop);
     if (err != hipSuccess) {
-        batch_mp_array_release(ret);
-        if (digits) (void)hipFree(digits);
+        gpupi::ctx_free(ctx, ret.data);
+        gpupi::ctx_free(ctx, digits);
         return err;
     }
     result->requested_digits = requested_digits;
@@ -1190,6 +1190,23 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
     result->digits = digits;
     result->elapsed_ms = ms;
     result->inverse_bits = inverse_bits;
+    return hipSuccess;
+}
+
+hipError_t gpupi_set_memory_pool(BatchMPContext *ctx, hipMemPool_t pool) {
+    if (ctx == nullptr) return hipErrorInvalidValue;
+    if (pool != ctx->cache->pool()) {
+        GPUPI_RETURN_IF(ctx->cache->trim());
+        if (ctx->workspace != nullptr) {
+            GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+            if (ctx->cache->pool() != nullptr) GPUPI_RETURN_IF(hipFreeAsync(ctx->workspace, ctx->stream));
+            else GPUPI_RETURN_IF(hipFree(ctx->workspace));
+            GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+            ctx->workspace = nullptr;
+            ctx->workspace_size_bytes = 0;
+        }
+    }
+    ctx->cache->set_pool(pool);
     return hipSuccess;
 }
 

@@ -20,6 +20,12 @@
 
 namespace {
 
+#define GPUPI_SERVE_RETURN_IF(x)                  \
+    do {                                          \
+        const hipError_t err_ = (x);              \
+        if (err_ != hipSuccess) return err_;      \
+    } while (0)
+
 constexpr uint64_t kHashMod = 1000000007ull;
 
 uint32_t ret_hash(const uint32_t *v, uint32_t n) {  // pi_bs_gpu.cu:852-858
@@ -64,6 +70,10 @@ struct GpupiServer {
     int64_t pending = 0;  // submitted and not yet returned as finished
     bool stop = false;
     std::vector<std::thread> threads;
+    // one stream-ordered memory pool for every worker: device memory of a
+    // finished job is reused by the next without hipMalloc / hipFree (which
+    // wait for the whole device); memory above the HBM budget is released
+    hipMemPool_t pool = nullptr;
 
     double now_ms() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -118,7 +128,10 @@ struct GpupiServer {
 void GpupiServer::pi_worker() {
     BatchMPContext *ctx = batch_mp_init();
     hipStream_t stream = nullptr;
-    if (ctx && hipStreamCreate(&stream) == hipSuccess) (void)batch_mp_set_stream(ctx, stream);
+    if (ctx && hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess) {
+        (void)batch_mp_set_stream(ctx, stream);
+        (void)gpupi_set_memory_pool(ctx, pool);
+    }
     for (;;) {
         std::unique_ptr<Job> j;
         {
@@ -128,8 +141,8 @@ void GpupiServer::pi_worker() {
             j = std::move(pi_ready.front());
             pi_ready.pop_front();
         }
-        int status = ctx ? hipSuccess : hipErrorNotInitialized;
-        if (ctx) {
+        int status = ctx && stream ? hipSuccess : hipErrorNotInitialized;
+        if (status == hipSuccess) {
             GpupiPiResult r{};
             status = gpupi_pi_run(ctx, j->digits, GPUPI_PREC_REQUESTED, &r);
             if (status == hipSuccess) {
@@ -140,19 +153,21 @@ void GpupiServer::pi_worker() {
                     own.resize(j->digits);
                     dig = own.data();
                 }
-                status = hipMemcpy(ret.data(), r.ret, ret.size() * 4, hipMemcpyDeviceToHost);
+                // copies and frees in this worker's stream only: nothing
+                // waits for the other workers' work
+                status = hipMemcpyAsync(ret.data(), r.ret, ret.size() * 4, hipMemcpyDeviceToHost, stream);
                 if (status == hipSuccess && j->digits)
-                    status = hipMemcpy(dig, r.digits, j->digits, hipMemcpyDeviceToHost);
+                    status = hipMemcpyAsync(dig, r.digits, j->digits, hipMemcpyDeviceToHost, stream);
+                (void)hipFreeAsync(r.ret, stream);
+                (void)hipFreeAsync(r.digits, stream);
+                const hipError_t synced = hipStreamSynchronize(stream);
+                if (status == hipSuccess) status = synced;
                 if (status == hipSuccess) {
                     j->res.ret_length = r.ret_length;
                     j->res.ret_hash = ret_hash(ret.data(), r.ret_length);
                     j->res.dec_hash = dec_hash(dig, j->digits);
                 }
-                gpupi_pi_free(&r);
             }
-            // the job's memory goes back before its budget share does
-            const hipError_t trimmed = gpupi_cache_trim(ctx);
-            if (status == hipSuccess) status = trimmed;
         }
         finish(std::move(j), status);
     }
@@ -163,7 +178,10 @@ void GpupiServer::pi_worker() {
 void GpupiServer::mul_worker() {
     BatchMPContext *ctx = batch_mp_init();
     hipStream_t stream = nullptr;
-    if (ctx && hipStreamCreate(&stream) == hipSuccess) (void)batch_mp_set_stream(ctx, stream);
+    if (ctx && hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess) {
+        (void)batch_mp_set_stream(ctx, stream);
+        (void)gpupi_set_memory_pool(ctx, pool);
+    }
     for (;;) {
         std::vector<std::unique_ptr<Job>> batch;
         {
@@ -179,8 +197,10 @@ void GpupiServer::mul_worker() {
         size_t words = 0;
         for (auto &j : batch) words += 2 * ((size_t)j->La + j->Lb);
         uint32_t *arena = nullptr;
-        int status = ctx ? hipSuccess : hipErrorNotInitialized;
-        if (status == hipSuccess) status = hipMalloc(&arena, std::max<size_t>(words, 1) * 4);
+        int status = ctx && stream ? hipSuccess : hipErrorNotInitialized;
+        if (status == hipSuccess)
+            status = hipMallocFromPoolAsync(reinterpret_cast<void **>(&arena), std::max<size_t>(words, 1) * 4, pool,
+                                            stream);
         std::vector<GpupiMulItem> items(batch.size());
         size_t off = 0;
         for (size_t i = 0; i < batch.size() && status == hipSuccess; ++i) {
@@ -196,8 +216,11 @@ void GpupiServer::mul_worker() {
             const size_t n = (size_t)batch[i]->La + batch[i]->Lb;
             if (n) status = hipMemcpyAsync(batch[i]->C_out, items[i].C, n * 4, hipMemcpyDeviceToHost, stream);
         }
-        if (status == hipSuccess) status = hipStreamSynchronize(stream);
-        if (arena) (void)hipFree(arena);
+        if (arena) (void)hipFreeAsync(arena, stream);
+        if (stream) {
+            const hipError_t synced = hipStreamSynchronize(stream);
+            if (status == hipSuccess) status = synced;
+        }
         const uint32_t nb = (uint32_t)batch.size();
         for (auto &j : batch) {
             j->res.batch_size = nb;
@@ -250,6 +273,17 @@ GpupiServer *gpupi_server_create(const GpupiServeConfig *cfg) {
     if (s->cfg.pi_workers == 0) s->cfg.pi_workers = 4;
     if (s->cfg.mul_batch_max == 0) s->cfg.mul_batch_max = 512;
     s->t0 = std::chrono::steady_clock::now();
+    int device = 0;
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    uint64_t threshold = s->cfg.hbm_budget_bytes;
+    if (hipGetDevice(&device) != hipSuccess || (props.location.id = device, hipMemPoolCreate(&s->pool, &props)) != hipSuccess ||
+        hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &threshold) != hipSuccess) {
+        if (s->pool) (void)hipMemPoolDestroy(s->pool);
+        delete s;
+        return nullptr;
+    }
     s->free_pi = (int)s->cfg.pi_workers;
     for (uint32_t i = 0; i < s->cfg.pi_workers; ++i) s->threads.emplace_back([s] { s->pi_worker(); });
     s->threads.emplace_back([s] { s->mul_worker(); });
@@ -331,7 +365,14 @@ void gpupi_server_destroy(GpupiServer *s) {
     s->cv_pi.notify_all();
     s->cv_mul.notify_all();
     for (auto &t : s->threads) t.join();
+    if (s->pool) (void)hipMemPoolDestroy(s->pool);
     delete s;
+}
+
+hipError_t gpupi_server_memory(GpupiServer *s, uint64_t *used_high, uint64_t *reserved_high) {
+    if (s == nullptr || used_high == nullptr || reserved_high == nullptr) return hipErrorInvalidValue;
+    GPUPI_SERVE_RETURN_IF(hipMemPoolGetAttribute(s->pool, hipMemPoolAttrUsedMemHigh, used_high));
+    return hipMemPoolGetAttribute(s->pool, hipMemPoolAttrReservedMemHigh, reserved_high);
 }
 
 }  // extern "C"

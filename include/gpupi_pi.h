@@ -160,6 +160,13 @@ int gpupi_stage_profile_query(const BatchMPContext *ctx, GpupiStageEntry *out, i
  * entry points are ordinary device allocations: release them as before. */
 hipError_t gpupi_cache_query(const BatchMPContext *ctx, size_t *cached_bytes, uint64_t *allocations);
 hipError_t gpupi_cache_trim(BatchMPContext *ctx);
+/* Attaches a stream-ordered memory pool (hipMemPoolCreate) to the context:
+ * from then on its pipeline buffers, NTT workspace and pi results come from
+ * that pool with hipMallocFromPoolAsync on the context's stream and go back
+ * with hipFreeAsync (no hipMalloc / hipFree, which wait for the whole
+ * device); results of gpupi_pi_run are then freed with hipFreeAsync rather
+ * than gpupi_pi_free.  Attach before the first allocation; null detaches. */
+hipError_t gpupi_set_memory_pool(BatchMPContext *ctx, hipMemPool_t pool);
 
 /* Sizing helpers (pi_bs_gpu.cu:924-944) */
 uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits);

@@ -66,6 +66,11 @@ int gpupi_server_results(GpupiServer *s, GpupiJobResult *out, int max, int wait)
 /* Blocks until every submitted job has finished. */
 void gpupi_server_drain(GpupiServer *s);
 void gpupi_server_destroy(GpupiServer *s);
+/* Peak device bytes the server's memory pool has handed out (used) and held
+ * (reserved) since it was created.  Every worker takes its device memory
+ * (pipeline buffers, NTT workspace, operand arenas) from this one pool in
+ * stream order; memory above the HBM budget goes back to the device. */
+hipError_t gpupi_server_memory(GpupiServer *s, uint64_t *used_high, uint64_t *reserved_high);
 
 /* Device footprint estimate used for admission (bytes). */
 uint64_t gpupi_serve_job_bytes(int kind, uint32_t digits_or_la, uint32_t lb);

@@ -146,6 +146,39 @@ static void check_plan(const Plan &p, bool full) {
     }
 }
 
+// Paired column tiles: the block's threads map one to one onto (tile,
+// tile-local thread); inside each group of 2^(lc + pl) consecutive threads
+// the columns of one row run consecutively across both tiles (tiles 2b and
+// 2b + 1 of a slab are adjacent); a final-level run of 2^RL aligned lanes is
+// one row of one tile.
+static void check_col_pairs() {
+    for (int lm = 5; lm <= kColMaxLog2; ++lm) {
+        const int pl = col_pair_log2(lm), lc = kTileLog2 - lm;
+        const uint32_t threads = 256u << pl;
+        std::vector<int> seen(threads, 0);
+        for (uint32_t t = 0; t < threads; ++t) {
+            const uint32_t h = col_pair_half(t, lm), v = col_pair_vtid(t, lm);
+            EXPECT(h < (1u << pl) && v < 256u, "pair range lm=%d t=%u", lm, t);
+            if (h < (1u << pl) && v < 256u) seen[h * 256u + v]++;
+            // absolute column of thread t within a slab (tiles of this block
+            // start at column 0), and its row
+            const uint32_t col = (h << lc) + (v & ((1u << lc) - 1u)), row = v >> lc;
+            if ((t & ((1u << (lc + pl)) - 1u)) != 0) {
+                const uint32_t hp = col_pair_half(t - 1, lm), vp = col_pair_vtid(t - 1, lm);
+                const uint32_t colp = (hp << lc) + (vp & ((1u << lc) - 1u)), rowp = vp >> lc;
+                EXPECT(row == rowp && col == colp + 1, "pair lanes adjacent lm=%d t=%u", lm, t);
+            }
+            const int rl = run_log2_for(lm);
+            const uint32_t first = t & ~((1u << rl) - 1u);
+            EXPECT(col_pair_half(first, lm) == h && (col_pair_vtid(first, lm) >> lc) == row,
+                   "run inside one tile row lm=%d t=%u", lm, t);
+        }
+        for (uint32_t i = 0; i < threads; ++i) EXPECT(seen[i] == 1, "pair bijection lm=%d i=%u", lm, i);
+    }
+    // paired levels exist only where a product has an even number of tiles
+    for (int k = kTileLog2 + 1; k <= kMaxLog2; ++k) EXPECT(tiles_per_product(make_plan(k)) % 2 == 0, "tiles even");
+}
+
 static void check_xcd_tile() {
     for (uint32_t n = 1; n < 3000; n += (n < 100 ? 1 : 37)) {
         std::vector<int> seen(n, 0);
@@ -164,6 +197,7 @@ static void check_xcd_tile() {
 
 int main() {
     check_tile_steps();
+    check_col_pairs();
     check_xcd_tile();
     check_bitrev_split();
     // the built-in plans, then every split the run-time table accepts

@@ -33,9 +33,55 @@ hipError_t hipMemcpyAsync(void *d, const void *s, size_t b, hipMemcpyKind, hipSt
 hipError_t hipStreamCreate(hipStream_t *s) { *s = (hipStream_t)malloc(1); return hipSuccess; }
 hipError_t hipStreamDestroy(hipStream_t s) { free(s); return hipSuccess; }
 hipError_t hipStreamSynchronize(hipStream_t) { jitter(200); return hipSuccess; }
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { return hipStreamCreate(s); }
+hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+// the pool: host memory, with live and peak byte counts
+static std::mutex g_pool_mu;
+static std::map<void *, size_t> g_pool_live;
+static uint64_t g_pool_used = 0, g_pool_high = 0;
+hipError_t hipMemPoolCreate(hipMemPool_t *pool, const hipMemPoolProps *) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool_used = g_pool_high = 0;
+    *pool = (hipMemPool_t)malloc(1);
+    return hipSuccess;
+}
+hipError_t hipMemPoolDestroy(hipMemPool_t pool) { free(pool); return hipSuccess; }
+hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void *) { return hipSuccess; }
+hipError_t hipMemPoolGetAttribute(hipMemPool_t, hipMemPoolAttr attr, void *v) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    *(uint64_t *)v = attr == hipMemPoolAttrUsedMemHigh || attr == hipMemPoolAttrReservedMemHigh ? g_pool_high : 0;
+    return hipSuccess;
+}
+hipError_t hipMallocFromPoolAsync(void **p, size_t bytes, hipMemPool_t, hipStream_t) {
+    *p = malloc(bytes ? bytes : 1);
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool_live[*p] = bytes;
+    g_pool_used += bytes;
+    if (g_pool_used > g_pool_high) g_pool_high = g_pool_used;
+    return hipSuccess;
+}
+hipError_t hipFreeAsync(void *p, hipStream_t) {
+    if (p == nullptr) return hipSuccess;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto it = g_pool_live.find(p);
+    if (it != g_pool_live.end()) {
+        g_pool_used -= it->second;
+        g_pool_live.erase(it);
+    }
+    free(p);
+    return hipSuccess;
+}
+hipError_t gpupi_set_memory_pool(BatchMPContext *, hipMemPool_t) { return hipSuccess; }
 BatchMPContext *batch_mp_init(void) { jitter(2000); return new BatchMPContext{1}; }
 void batch_mp_destroy(BatchMPContext *c) { delete c; }
 hipError_t batch_mp_set_stream(BatchMPContext *, hipStream_t) { return hipSuccess; }
+// the pi result is pool memory in the server (freed with hipFreeAsync)
+static void pool_track(void *p, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool_live[p] = bytes;
+    g_pool_used += bytes;
+    if (g_pool_used > g_pool_high) g_pool_high = g_pool_used;
+}
 hipError_t gpupi_pi_run(BatchMPContext *, uint32_t digits, int, GpupiPiResult *r) {
     jitter(3000);
     memset(r, 0, sizeof(*r));
@@ -44,6 +90,8 @@ hipError_t gpupi_pi_run(BatchMPContext *, uint32_t digits, int, GpupiPiResult *r
     for (uint32_t i = 0; i < r->ret_length; ++i) r->ret[i] = digits * 7 + i;
     r->digits = (char *)malloc(digits + 1);
     for (uint32_t i = 0; i < digits; ++i) r->digits[i] = (char)('0' + (digits + i) % 10);
+    pool_track(r->ret, 4 * r->ret_length);
+    pool_track(r->digits, digits + 1);
     return hipSuccess;
 }
 void gpupi_pi_free(GpupiPiResult *r) { free(r->ret); free(r->digits); }
@@ -142,7 +190,12 @@ static void scenario(int seed, uint64_t budget, int pi_workers, int jobs) {
     }
     submitted = true;
     collector.join();
+    uint64_t used_high = 0, reserved_high = 0;
+    EXPECT(gpupi_server_memory(s, &used_high, &reserved_high) == hipSuccess, "memory query");
     gpupi_server_destroy(s);
+    // the one pool never held more than the budget
+    EXPECT(used_high <= budget, "pool peak %llu > budget %llu", (unsigned long long)used_high,
+           (unsigned long long)budget);
     std::map<int64_t, int> seen;
     for (auto &r : results) {
         seen[r.id]++;

@@ -18,3 +18,20 @@ hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKin
 hipError_t hipStreamCreate(hipStream_t *s);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamSynchronize(hipStream_t s);
+enum { hipStreamNonBlocking = 1 };
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
+hipError_t hipGetDevice(int *d);
+// stream-ordered memory pool
+typedef struct ihipMemPool_t *hipMemPool_t;
+enum hipMemAllocationType { hipMemAllocationTypePinned = 1 };
+enum hipMemLocationType { hipMemLocationTypeDevice = 1 };
+struct hipMemLocation { hipMemLocationType type; int id; };
+struct hipMemPoolProps { hipMemAllocationType allocType; int handleTypes; hipMemLocation location; };
+enum hipMemPoolAttr { hipMemPoolAttrReleaseThreshold = 4, hipMemPoolAttrReservedMemHigh = 6,
+                      hipMemPoolAttrUsedMemHigh = 8 };
+hipError_t hipMemPoolCreate(hipMemPool_t *pool, const hipMemPoolProps *props);
+hipError_t hipMemPoolDestroy(hipMemPool_t pool);
+hipError_t hipMemPoolSetAttribute(hipMemPool_t pool, hipMemPoolAttr attr, void *value);
+hipError_t hipMemPoolGetAttribute(hipMemPool_t pool, hipMemPoolAttr attr, void *value);
+hipError_t hipMallocFromPoolAsync(void **p, size_t bytes, hipMemPool_t pool, hipStream_t s);
+hipError_t hipFreeAsync(void *p, hipStream_t s);

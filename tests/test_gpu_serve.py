@@ -49,6 +49,29 @@ def test_mix_equals_alone_and_oracle():
     assert max(r.batch_size for k, (r, _) in mix.items() if k[0] == "mul") > 1
 
 
+def test_pool_stays_within_budget_under_load():
+    # many jobs against a budget of about three jobs: the pool's peak (all
+    # workers' pipeline buffers, workspaces and arenas) stays within it
+    rng = np.random.default_rng(11)
+    pis = [3000, 7000, 11000, 20000, 30000]
+    muls = [(rand_rows(rng, 1, 1 << 17)[0], rand_rows(rng, 1, (1 << 17) - i)[0]) for i in range(5)]
+    budget = 3 * max(g.lib().gpupi_serve_job_bytes(g.JOB_PI, 30000, 0),
+                     g.lib().gpupi_serve_job_bytes(g.JOB_MUL, 1 << 17, 1 << 17))
+    srv = g.Server(budget_bytes=budget, pi_workers=3)
+    try:
+        out = run_jobs(srv, pis, muls)
+        used_high, _ = srv.memory()
+    finally:
+        srv.close()
+    assert 0 < used_high <= budget, (used_high, budget)
+    for d in pis:
+        r, data = out[("pi", d)]
+        _, odec, _ = of.pi_digits(d)
+        assert data.decode() == odec
+    for A, B in muls:
+        assert np.array_equal(out[("mul", (A.size, B.size))][1], oracle_mul(A, B))
+
+
 def test_small_budget_serializes_but_stays_exact():
     rng = np.random.default_rng(3)
     # three shapes (results are keyed by shape), each over half the budget
@@ -62,6 +85,10 @@ def test_small_budget_serializes_but_stays_exact():
         out = run_jobs(srv, [5000], muls)
         # a job larger than the whole budget is refused at submission
         assert srv.submit_mul(np.ones(1 << 22, np.uint32), np.ones(1 << 22, np.uint32)) == -1
+        # every worker's device memory came from the server's one pool, whose
+        # peak stayed within the budget (one job at a time)
+        used_high, reserved_high = srv.memory()
+        assert 0 < used_high <= need, (used_high, need)
     finally:
         srv.close()
     for A, B in muls:
