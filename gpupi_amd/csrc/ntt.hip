@@ -343,8 +343,10 @@ struct ColPair {
 // step-0 elements.
 template <int M>
 struct ColThread {
-    uint64_t off0;     // element offset of u = 0 within the product
-    uint32_t ustep;    // log2 of the offset step between u and u + 1
+    uint64_t off0;     // element (coefficient) index of u = 0 within the product
+    uint32_t ustep;    // log2 of the index step between u and u + 1
+    uint64_t poff0;    // plane position of u = 0 (padded planes)
+    uint64_t pstep;    // plane position step between u and u + 1
     uint32_t cg;       // (col0 + g) << (30 - lk): twiddle exponent per bitrev(r)
     uint32_t b0;       // bitrev(r0, M)
     uint32_t lds0;     // LDS slot of u = 0
@@ -358,11 +360,19 @@ struct ColThread {
         const uint32_t r0 = tid >> LC;
         off0 = col_thread_offset(p, level, ct, tid);
         ustep = (uint32_t)col_row_step_log2(p, level);
+        poff0 = plane_index(p, off0);
+        // a step of 2^ustep elements crosses 2^(ustep - s) padded rows when
+        // it is at least one row long, and none otherwise
+        const int s = plane_pad_shift(p);
+        pstep = (1ull << ustep) + ((s && (int)ustep >= s) ? ((uint64_t)kPlanePad << (ustep - s)) : 0ull);
         cg = (ct.col0 + g) << (kMaxLog2 - slab_log2(p, level));
         b0 = __brev(r0) >> (32 - M);
         lds0 = lds_index(tid);
     }
+    // coefficient index of element u (limbs)
     __device__ __forceinline__ uint64_t off(int u) const { return off0 + ((uint64_t)u << ustep); }
+    // plane position of element u (residues)
+    __device__ __forceinline__ uint64_t poff(int u) const { return poff0 + (uint64_t)u * pstep; }
 };
 
 // One prime of a forward column level on the thread's step-0 elements v:
@@ -399,9 +409,8 @@ __global__ void __launch_bounds__((ColPair<M, FROM_LIMBS ? kPairFwdLimbs : kPair
     double *const lds = lds_tiles + CP::half() * kLdsSlots;
     const BlockPos bp = block_pos(a.op_start, a.n_ops, tiles_per_product(a.plan) >> CP::kLog2);
     const NttOp o = a.ops[bp.seg];
-    const uint64_t K = 1ull << a.plan.log2K;
     const ColThread<M> ct(a.plan, level, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
-    double *T0 = a.T + o.base + (size_t)bp.row * K;
+    double *T0 = a.T + o.base + (size_t)bp.row * plane_row_elems(a.plan);
     double *T1 = T0 + o.pstride;
     double v0[16], v1[16];
     if (FROM_LIMBS) {
@@ -421,7 +430,7 @@ __global__ void __launch_bounds__((ColPair<M, FROM_LIMBS ? kPairFwdLimbs : kPair
         }
     } else {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v0[u] = T0[ct.off(u)];
+        for (int u = 0; u < 16; ++u) v0[u] = T0[ct.poff(u)];
     }
     const ColTw tw0 = col_tw_load(a.tables, ct.cg, ct.b0, false);
     const ColTw tw1 = col_tw_load(a.tables + kPrimeTable, ct.cg, ct.b0, false);
@@ -430,17 +439,17 @@ __global__ void __launch_bounds__((ColPair<M, FROM_LIMBS ? kPairFwdLimbs : kPair
         // register budget (occupancy 4 needs <= 128 VGPRs): prime 0 goes out
         // before prime 1 is loaded and transformed
 #pragma unroll
-        for (int u = 0; u < 16; ++u) T0[ct.off(u)] = v0[u];
+        for (int u = 0; u < 16; ++u) T0[ct.poff(u)] = v0[u];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v1[u] = T1[ct.off(u)];
+        for (int u = 0; u < 16; ++u) v1[u] = T1[ct.poff(u)];
     }
     col_fwd_prime<M, 1>(v1, lds, ct, a.tables, tw1);
     if (FROM_LIMBS) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) T0[ct.off(u)] = v0[u];
+        for (int u = 0; u < 16; ++u) T0[ct.poff(u)] = v0[u];
     }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) T1[ct.off(u)] = v1[u];
+    for (int u = 0; u < 16; ++u) T1[ct.poff(u)] = v1[u];
 }
 
 // Step-0 elements of a row tile (DFT size 2^M): v[j] is tile index
@@ -460,10 +469,10 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan));
     const NttProd pr = a.prods[bp.seg];
     const NttOp oa = a.ops[pr.a], ob = a.ops[pr.b];
-    const uint64_t K = 1ull << a.plan.log2K;
-    const size_t off = (size_t)bp.t * kTile + row_step0_base<M>();
-    double *TA0 = a.T + oa.base + (size_t)bp.row * K + off;
-    const double *TB0 = a.T + ob.base + (size_t)(pr.b_bcast ? 0u : bp.row) * K + off;
+    const uint64_t Kp = plane_row_elems(a.plan);
+    const size_t off = plane_index(a.plan, (size_t)bp.t * kTile) + row_step0_base<M>();
+    double *TA0 = a.T + oa.base + (size_t)bp.row * Kp + off;
+    const double *TB0 = a.T + ob.base + (size_t)(pr.b_bcast ? 0u : bp.row) * Kp + off;
 #pragma clang loop unroll(disable)
     for (int P = 0; P < 2; ++P) {
         const Mod m = mod_of(P);
@@ -513,9 +522,9 @@ __global__ void __launch_bounds__(256) ntt_row_pair_kernel(PassArgs a, uint32_t 
     const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t row = bid / tiles, t = bid % tiles;
     const NttOp ob = a.ops[a.prods[0].b];
-    const uint64_t K = 1ull << a.plan.log2K;
-    const size_t off = (size_t)t * kTile + row_step0_base<M>();
-    const double *TB0 = a.T + ob.base + (size_t)(a.prods[0].b_bcast ? 0u : row) * K + off;
+    const uint64_t Kp = plane_row_elems(a.plan);
+    const size_t off = plane_index(a.plan, (size_t)t * kTile) + row_step0_base<M>();
+    const double *TB0 = a.T + ob.base + (size_t)(a.prods[0].b_bcast ? 0u : row) * Kp + off;
 #pragma clang loop unroll(disable)
     for (int P = 0; P < 2; ++P) {
         const Mod m = mod_of(P);
@@ -529,7 +538,7 @@ __global__ void __launch_bounds__(256) ntt_row_pair_kernel(PassArgs a, uint32_t 
         for (int q = 0; q < 2; ++q) {
             if (row >= (q ? rows1 : rows0)) continue;  // uniform over the block
             const NttOp oa = a.ops[a.prods[q].a];
-            double *TA = a.T + oa.base + (size_t)row * K + off + P * oa.pstride;
+            double *TA = a.T + oa.base + (size_t)row * Kp + off + P * oa.pstride;
             // the table pointer is opaque per product, so that the twiddle
             // loads stay inside the loop instead of occupying registers
             // across it
@@ -574,9 +583,9 @@ template <int M>
 __device__ __forceinline__ void col_inv_load(double *v0, double *v1, ColTw &tw0, ColTw &tw1, const PassArgs &a,
                                              const ColThread<M> &ct, const double *T0, const double *T1) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v0[u] = T0[ct.off(u)];
+    for (int u = 0; u < 16; ++u) v0[u] = T0[ct.poff(u)];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v1[u] = T1[ct.off(u)];
+    for (int u = 0; u < 16; ++u) v1[u] = T1[ct.poff(u)];
     tw0 = col_tw_load(a.tables, ct.cg, ct.b0, true);
     tw1 = col_tw_load(a.tables + kPrimeTable, ct.cg, ct.b0, true);
 }
@@ -591,7 +600,7 @@ __global__ void __launch_bounds__((ColPair<M, kPairInv>::kThreads)) ntt_col_inv_
     const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan) >> CP::kLog2);
     const NttOp oa = a.ops[a.prods[bp.seg].a];
     const ColThread<M> ct(a.plan, level, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
-    double *T0 = a.T + oa.base + (size_t)bp.row * (1ull << a.plan.log2K);
+    double *T0 = a.T + oa.base + (size_t)bp.row * plane_row_elems(a.plan);
     double *T1 = T0 + oa.pstride;
     // register budget (occupancy 4 needs <= 128 VGPRs): one prime's
     // elements at a time
@@ -599,16 +608,16 @@ __global__ void __launch_bounds__((ColPair<M, kPairInv>::kThreads)) ntt_col_inv_
     const ColTw tw1 = col_tw_load(a.tables + kPrimeTable, ct.cg, ct.b0, true);
     double v[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = T0[ct.off(u)];
+    for (int u = 0; u < 16; ++u) v[u] = T0[ct.poff(u)];
     col_inv_prime<M, 0>(v, lds, ct, a.tables, tw0);
 #pragma unroll
-    for (int u = 0; u < 16; ++u) T0[ct.off(u)] = v[u];
+    for (int u = 0; u < 16; ++u) T0[ct.poff(u)] = v[u];
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = T1[ct.off(u)];
+    for (int u = 0; u < 16; ++u) v[u] = T1[ct.poff(u)];
     col_inv_prime<M, 1>(v, lds, ct, a.tables, tw1);
 #pragma unroll
-    for (int u = 0; u < 16; ++u) T1[ct.off(u)] = v[u];
+    for (int u = 0; u < 16; ++u) T1[ct.poff(u)] = v[u];
 }
 
 // First inverse column level: both primes through one tile, 1/K, Garner,
@@ -636,7 +645,7 @@ __global__ void __launch_bounds__((ColPair<M, kPairInvLimbs>::kThreads)) ntt_col
     const NttOp oa = a.ops[pr.a];
     const int log2K = a.plan.log2K;
     const ColThread<M> ct(a.plan, 0, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
-    const double *T0 = a.T + oa.base + (size_t)prod * (1ull << log2K);
+    const double *T0 = a.T + oa.base + (size_t)prod * plane_row_elems(a.plan);
     double v0[16], v1[16];
     ColTw tw0, tw1;
     col_inv_load<M>(v0, v1, tw0, tw1, a, ct, T0, T0 + oa.pstride);
@@ -1122,7 +1131,7 @@ JobLayout layout_of(const NttJob &j, const JobShape &sh) {
     size_t off = 0;
     for (uint32_t o = 0; o < j.n_ops; ++o) {
         const size_t cap = sh.op_bcast[o] ? 1 : std::min<size_t>(l.nc, sh.op_rows[o]);
-        l.plane_elems[o] = align256(cap * K * sizeof(double)) / sizeof(double);
+        l.plane_elems[o] = align256(cap * plane_row_elems(active_plan(j.k)) * sizeof(double)) / sizeof(double);
         l.op_base[o] = off / sizeof(double);
         off += 2 * l.plane_elems[o] * sizeof(double);
     }

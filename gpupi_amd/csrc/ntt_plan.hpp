@@ -151,6 +151,31 @@ GPUPI_HD uint64_t col_thread_offset(const Plan &p, int level, ColTile ct, uint32
 }
 GPUPI_HD int col_row_step_log2(const Plan &p, int level) { return slab_log2(p, level) - 4; }
 
+// Residue planes are padded: after every row of the first level's 2D view
+// (2^(k - M0) elements, the rows a first-level column tile reads 2^M0 of)
+// come kPlanePad unused elements, so those rows are not a power-of-two
+// distance apart (at K = 2^28: 8 MB + 128 B instead of 8 MB).  Deeper levels
+// and the contiguous level work inside one such row (their tiles and slabs
+// divide it), so only the row's start moves.  Plans whose first-level rows
+// are shorter than a tile (2^(k - M0) < 4096) are not padded.
+constexpr uint32_t kPlanePad = 16;  // binary64 elements (128 bytes)
+GPUPI_HD int plane_pad_shift(const Plan &p) {
+    if (p.levels < 2) return 0;
+    const int s = p.log2K - p.log2M[0];
+    return s >= kTileLog2 ? s : 0;
+}
+// Physical index of logical element e of a product's plane.
+GPUPI_HD uint64_t plane_index(const Plan &p, uint64_t e) {
+    const int s = plane_pad_shift(p);
+    return s ? e + (e >> s) * kPlanePad : e;
+}
+// Elements one product's plane takes (its row stride in a launch).
+GPUPI_HD uint64_t plane_row_elems(const Plan &p) {
+    const int s = plane_pad_shift(p);
+    const uint64_t K = 1ull << p.log2K;
+    return s ? K + (K >> s) * kPlanePad : K;
+}
+
 // Column levels of 2^7 or more points (32 or fewer adjacent columns per
 // tile) run 2^col_pair_log2 neighbouring tiles per block of 256 *
 // 2^col_pair_log2 threads.  Thread t works for tile col_pair_half(t) of the

@@ -102,23 +102,41 @@ static void check_plan(const Plan &p, bool full) {
                 }
                 EXPECT(off < K, "offset k=%d l=%d t=%u i=%u", k, l, t, i);
                 if (full && off < K) seen[off]++;
+                // padded plane position: inside the product's plane
+                const uint64_t pos = plane_index(p, off);
+                EXPECT(pos < plane_row_elems(p), "plane position k=%d l=%d", k, l);
                 if (col) {
                     // thread tid = i mod 256, u = i / 256: offset and twiddle exponent
                     const ColTile ct = col_tile(p, l, t);
                     const uint32_t tid = i & 255u, u = i >> 8;
                     const uint64_t want = col_thread_offset(p, l, ct, tid) + ((uint64_t)u << col_row_step_log2(p, l));
                     EXPECT(off == want, "col thread offset k=%d l=%d", k, l);
+                    // the kernels' plane position: base + u * step
+                    const int ps = plane_pad_shift(p);
+                    const int us = col_row_step_log2(p, l);
+                    const uint64_t pstep = (1ull << us) + ((ps && us >= ps) ? ((uint64_t)kPlanePad << (us - ps)) : 0ull);
+                    EXPECT(pos == plane_index(p, col_thread_offset(p, l, ct, tid)) + u * pstep,
+                           "col plane step k=%d l=%d", k, l);
                     const int lk = slab_log2(p, l);
                     const uint32_t g = i & (C - 1), r = i >> (kTileLog2 - lm);
                     const uint64_t cg = (uint64_t)(ct.col0 + g) << (kMaxLog2 - lk);
                     EXPECT(cg < (1ull << kMaxLog2) && cg * bitrev(r, lm) < (1ull << kMaxLog2), "twiddle exp k=%d", k);
                 } else {
                     EXPECT(off == (uint64_t)t * kTile + i, "row offset contiguous k=%d", k);
+                    // a row tile stays contiguous in the padded plane
+                    EXPECT(pos == plane_index(p, (uint64_t)t * kTile) + i, "row tile padded contiguous k=%d", k);
                 }
             }
         }
         if (full)
             for (uint64_t x = 0; x < K; ++x) EXPECT(seen[x] == 1, "coverage k=%d l=%d x=%llu", k, l, (unsigned long long)x);
+        // padding: rows of the first level's view start kPlanePad apart from
+        // a power-of-two distance, and positions stay one to one
+        if (plane_pad_shift(p)) {
+            const int ps = plane_pad_shift(p);
+            EXPECT(plane_index(p, 1ull << ps) == (1ull << ps) + kPlanePad, "pad k=%d", k);
+            EXPECT(plane_index(p, K - 1) == plane_row_elems(p) - 1 - kPlanePad, "pad end k=%d", k);
+        }
         if (l == 0) {
             // final inverse level: thread tid's element u is coefficient
             // col_thread_offset(tid) + u 2^(k-4); the 2^rl lanes of a run hold
