@@ -298,6 +298,7 @@ void batch_mp_destroy(BatchMPContext *ctx) {
     if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
     gpupi::ntt_tables_release(&ctx->ntt_tables);
     stage_profiler_free(ctx->profiler);
+    tree_graph_free(ctx->tree_graph);
     delete ctx->cache;
     delete ctx;
 }

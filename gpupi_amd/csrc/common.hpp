@@ -167,6 +167,9 @@ class BlockCache {
 // Device memory of a context: from its memory pool in stream order when one
 // is attached, else hipMalloc / hipFree.
 hipError_t ctx_alloc(BatchMPContext *ctx, void **ptr, size_t bytes);
+// While set on this thread, NTT launches record no kernel statistics (no
+// events: work being captured into a graph).
+void ntt_stats_suspend(bool suspend);
 void ctx_free(BatchMPContext *ctx, void *ptr);
 
 constexpr int kScalarWords = 4;  // device words of BatchMPContext::d_scalar
@@ -181,6 +184,10 @@ hipError_t compact_arrays(BatchMPContext *ctx, BatchMPArray *const *arrays, int 
 // Per-stage event timings of the pi pipeline (defined with the pipeline).
 struct StageProfiler;
 void stage_profiler_free(StageProfiler *p);
+// The bottom levels of the binary-splitting tree as a hipGraph (defined with
+// the pipeline).
+struct TreeGraph;
+void tree_graph_free(TreeGraph *g);
 
 struct BatchMPContext {
     NTTPrecomputedTables ntt_tables;
@@ -193,4 +200,5 @@ struct BatchMPContext {
     StageProfiler *profiler;  // null unless stage profiling is on
     gpupi::BlockCache *cache; // blocks of the pipeline's buffers
     int pipeline_form;        // GPUPI_FORM_BATCHED or GPUPI_FORM_PLAIN
+    TreeGraph *tree_graph;    // captured tree levels (null until first used)
 };

@@ -963,6 +963,8 @@ size_t spill_runs_bound(int k) { return (size_t(1) << k) >> run_log2_for(kColMax
 
 }  // namespace
 
+void ntt_stats_suspend(bool suspend) { t_stats_off = suspend; }
+
 bool ntt_supports(uint32_t L_a, uint32_t L_b) {
     return L_a > 0 && L_b > 0 && transform_log2(L_a, L_b) <= kMaxLog2;
 }
@@ -1446,9 +1448,10 @@ hipError_t launch_mul_ntt_bounded(const uint32_t *A, const uint32_t *B, uint32_t
             nc = (nc + 1) / 2;
         }
     }
+    const bool prev = t_stats_off;
     t_stats_off = true;
     const hipError_t err = run_ntt(j, workspace, workspace_bytes, tables, stream);
-    t_stats_off = false;
+    t_stats_off = prev;
     return err;
 }
 

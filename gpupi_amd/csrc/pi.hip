@@ -444,7 +444,10 @@ struct TreeBufs {
 //   kPerOp: the reference's sequence (pi_bs_gpu.cu:637-723): P, compact P,
 //     Q, compact Q, R*Q, P*R, R, compact R, each product on its own.
 // All three leave the same integers.
-enum class Lengths { kBound, kShared, kPerOp };
+//   kBoundShared: the batched pipeline's products and rows without the
+//     compaction (lengths are the products' bounds; the tree's captured
+//     levels).
+enum class Lengths { kBound, kShared, kBoundShared, kPerOp };
 
 // One pairwise level n1 of the tree (pi_bs_gpu.cu:625-723) over the rows of
 // P, Q, R.
@@ -465,8 +468,9 @@ hipError_t tree_level(BatchMPContext *ctx, TreeBufs &t, uint32_t N, uint32_t n1,
     // NTT's carry pass and the shift-add take four limbs per lane); the
     // other forms keep the reference's stride = length (the group stage's
     // buffers keep its preallocated sizes and may be frozen for capture)
+    const bool shared = mode == Lengths::kShared || mode == Lengths::kBoundShared;
     auto out_view = [&](Buf &b, uint32_t batch, uint32_t length, BatchMPArray &v) {
-        return mode == Lengths::kShared ? b.view_aligned(batch, length, v) : b.view(batch, length, v);
+        return shared ? b.view_aligned(batch, length, v) : b.view(batch, length, v);
     };
     const uint32_t terms_right = (n / n1) + ((n / n1) >> 4);
     const uint32_t shift = 15u * terms_right;
@@ -674,6 +678,146 @@ hipError_t leaf_stage_capture(GpupiLeafStage *s) {
 
 }  // namespace
 
+// ------------------------------------------- tree levels as a hipGraph
+// The bottom levels of the binary-splitting tree (rows up to one-tile
+// products: schoolbook and single-tile NTT, where launch latency and the
+// per-level compaction round trip cost as much as the work) run with
+// lengths fixed on the host and no compaction (the values are the same
+// integers: compaction only drops zero top limbs), captured once as a
+// hipGraph on a private stream and replayed.  The graph belongs to the
+// context and is keyed by the group count, the compacted lengths of the
+// group stage's rows and the schoolbook/NTT crossover; its buffers and
+// workspace are its own (hipMalloc'd, never from the block cache), so
+// nothing it points at moves between replays.
+struct TreeGraph {
+    uint32_t N = 0, LP = 0, LQ = 0, LR = 0, levels = 0;
+    int ntt_min = 0;
+    Buf in_P, in_Q, in_R;  // the group stage's rows: the graph's input
+    TreeBufs t;            // the captured levels' buffers
+    uint32_t *ws = nullptr;
+    size_t ws_bytes = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    BatchMPArray P{}, Q{}, R{};  // outputs of the last captured level
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        t.freeze(false);
+    }
+    ~TreeGraph() {
+        drop_graph();
+        if (ws) (void)hipFree(ws);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+void tree_graph_free(TreeGraph *g) { delete g; }
+
+namespace {
+
+// Largest L_a + L_b a captured level may multiply: one 4096-element tile.
+constexpr uint32_t kGraphMaxProduct = 4096;
+
+// Levels (from n1 = N down) whose products all fit one tile when lengths
+// grow by the rules of tree_level without compaction, starting from the
+// compacted group-stage lengths; never the top level (n1 = 2).
+uint32_t graph_levels(uint32_t N, uint32_t LP, uint32_t LQ, uint32_t LR) {
+    const uint32_t n = N * 16u;
+    uint32_t levels = 0;
+    for (uint32_t n1 = N; n1 > 2; n1 >>= 1) {
+        const uint64_t pp = 2ull * LP, qq = 2ull * LQ, rq = (uint64_t)LR + LQ, pr = (uint64_t)LP + LR;
+        if (std::max(std::max(pp, qq), std::max(rq, pr)) > kGraphMaxProduct) break;
+        const bool subtract = (n1 == n) || (n1 == (n >> 4));
+        const uint32_t shift = 15u * ((n / n1) + ((n / n1) >> 4));
+        const uint64_t shifted = rq + (shift >> 5) + ((shift & 31u) != 0);
+        LR = (uint32_t)(std::max<uint64_t>(shifted, pr) + (subtract ? 0u : 1u));
+        LP = (uint32_t)pp;
+        LQ = (uint32_t)qq;
+        ++levels;
+    }
+    return levels;
+}
+
+// Runs `fn` on the graph's stream with its workspace as the context's.
+template <typename F>
+hipError_t with_graph_context(BatchMPContext *ctx, TreeGraph *g, F &&fn) {
+    const hipStream_t old_stream = ctx->stream;
+    uint32_t *const old_ws = ctx->workspace;
+    const size_t old_bytes = ctx->workspace_size_bytes;
+    ctx->stream = g->stream;
+    ctx->workspace = g->ws;
+    ctx->workspace_size_bytes = g->ws_bytes;
+    const hipError_t err = fn();
+    g->ws = ctx->workspace;
+    g->ws_bytes = ctx->workspace_size_bytes;
+    ctx->stream = old_stream;
+    ctx->workspace = old_ws;
+    ctx->workspace_size_bytes = old_bytes;
+    return err;
+}
+
+// The captured levels over the graph's input rows (views P, Q, R of the
+// compacted group stage); leaves the last level's rows in P, Q, R.
+hipError_t graph_body(BatchMPContext *ctx, TreeGraph *g, BatchMPArray &P, BatchMPArray &Q, BatchMPArray &R) {
+    uint32_t n1 = g->N;
+    for (uint32_t l = 0; l < g->levels; ++l, n1 >>= 1)
+        GPUPI_RETURN_IF(tree_level(ctx, g->t, g->N, n1, Lengths::kBoundShared, P, Q, R));
+    return hipSuccess;
+}
+
+// Captures the levels of g (two eager runs first: every ping-pong buffer
+// and the workspace reach the sizes of both roles), then instantiates.
+hipError_t graph_capture(BatchMPContext *ctx, TreeGraph *g, const BatchMPArray &P0, const BatchMPArray &Q0,
+                         const BatchMPArray &R0) {
+    if (g->stream == nullptr) GPUPI_RETURN_IF(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    // the captured work follows the context stream's earlier work (the
+    // group stage that wrote the input rows)
+    hipEvent_t ready = nullptr;
+    GPUPI_RETURN_IF(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    hipError_t err = hipEventRecord(ready, ctx->stream);
+    if (err == hipSuccess) err = hipStreamWaitEvent(g->stream, ready, 0);
+    (void)hipEventDestroy(ready);
+    GPUPI_RETURN_IF(err);
+    return with_graph_context(ctx, g, [&]() -> hipError_t {
+        BatchMPArray P, Q, R;
+        for (int i = 0; i < 2; ++i) {
+            P = P0;
+            Q = Q0;
+            R = R0;
+            GPUPI_RETURN_IF(graph_body(ctx, g, P, Q, R));
+        }
+        GPUPI_RETURN_IF(hipStreamSynchronize(g->stream));
+        g->t.freeze(true);
+        gpupi::ntt_stats_suspend(true);
+        P = P0;
+        Q = Q0;
+        R = R0;
+        hipError_t body = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+        if (body == hipSuccess) {
+            body = graph_body(ctx, g, P, Q, R);
+            hipGraph_t graph = nullptr;
+            const hipError_t end = hipStreamEndCapture(g->stream, &graph);
+            g->graph = graph;
+            if (body == hipSuccess) body = end;
+        }
+        gpupi::ntt_stats_suspend(false);
+        if (body != hipSuccess) {
+            g->drop_graph();
+            return body;
+        }
+        GPUPI_RETURN_IF(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        g->P = P;
+        g->Q = Q;
+        g->R = R;
+        return hipStreamSynchronize(g->stream);
+    });
+}
+
+}  // namespace
+
 extern "C" {
 
 hipError_t gpupi_construct_leaf_17(BatchMPContext *ctx, int is_head, uint32_t *P, uint32_t *Q, uint32_t *R,
@@ -692,17 +836,81 @@ hipError_t gpupi_binary_split(BatchMPContext *ctx, uint32_t N, BatchMPArray *Q_o
         // compactions one by one
         GPUPI_RETURN_IF(leaf_stage(ctx, t, N, Lengths::kPerOp, P, Q, R));
         for (uint32_t n1 = N; n1 > 1; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, Lengths::kPerOp, P, Q, R));
-    } else {
-        // the 17-term group stage with bound lengths (no synchronization
-        // inside), then one compaction of each array
+    } else if (ctx->cache->pool() != nullptr || (ctx->profiler != nullptr && ctx->profiler->enabled)) {
+        // the batched form, every level launched and compacted in turn (a
+        // context on a memory pool keeps no buffers between runs; the stage
+        // profiler times each operation)
         GPUPI_RETURN_IF(leaf_stage_fused(ctx, t, N, P, Q, R));
         GPUPI_RETURN_IF(compact_many(ctx, {&P, &Q, &R}, "P,Q,R", 2 * N));
         for (uint32_t n1 = N; n1 > 1; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, Lengths::kShared, P, Q, R));
+    } else {
+        // the 17-term group stage with bound lengths (no synchronization
+        // inside) into the tree graph's input rows, one compaction of each
+        // array, the bottom levels replayed from the graph, one compaction,
+        // the remaining levels compacted level by level
+        if (ctx->tree_graph == nullptr) {
+            ctx->tree_graph = new (std::nothrow) TreeGraph();
+            if (ctx->tree_graph == nullptr) return hipErrorOutOfMemory;
+        }
+        TreeGraph *g = ctx->tree_graph;
+        if (g->N != N) {
+            g->drop_graph();
+            g->N = N;
+        }
+        GPUPI_RETURN_IF(g->in_P.view(N, 51, P));
+        GPUPI_RETURN_IF(g->in_Q.view(N, 68, Q));
+        GPUPI_RETURN_IF(g->in_R.view(N, 83, R));
+        GPUPI_RETURN_IF(stage(ctx, "leaf", "fused_group_stage", N * 17u, N, 51, 0, false,
+                              [&] { return gpupi::launch_group_stage(N, P.data, Q.data, R.data, ctx->stream); }));
+        GPUPI_RETURN_IF(compact_many(ctx, {&P, &Q, &R}, "P,Q,R", 2 * N));
+        const uint32_t levels = graph_levels(N, P.length, Q.length, R.length);
+        uint32_t n1 = N;
+        if (levels > 0) {
+            if (g->exec == nullptr || g->LP != P.length || g->LQ != Q.length || g->LR != R.length ||
+                g->levels != levels || g->ntt_min != ctx->ntt_min_limbs) {
+                g->drop_graph();
+                g->LP = P.length;
+                g->LQ = Q.length;
+                g->LR = R.length;
+                g->levels = levels;
+                g->ntt_min = ctx->ntt_min_limbs;
+                GPUPI_RETURN_IF(graph_capture(ctx, g, P, Q, R));
+            }
+            GPUPI_RETURN_IF(hipGraphLaunch(g->exec, ctx->stream));
+            P = g->P;
+            Q = g->Q;
+            R = g->R;
+            GPUPI_RETURN_IF(compact_many(ctx, {&P, &Q, &R}, "P,Q,R", n1 >> (levels - 1)));
+            n1 >>= levels;
+        }
+        for (; n1 > 1; n1 >>= 1) GPUPI_RETURN_IF(tree_level(ctx, t, N, n1, Lengths::kShared, P, Q, R));
+        if (t.bQ.ptr == nullptr) {
+            // no level ran on this call's buffers (a single group): the rows
+            // are the graph's input rows; copy them out for the caller
+            BatchMPArray Qc, Rc;
+            GPUPI_RETURN_IF(t.bQ.view(1, Q.length, Qc));
+            GPUPI_RETURN_IF(t.bR.view(1, R.length, Rc));
+            GPUPI_RETURN_IF(hipMemcpyAsync(Qc.data, Q.data, (size_t)Q.length * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            GPUPI_RETURN_IF(hipMemcpyAsync(Rc.data, R.data, (size_t)R.length * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            Q = Qc;
+            R = Rc;
+        }
     }
     // Q and R now hold one row each; hand their buffers to the caller.
     *Q_out = view_of(t.bQ.release(), Q.length, 1, Q.stride);
     *R_out = view_of(t.bR.release(), R.length, 1, R.stride);
     return hipSuccess;
+}
+
+// Limbs that hold p_j (or q_j) after j Pell doublings from (4001, 40):
+// p_j + q_j sqrt(10005) = x^(2^j) with x = 4001 + 40 sqrt(10005) and
+// p_j^2 - 10005 q_j^2 = 1, so p_j < x^(2^j) and q_j < x^(2^j) / sqrt(10005);
+// one limb of margin covers the rounding of the double-precision logarithm.
+static uint32_t sqrt_limbs_bound(uint32_t doublings, bool q) {
+    const double log2x = std::log2(4001.0 + 40.0 * std::sqrt(10005.0));
+    double bits = std::ldexp(log2x, (int)doublings);
+    if (q) bits -= 0.5 * std::log2(10005.0);
+    return (uint32_t)(std::ceil(bits / 32.0)) + 1u;
 }
 
 // pi_bs_gpu.cu:1346-1522
@@ -718,14 +926,17 @@ hipError_t gpupi_sqrt_10005(BatchMPContext *ctx, int target_digits, BatchMPArray
     GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
     const bool plain = ctx->pipeline_form == GPUPI_FORM_PLAIN;
     double valid_digits = 7.5;
+    uint32_t doublings = 0;
     while (valid_digits < target_digits) {
         valid_digits *= 2;
+        ++doublings;
         const uint32_t sd = (uint32_t)valid_digits;
         BatchMPArray PQ, Qn, PP, QQ, QQ5, Pn;
-        // batched form: intermediate results keep their bound lengths (their
-        // top limbs may be zero); only the new P and Q are compacted, in one
-        // round trip.  Plain form: every result compacted as the reference
-        // does (pi_bs_gpu.cu:1443-1515).
+        // batched form: every result keeps a length known on the host, so
+        // nothing synchronizes: products and sums their allocation lengths,
+        // the new P and Q the bound sqrt_limbs_bound (the values are the same
+        // integers; the top limbs may be zero).  Plain form: every result
+        // compacted as the reference does (pi_bs_gpu.cu:1443-1515).
         GPUPI_RETURN_IF(bPQ.view(1, P.length + Q.length, PQ));
         GPUPI_RETURN_IF(p_mul(ctx, "P*Q", sd, P, Q, PQ));
         if (plain) GPUPI_RETURN_IF(compact(ctx, PQ, "P*Q", sd));
@@ -743,13 +954,19 @@ hipError_t gpupi_sqrt_10005(BatchMPContext *ctx, int target_digits, BatchMPArray
         if (plain) GPUPI_RETURN_IF(compact(ctx, QQ5, "10005*Q*Q", sd));
         GPUPI_RETURN_IF(bPn.view(1, std::max(PP.length, QQ5.length) + 1, Pn));
         GPUPI_RETURN_IF(p_add(ctx, "P*P+10005*Q*Q", sd, PP, QQ5, Pn));
-        if (plain) GPUPI_RETURN_IF(compact(ctx, Pn, "P", sd));
-        else GPUPI_RETURN_IF(compact_many(ctx, {&Pn, &Qn}, "P,Q", sd));
+        if (plain) {
+            GPUPI_RETURN_IF(compact(ctx, Pn, "P", sd));
+        } else {
+            Pn.length = std::min(Pn.length, sqrt_limbs_bound(doublings, false));
+            Qn.length = std::min(Qn.length, sqrt_limbs_bound(doublings, true));
+        }
         bP.swap(bPn);
         bQ.swap(bQn);
         P = Pn;
         Q = Qn;
     }
+    // the reference's results are compacted (one round trip here for both)
+    if (!plain) GPUPI_RETURN_IF(compact_many(ctx, {&P, &Q}, "P,Q", (uint32_t)valid_digits));
     *P_out = view_of(bP.release(), P.length, 1, P.stride);
     *Q_out = view_of(bQ.release(), Q.length, 1, Q.stride);
     return hipSuccess;

@@ -113,6 +113,22 @@ def test_repeat_run_reuses_cached_blocks():
     assert np.array_equal(ret1, oret) and dec1 == odec
 
 
+def test_tree_graph_replays_and_recaptures():
+    # the bottom tree levels run from a hipGraph captured on first use and
+    # replayed after; a new group count or crossover captures anew
+    sizes = [(300000, None), (300000, None), (120000, None), (300000, 256), (300000, None), (5000, None)]
+    with g.Context() as c:
+        for digits, threshold in sizes:
+            if threshold is not None:
+                g.check(c.set_ntt_threshold(threshold))
+            ret, dec, _ = c.pi(digits)
+            if threshold is not None:
+                g.check(c.set_ntt_threshold(384))
+            oret, odec, _ = of.pi_digits(digits)
+            assert np.array_equal(ret, oret), (digits, threshold)
+            assert dec == odec, (digits, threshold)
+
+
 @pytest.mark.parametrize("digits", [1, 64, 763, 768, 20000])
 def test_pi_digits_match_golden_fixture(ctx, digits):
     # tests/golden/pi_digits.txt: pi from an independent implementation
