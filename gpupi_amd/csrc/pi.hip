@@ -1201,7 +1201,13 @@ namespace {
 
 // 5^leaf_digits and its repeated squares: powers[l] = 5^(leaf_digits 2^l)
 // for l < levels, row l in the block of bufs[l] (convert_decimal.cu:198-269).
-hipError_t fill_powers(BatchMPContext *ctx, uint32_t leaf_digits, uint32_t levels, BatchMPArray *powers, Buf *bufs) {
+// With host_lengths, each power keeps a length fixed on the host (at least
+// its limbs: 5^e has floor(e log2 5) + 1 bits, one limb of margin for the
+// rounding of the logarithm) instead of a synchronizing compaction; the
+// conversion reads bits at positions fixed by the digit counts, so a power's
+// zero top limbs change nothing.
+hipError_t fill_powers(BatchMPContext *ctx, uint32_t leaf_digits, uint32_t levels, BatchMPArray *powers, Buf *bufs,
+                       bool host_lengths) {
     // 5^leaf_digits on the host (at most ~1400 bits), then squarings.
     Limbs five(1, 1);
     for (uint32_t k = 0; k < leaf_digits; ++k) {
@@ -1219,7 +1225,12 @@ hipError_t fill_powers(BatchMPContext *ctx, uint32_t leaf_digits, uint32_t level
     for (uint32_t l = 1; l < levels; ++l) {
         GPUPI_RETURN_IF(bufs[l].view(1, powers[l - 1].length * 2, powers[l]));
         GPUPI_RETURN_IF(batch_mp_mul(ctx, powers[l - 1], powers[l - 1], powers[l]));
-        GPUPI_RETURN_IF(batch_mp_array_compact(ctx, &powers[l]));
+        if (host_lengths) {
+            const double bits = std::ldexp((double)leaf_digits * std::log2(5.0), (int)l) + 1.0;
+            powers[l].length = std::min(powers[l].length, (uint32_t)std::ceil(bits / 32.0) + 1u);
+        } else {
+            GPUPI_RETURN_IF(batch_mp_array_compact(ctx, &powers[l]));
+        }
     }
     return hipSuccess;
 }
@@ -1236,7 +1247,8 @@ struct CachedPowers {
         if (!bufs) return hipErrorOutOfMemory;
         for (uint32_t l = 0; l < levels; ++l) bufs[l].ctx = ctx;
         arrays.assign(levels, BatchMPArray{});
-        GPUPI_RETURN_IF(fill_powers(ctx, leaf_digits, levels, arrays.data(), bufs.get()));
+        GPUPI_RETURN_IF(fill_powers(ctx, leaf_digits, levels, arrays.data(), bufs.get(),
+                                    ctx->pipeline_form != GPUPI_FORM_PLAIN));
         table.powers = arrays.data();
         table.levels = levels;
         return hipSuccess;
@@ -1263,7 +1275,7 @@ hipError_t convert_decimal_precompute_powers(BatchMPContext *ctx, ConvertDecimal
         return hipErrorOutOfMemory;
     }
     for (uint32_t i = 0; i < levels; ++i) powers[i] = BatchMPArray{};
-    const hipError_t err = fill_powers(ctx, leaf_digits, levels, powers, bufs.get());
+    const hipError_t err = fill_powers(ctx, leaf_digits, levels, powers, bufs.get(), false);
     if (err != hipSuccess) {
         delete[] powers;
         return err;
