@@ -236,14 +236,24 @@ __device__ __forceinline__ void column_twiddles(double *x, const ColTw &tw, Mod 
     }
 }
 
-// Garner recombination of centred residues -> exact coefficient (< 2^98).
+// An integral double in [0, 2^52) as an integer: adding 2^52 puts its value
+// in the mantissa bits (exact), which one integer subtraction reads out.
+__device__ __forceinline__ uint64_t small_double_to_u64(double x) {
+    return (uint64_t)__double_as_longlong(x + 0x1p52) - (uint64_t)__double_as_longlong(0x1p52);
+}
+
+// Garner recombination of centred residues -> exact coefficient (< 2^98):
+// r0 + p0 t with t = (r1 - r0) / p0 mod p1; p0 t is formed from shifts, since
+// p0 = 2^49 - 2^30 + 1.
 __device__ __forceinline__ unsigned __int128 recombine(double v0, double v1) {
+    static_assert(kPrime0 == (1ull << 49) - (1ull << 30) + 1ull, "p0 = 2^49 - 2^30 + 1");
     const Mod m1 = mod_of(1);
     const double r0 = v0 < 0.0 ? v0 + kP0 : v0;
     const double r1 = v1 < 0.0 ? v1 + kP1 : v1;
     double t = mulp(r1 - r0, c_ntt.garner, m1);
     t = t < 0.0 ? t + kP1 : t;
-    return (unsigned __int128)kPrime0 * (uint64_t)t + (uint64_t)r0;
+    const unsigned __int128 T = small_double_to_u64(t);
+    return (T << 49) - (T << 30) + T + small_double_to_u64(r0);
 }
 
 // Forward operand o of a multi-level launch: limbs of row i at X + i s (the
