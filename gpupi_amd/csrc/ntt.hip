@@ -205,33 +205,69 @@ __device__ __forceinline__ void inv_dft(double *v, double *lds, const double *ta
     }
 }
 
-// Table entries for one thread's column twiddles: w^(cg b0) and w^cg as
-// lo/hi pairs of the two-level table (w_{2^30}^E = lo[E mod 2^15] *
-// hi[E >> 15], E < 2^30), loaded early (they depend only on the tile geometry) so their
-// latency overlaps the data loads.
+constexpr uint32_t brev_c(uint32_t x, int bits) {
+    uint32_t r = 0;
+    for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
+    return r;
+}
+
+// Column twiddles are applied in the last in-tile step's layout: value
+// k = u R + j (R = 2^LR the last step's radix) takes w^(cg E) with
+// E = B0 + bitrev(u, 4 - LR) + bitrev(j, LR) 2^(M - LR) (cg: the column's
+// exponent, B0: the thread's row base).  Table entries, as lo/hi pairs of
+// the two-level table (w_{2^30}^E = lo[E mod 2^15] * hi[E >> 15],
+// E < 2^30), are loaded early (they depend only on the tile geometry) so
+// their latency overlaps the data loads: w^(cg B0) and the step
+// w^(cg 2^(M - LR)) between j's when a thread holds one small DFT (M = 8),
+// else w^cg, the step between u's.
 struct ColTw {
     double lb, hb, ls, hs;
 };
-__device__ __forceinline__ ColTw col_tw_load(const double *tab, uint32_t cg, uint32_t b0, bool inv) {
+template <int M>
+__device__ __forceinline__ ColTw col_tw_load(const double *tab, uint32_t cg, uint32_t B0, bool inv) {
+    constexpr int LR = col_last_lr(M);
     const double *lo = tab + (inv ? kOffLoInv : kOffLo), *hi = tab + (inv ? kOffHiInv : kOffHi);
-    const uint32_t eb = cg * b0;
-    return ColTw{lo[eb & 32767u], hi[eb >> 15], lo[cg & 32767u], hi[cg >> 15]};
+    const uint32_t eb = cg * B0;
+    const uint32_t es = (16 >> LR) == 1 ? cg << (M - LR) : cg;
+    return ColTw{lo[eb & 32767u], hi[eb >> 15], lo[es & 32767u], hi[es >> 15]};
 }
 
-// Multiplies x[u] (u < 16) by w^(cg * (b0 + bitrev(u, 4))): base = w^(cg b0),
-// step = w^cg, and two interleaved chains base * step^v.
+// Multiplies value k of x by its column twiddle (layout above): chains
+// base * su^a * sj^b over a = bitrev(u), b = bitrev(j); two interleaved
+// chains when the thread holds one small DFT of 16.
+template <int M>
 __device__ __forceinline__ void column_twiddles(double *x, const ColTw &tw, Mod m) {
+    constexpr int LR = col_last_lr(M), R = 1 << LR, PER = 16 / R;
     const double base = mulp(tw.lb, tw.hb, m);
     const double step = mulp(tw.ls, tw.hs, m);
-    const double step2 = mulp(step, step, m);
-    double c0 = base, c1 = mulp(base, step, m);
+    if constexpr (PER == 1) {
+        const double step2 = mulp(step, step, m);
+        double c0 = base, c1 = mulp(base, step, m);
 #pragma unroll
-    for (int v = 0; v < 16; v += 2) {
-        x[brev4(v)] = mulp(x[brev4(v)], c0, m);
-        x[brev4(v + 1)] = mulp(x[brev4(v + 1)], c1, m);
-        if (v + 2 < 16) {
-            c0 = mulp(c0, step2, m);
-            c1 = mulp(c1, step2, m);
+        for (int b = 0; b < 16; b += 2) {
+            x[brev4(b)] = mulp(x[brev4(b)], c0, m);
+            x[brev4(b + 1)] = mulp(x[brev4(b + 1)], c1, m);
+            if (b + 2 < 16) {
+                c0 = mulp(c0, step2, m);
+                c1 = mulp(c1, step2, m);
+            }
+        }
+    } else {
+        double sj = step;  // w^(cg 2^(M - LR))
+#pragma unroll
+        for (int i = 0; i < M - LR; ++i) sj = mulp(sj, sj, m);
+        double cu = base;
+#pragma unroll
+        for (int a = 0; a < PER; ++a) {
+            const int u = (int)brev_c((uint32_t)a, 4 - LR);
+            double c = cu;
+#pragma unroll
+            for (int b = 0; b < R; ++b) {
+                const int k = u * R + (int)brev_c((uint32_t)b, LR);
+                x[k] = mulp(x[k], c, m);
+                if (b + 1 < R) c = mulp(c, sj, m);
+            }
+            if (a + 1 < PER) cu = mulp(cu, step, m);
         }
     }
 }
@@ -349,16 +385,21 @@ struct ColPair {
 };
 
 // Per-thread geometry of a column tile (level `level`, DFT size 2^M): the
-// thread (tile-local index tid) holds elements tid + 256 u, exactly its
-// step-0 elements.
+// thread (tile-local index tid) holds elements tid + 256 u (u < 16), its
+// step-0 elements, at the start of a forward level and the end of an
+// inverse one, and its last-step values k (rows col_last_row0 +
+// col_last_row_delta(k) of its column) at the end of a forward level and
+// the start of an inverse one.
 template <int M>
 struct ColThread {
     uint64_t off0;     // element (coefficient) index of u = 0 within the product
     uint32_t ustep;    // log2 of the index step between u and u + 1
     uint64_t poff0;    // plane position of u = 0 (padded planes)
     uint64_t pstep;    // plane position step between u and u + 1
+    uint64_t plast0;   // plane position of last-step value k = 0
+    uint64_t prow;     // plane position step between rows of the tile
     uint32_t cg;       // (col0 + g) << (30 - lk): twiddle exponent per bitrev(r)
-    uint32_t b0;       // bitrev(r0, M)
+    uint32_t B0;       // bitrev of the last step's row base (col_last_b0)
     uint32_t lds0;     // LDS slot of u = 0
     uint32_t tid;      // index within the tile's 256 threads
 
@@ -367,7 +408,6 @@ struct ColThread {
         tid = tid_;
         const ColTile ct = col_tile(p, level, t);
         const uint32_t g = tid & ((1u << LC) - 1u);
-        const uint32_t r0 = tid >> LC;
         off0 = col_thread_offset(p, level, ct, tid);
         ustep = (uint32_t)col_row_step_log2(p, level);
         poff0 = plane_index(p, off0);
@@ -375,32 +415,34 @@ struct ColThread {
         // it is at least one row long, and none otherwise
         const int s = plane_pad_shift(p);
         pstep = (1ull << ustep) + ((s && (int)ustep >= s) ? ((uint64_t)kPlanePad << (ustep - s)) : 0ull);
+        plast0 = plane_index(p, col_offset(p, level, ct, g, col_last_row0(M, tid)));
+        prow = col_plane_row_step(p, level);
         cg = (ct.col0 + g) << (kMaxLog2 - slab_log2(p, level));
-        b0 = __brev(r0) >> (32 - M);
+        B0 = (__brev(tid >> LC) >> (32 - (M - 4))) << (4 - col_last_lr(M));
         lds0 = lds_index(tid);
     }
     // coefficient index of element u (limbs)
     __device__ __forceinline__ uint64_t off(int u) const { return off0 + ((uint64_t)u << ustep); }
     // plane position of element u (residues)
     __device__ __forceinline__ uint64_t poff(int u) const { return poff0 + (uint64_t)u * pstep; }
+    // plane position of last-step value k
+    __device__ __forceinline__ uint64_t plast(int k) const {
+        return plast0 + (uint64_t)col_last_row_delta(M, k) * prow;
+    }
 };
 
 // One prime of a forward column level on the thread's step-0 elements v:
 // DFT down the columns, then the inter-level twiddle; v is left holding the
-// elements to store at the same offsets.  Ends with a barrier, so the next
-// prime may reuse `lds`.
-template <int M, int P>
+// thread's last-step values (stored at ct.plast(k)).  With BARRIER it ends
+// with a barrier, so the next prime may reuse `lds`.
+template <int M, int P, bool BARRIER>
 __device__ __forceinline__ void col_fwd_prime(double *v, double *lds, const ColThread<M> &ct, const double *tables,
                                               const ColTw &tw) {
     const Mod m = mod_of(P);
     const double *tab = tables + P * kPrimeTable;
     fwd_dft<M, true, kDifToTwiddle>(v, lds, tab, P, m, ct.tid);
-    lds_store(lds, v, StepMap<M, step_count(M) - 1, true>(ct.tid));
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = lds[ct.lds0 + 272u * u];
-    column_twiddles(v, tw, m);
-    __syncthreads();
+    column_twiddles<M>(v, tw, m);
+    if (BARRIER) __syncthreads();
 }
 
 // Forward column level: blocks cover (product, operand, tile); both primes
@@ -441,25 +483,17 @@ __global__ void __launch_bounds__((ColPair<M, FROM_LIMBS ? kPairFwdLimbs : kPair
     } else {
 #pragma unroll
         for (int u = 0; u < 16; ++u) v0[u] = T0[ct.poff(u)];
-    }
-    const ColTw tw0 = col_tw_load(a.tables, ct.cg, ct.b0, false);
-    const ColTw tw1 = col_tw_load(a.tables + kPrimeTable, ct.cg, ct.b0, false);
-    col_fwd_prime<M, 0>(v0, lds, ct, a.tables, tw0);
-    if (!FROM_LIMBS) {
-        // register budget (occupancy 4 needs <= 128 VGPRs): prime 0 goes out
-        // before prime 1 is loaded and transformed
-#pragma unroll
-        for (int u = 0; u < 16; ++u) T0[ct.poff(u)] = v0[u];
 #pragma unroll
         for (int u = 0; u < 16; ++u) v1[u] = T1[ct.poff(u)];
     }
-    col_fwd_prime<M, 1>(v1, lds, ct, a.tables, tw1);
-    if (FROM_LIMBS) {
+    const ColTw tw0 = col_tw_load<M>(a.tables, ct.cg, ct.B0, false);
+    const ColTw tw1 = col_tw_load<M>(a.tables + kPrimeTable, ct.cg, ct.B0, false);
+    col_fwd_prime<M, 0, true>(v0, lds, ct, a.tables, tw0);
+    col_fwd_prime<M, 1, false>(v1, lds, ct, a.tables, tw1);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) T0[ct.poff(u)] = v0[u];
-    }
+    for (int k = 0; k < 16; ++k) T0[ct.plast(k)] = v0[k];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) T1[ct.poff(u)] = v1[u];
+    for (int k = 0; k < 16; ++k) T1[ct.plast(k)] = v1[k];
 }
 
 // Step-0 elements of a row tile (DFT size 2^M): v[j] is tile index
@@ -568,36 +602,30 @@ __global__ void __launch_bounds__(256) ntt_row_pair_kernel(PassArgs a, uint32_t 
     }
 }
 
-// Inverse column level of prime P on the thread's elements v (as loaded):
-// inverse twiddle, inverse DFT; leaves the thread's elements tid + 256 u in
-// v.  The caller must make sure no thread still reads `lds`.
+// Inverse column level of prime P on the thread's last-step values v (as
+// loaded from ct.plast(k)): inverse twiddle, inverse DFT; leaves the
+// thread's elements tid + 256 u in v.  The caller must make sure no thread
+// still reads `lds`.
 template <int M, int P>
 __device__ __forceinline__ void col_inv_prime(double *v, double *lds, const ColThread<M> &ct, const double *tables,
                                               const ColTw &tw) {
-    constexpr int kLast = step_count(M) - 1;
     const Mod m = mod_of(P);
     const double *tab = tables + P * kPrimeTable;
-    column_twiddles(v, tw, m);
-#pragma unroll
-    for (int u = 0; u < 16; ++u) lds[ct.lds0 + 272u * u] = v[u];
-    __syncthreads();
-    lds_steps<M, true, true, kLast, 1>(lds, tab, P, m, ct.tid);
-    const StepMap<M, 0, true> s0(ct.tid);
-    lds_load(v, lds, s0);
-    step_regs<M, 0, true, true, kDitLast>(v, s0, tab, P, m);
+    column_twiddles<M>(v, tw, m);
+    inv_dft<M, true>(v, lds, tab, P, m, ct.tid);
 }
 
-// Loads both primes' elements of a column tile and their inverse column
+// Loads both primes' values of a column tile and their inverse column
 // twiddle table entries (every load before any store).
 template <int M>
 __device__ __forceinline__ void col_inv_load(double *v0, double *v1, ColTw &tw0, ColTw &tw1, const PassArgs &a,
                                              const ColThread<M> &ct, const double *T0, const double *T1) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v0[u] = T0[ct.poff(u)];
+    for (int k = 0; k < 16; ++k) v0[k] = T0[ct.plast(k)];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v1[u] = T1[ct.poff(u)];
-    tw0 = col_tw_load(a.tables, ct.cg, ct.b0, true);
-    tw1 = col_tw_load(a.tables + kPrimeTable, ct.cg, ct.b0, true);
+    for (int k = 0; k < 16; ++k) v1[k] = T1[ct.plast(k)];
+    tw0 = col_tw_load<M>(a.tables, ct.cg, ct.B0, true);
+    tw1 = col_tw_load<M>(a.tables + kPrimeTable, ct.cg, ct.B0, true);
 }
 
 // Inverse column level above the first: both primes through one tile,
@@ -612,22 +640,25 @@ __global__ void __launch_bounds__((ColPair<M, kPairInv>::kThreads)) ntt_col_inv_
     const ColThread<M> ct(a.plan, level, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
     double *T0 = a.T + oa.base + (size_t)bp.row * plane_row_elems(a.plan);
     double *T1 = T0 + oa.pstride;
-    // register budget (occupancy 4 needs <= 128 VGPRs): one prime's
-    // elements at a time
-    const ColTw tw0 = col_tw_load(a.tables, ct.cg, ct.b0, true);
-    const ColTw tw1 = col_tw_load(a.tables + kPrimeTable, ct.cg, ct.b0, true);
-    double v[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = T0[ct.poff(u)];
-    col_inv_prime<M, 0>(v, lds, ct, a.tables, tw0);
-#pragma unroll
-    for (int u = 0; u < 16; ++u) T0[ct.poff(u)] = v[u];
+    double v0[16], v1[16];
+    ColTw tw0, tw1;
+    col_inv_load<M>(v0, v1, tw0, tw1, a, ct, T0, T1);
+    col_inv_prime<M, 0>(v0, lds, ct, a.tables, tw0);
     __syncthreads();
+    col_inv_prime<M, 1>(v1, lds, ct, a.tables, tw1);
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = T1[ct.poff(u)];
-    col_inv_prime<M, 1>(v, lds, ct, a.tables, tw1);
+    for (int u = 0; u < 16; ++u) T0[ct.poff(u)] = v0[u];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) T1[ct.poff(u)] = v[u];
+    for (int u = 0; u < 16; ++u) T1[ct.poff(u)] = v1[u];
+}
+
+// x of lane - D inside the lane's row of 16 lanes (DPP row_shr; 0 where that
+// lane lies before the row): a VALU operand modifier, where a general
+// shuffle would go through the LDS crossbar.
+template <int D>
+__device__ __forceinline__ uint32_t row_shr(uint32_t x) {
+    static_assert(D >= 1 && D <= 15, "row_shr:1..15");
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + D, 0xf, 0xf, true);
 }
 
 // First inverse column level: both primes through one tile, 1/K, Garner,
@@ -647,8 +678,9 @@ __global__ void __launch_bounds__((ColPair<M, kPairInvLimbs>::kThreads)) ntt_col
     constexpr int RL = run_log2_for(M);
     constexpr uint32_t kRun = 1u << RL;
     // a run (2^RL lanes aligned to 2^RL) lies inside one group of 2^LC
-    // adjacent lanes, which hold one row of one tile also in paired blocks
-    static_assert(kRun <= (1u << LC) && kRun <= kWave, "a run is one tile row inside one wave");
+    // adjacent lanes, which hold one row of one tile also in paired blocks,
+    // and inside one DPP row of 16 lanes (the neighbour shifts)
+    static_assert(kRun <= (1u << LC) && kRun <= 16u, "a run is one tile row inside one DPP row");
     const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan) >> CP::kLog2);
     const uint32_t prod = bp.row;
     const NttProd pr = a.prods[bp.seg];
@@ -683,10 +715,10 @@ __global__ void __launch_bounds__((ColPair<M, kPairInvLimbs>::kThreads)) ntt_col
         const unsigned __int128 c =
             recombine(mulp(v0[u], kinv0, mod_of(0)), mulp(lds[ct.lds0 + 272u * u], kinv1, mod_of(1)));
         const uint32_t w0 = (uint32_t)c, w1 = (uint32_t)(c >> 32), w2 = (uint32_t)(c >> 64);
-        const uint32_t w1a = __shfl_up(w1, 1), w2a = __shfl_up(w2, 1), w2b = __shfl_up(w2, 2);
+        const uint32_t w1a = row_shr<1>(w1), w2a = row_shr<1>(w2), w2b = row_shr<2>(w2);
         const uint64_t s = (uint64_t)w0 + (jr >= 1 ? w1a : 0u) + (jr >= 2 ? w2b : 0u);
         const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
-        const uint32_t hia = __shfl_up(hi, 1);
+        const uint32_t hia = row_shr<1>(hi);
         const uint64_t t = (uint64_t)lo + (jr >= 1 ? hia : 0u);
         const uint32_t lo2 = (uint32_t)t;
         const bool gen = (t >> 32) != 0, prop = lo2 == 0xffffffffu;

@@ -261,6 +261,40 @@ GPUPI_HD SmallDft small_dft(int m, int s, uint32_t q, bool col) {
     return d;
 }
 
+// Last in-tile step of a column tile (DFT size 2^lm): radix R = 2^lr with
+// lr = col_last_lr(lm), 16 / R small DFTs per thread.  Value k = u R + j of
+// tile-local thread tid is point j of its small DFT u: row
+// col_last_row0(lm, tid) + col_last_row_delta(lm, k) of column tid mod C.
+// After the forward DFT that row holds frequency bitrev(row, lm), the
+// multiplier of the inter-level twiddle, which splits as
+// B0 + bitrev(u, 4 - lr) + bitrev(j, lr) 2^(lm - lr) with
+// B0 = bitrev(tid >> lc, lm - 4) 2^(4 - lr).  Forward column levels store
+// their results and inverse column levels load their inputs in this layout:
+// for each k the lanes of a wave still cover the adjacent columns of a few
+// rows, and no LDS exchange to or from the step-0 layout is needed.
+GPUPI_HD constexpr int col_last_lr(int lm) { return step_log2radix(lm, step_count(lm) - 1); }
+GPUPI_HD constexpr uint32_t col_last_row_delta(int lm, int k) {
+    return ((uint32_t)(k >> col_last_lr(lm)) << (lm - 4 + col_last_lr(lm))) +
+           ((uint32_t)k & ((1u << col_last_lr(lm)) - 1u));
+}
+GPUPI_HD uint32_t col_last_row0(int lm, uint32_t tid) { return (tid >> (kTileLog2 - lm)) << col_last_lr(lm); }
+GPUPI_HD uint32_t col_last_b0(int lm, uint32_t tid) {
+    return bitrev(tid >> (kTileLog2 - lm), lm - 4) << (4 - col_last_lr(lm));
+}
+GPUPI_HD uint32_t col_last_brow(int lm, uint32_t tid, int k) {
+    const int lr = col_last_lr(lm);
+    const uint32_t u = (uint32_t)k >> lr, j = (uint32_t)k & ((1u << lr) - 1u);
+    return col_last_b0(lm, tid) + bitrev(u, 4 - lr) + (bitrev(j, lr) << (lm - lr));
+}
+// Plane positions between consecutive rows of a column tile at `level`
+// (first-level rows carry the plane padding, deeper levels stay inside one
+// padded row).
+GPUPI_HD uint64_t col_plane_row_step(const Plan &p, int level) {
+    const int rs = slab_log2(p, level) - p.log2M[level];
+    const int s = plane_pad_shift(p);
+    return (1ull << rs) + ((s && rs >= s) ? ((uint64_t)kPlanePad << (rs - s)) : 0ull);
+}
+
 // Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin:
 // block b runs on XCD b mod 8.  xcd_tile(b, n) renumbers blocks in groups
 // of 8 * kXcdChunk: within a group each XCD gets kXcdChunk consecutive

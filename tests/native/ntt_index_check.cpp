@@ -121,6 +121,24 @@ static void check_plan(const Plan &p, bool full) {
                     const uint32_t g = i & (C - 1), r = i >> (kTileLog2 - lm);
                     const uint64_t cg = (uint64_t)(ct.col0 + g) << (kMaxLog2 - lk);
                     EXPECT(cg < (1ull << kMaxLog2) && cg * bitrev(r, lm) < (1ull << kMaxLog2), "twiddle exp k=%d", k);
+                    // last-step layout: value k = i / 256 of thread tid is
+                    // the element the last in-tile step gives it, at a plane
+                    // position linear in its row, with twiddle multiplier
+                    // bitrev(row); the loaded step exponent stays below 2^30
+                    const int lr = col_last_lr(lm);
+                    const int kv = (int)u;
+                    const SmallDft d = small_dft(lm, step_count(lm) - 1, tid + 256u * (u >> lr), true);
+                    const uint32_t idx = d.base + (u & ((1u << lr) - 1u)) * d.stride;
+                    const uint32_t rl = col_last_row0(lm, tid) + col_last_row_delta(lm, kv);
+                    EXPECT(d.log2Ms == lr && idx == rl * C + (tid & (C - 1)), "last-step element k=%d l=%d", k, l);
+                    const uint64_t p0 = plane_index(p, col_offset(p, l, ct, tid & (C - 1), col_last_row0(lm, tid)));
+                    EXPECT(plane_index(p, col_offset(p, l, ct, tid & (C - 1), rl)) ==
+                               p0 + (uint64_t)col_last_row_delta(lm, kv) * col_plane_row_step(p, l),
+                           "last-step plane position k=%d l=%d", k, l);
+                    EXPECT(col_last_brow(lm, tid, kv) == bitrev(rl, lm), "last-step frequency k=%d l=%d", k, l);
+                    const uint64_t es = (16 >> lr) == 1 ? cg << (lm - lr) : cg;
+                    EXPECT(es < (1ull << kMaxLog2) && cg * col_last_b0(lm, tid) < (1ull << kMaxLog2),
+                           "last-step twiddle exponents k=%d", k);
                 } else {
                     EXPECT(off == (uint64_t)t * kTile + i, "row offset contiguous k=%d", k);
                     // a row tile stays contiguous in the padded plane
