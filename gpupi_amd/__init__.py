@@ -212,6 +212,10 @@ def _declare(lib):
         "gpupi_kernel_stats_query_sizes": (ctypes.c_int, [ctypes.POINTER(GpupiKernelStat), ctypes.c_int]),
         "gpupi_ntt_set_split": (err, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "gpupi_ntt_get_split": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+        "gpupi_ntt_set_launch": (err, [ctypes.c_uint32, ctypes.c_int]),
+        "gpupi_ntt_get_launch": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int)]),
+        "gpupi_group_stage_set_shape": (err, [ctypes.c_uint32]),
+        "gpupi_group_stage_get_shape": (ctypes.c_uint32, []),
         "gpupi_stage_profile_enable": (err, [c_ctx, ctypes.c_int]),
         "gpupi_set_pipeline_form": (err, [c_ctx, ctypes.c_int]),
         "gpupi_cache_query": (err, [c_ctx, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(u64)]),
@@ -601,6 +605,34 @@ def ntt_set_split(log2_k: int, log2_m0: int, log2_m1: int = 0) -> None:
     e = lib().gpupi_ntt_set_split(log2_k, log2_m0, log2_m1)
     if e != 0:
         raise ValueError(f"invalid NTT split 2^{log2_k} = 2^{log2_m0} x 2^{log2_m1} x rest (hip error {e})")
+
+
+def ntt_set_launch(pair_mask: int, xcd_chunk_log2: int) -> None:
+    """NTT launch shape: column level kinds with paired tiles (bit 0 first
+    forward, 1 second forward, 2 second inverse, 3 first inverse level) and
+    the log2 of the tiles each XCD takes in turn (0..8); raises ValueError
+    outside those ranges."""
+    e = lib().gpupi_ntt_set_launch(pair_mask, xcd_chunk_log2)
+    if e != 0:
+        raise ValueError(f"invalid NTT launch shape pair_mask={pair_mask} xcd_chunk_log2={xcd_chunk_log2}")
+
+
+def ntt_get_launch() -> tuple:
+    """(pair_mask, xcd_chunk_log2) in use."""
+    m, c = ctypes.c_uint32(), ctypes.c_int()
+    lib().gpupi_ntt_get_launch(ctypes.byref(m), ctypes.byref(c))
+    return m.value, c.value
+
+
+def group_stage_set_shape(groups_per_block: int) -> None:
+    """Groups per block of the fused group-stage kernel (1, 2, 4, 8, 16)."""
+    e = lib().gpupi_group_stage_set_shape(groups_per_block)
+    if e != 0:
+        raise ValueError(f"invalid group-stage shape {groups_per_block}")
+
+
+def group_stage_get_shape() -> int:
+    return int(lib().gpupi_group_stage_get_shape())
 
 
 def ntt_get_split(log2_k: int) -> tuple:

@@ -16,14 +16,20 @@
 // with two neighbour shifts and one ballot carry resolution per 64 columns.
 // Carries never cross from one product into the next because every product
 // fits its slot; the shifted add/sub of R works the same way.
+#include <atomic>
+
 #include "common.hpp"
+#include "gpupi_pi.h"
 #include "exact_div.hpp"
 #include "leaf_terms.hpp"
 
 namespace gpupi {
 namespace {
 
-constexpr uint32_t kGroupsPerBlock = 4;  // one wave each
+// Groups per block (one wave each): a launch parameter
+// (gpupi_group_stage_set_shape), 4 built in.
+constexpr uint32_t kGroupsPerBlockDefault = 4;
+std::atomic<uint32_t> g_groups_per_block{kGroupsPerBlockDefault};
 
 // LDS words per group: leaf terms (16 x 13) | head term (13) | two product
 // areas | two R areas.
@@ -153,10 +159,11 @@ __device__ __forceinline__ void exact_div_block(uint32_t *a, int n) {
     }
 }
 
-__global__ void __launch_bounds__(256) group_stage_kernel(uint32_t N, uint32_t *P, uint32_t *Q, uint32_t *R) {
-    __shared__ uint32_t smem[kGroupsPerBlock * kGroupWords];
+template <uint32_t GPB>
+__global__ void __launch_bounds__(GPB * kWave) group_stage_kernel(uint32_t N, uint32_t *P, uint32_t *Q, uint32_t *R) {
+    __shared__ uint32_t smem[GPB * kGroupWords];
     const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const uint32_t g = blockIdx.x * kGroupsPerBlock + wave;
+    const uint32_t g = blockIdx.x * GPB + wave;
     const bool live = g < N;
     uint32_t *sm = smem + wave * kGroupWords;
     // leaf terms: body i -> term 17 g + i, head -> term 17 g + 16
@@ -202,9 +209,32 @@ __global__ void __launch_bounds__(256) group_stage_kernel(uint32_t N, uint32_t *
 
 hipError_t launch_group_stage(uint32_t N, uint32_t *P, uint32_t *Q, uint32_t *R, hipStream_t stream) {
     if (N == 0) return hipSuccess;
-    const uint32_t blocks = ceil_div(N, kGroupsPerBlock);
-    hipLaunchKernelGGL(group_stage_kernel, dim3(blocks), dim3(256), 0, stream, N, P, Q, R);
+    const uint32_t gpb = g_groups_per_block.load(std::memory_order_relaxed);
+    const auto launch = [&](auto kernel, uint32_t per_block) {
+        hipLaunchKernelGGL(kernel, dim3(ceil_div(N, per_block)), dim3(per_block * kWave), 0, stream, N, P, Q, R);
+    };
+    switch (gpb) {
+        case 1: launch(group_stage_kernel<1>, 1); break;
+        case 2: launch(group_stage_kernel<2>, 2); break;
+        case 8: launch(group_stage_kernel<8>, 8); break;
+        case 16: launch(group_stage_kernel<16>, 16); break;
+        default: launch(group_stage_kernel<4>, 4); break;
+    }
     return hipGetLastError();
 }
 
 }  // namespace gpupi
+
+extern "C" {
+
+hipError_t gpupi_group_stage_set_shape(uint32_t groups_per_block) {
+    if (groups_per_block != 1 && groups_per_block != 2 && groups_per_block != 4 && groups_per_block != 8 &&
+        groups_per_block != 16)
+        return hipErrorInvalidValue;
+    gpupi::g_groups_per_block.store(groups_per_block);
+    return hipSuccess;
+}
+
+uint32_t gpupi_group_stage_get_shape(void) { return gpupi::g_groups_per_block.load(); }
+
+}  // extern "C"

@@ -334,6 +334,8 @@ struct PassArgs {
     // ragged launches: per-product operands and lengths (device array), or
     // null when the operands above describe every product
     const GpupiMulItem *items;
+    // log2 of the tiles each XCD takes in turn (xcd_tile)
+    int xcd_log2;
 };
 
 // The (segment, row, tile) a block works on: segments are operands (forward
@@ -341,8 +343,8 @@ struct PassArgs {
 struct BlockPos {
     uint32_t seg, row, t;
 };
-__device__ __forceinline__ BlockPos block_pos(const uint32_t *start, uint32_t nseg, uint32_t tiles) {
-    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ BlockPos block_pos(const uint32_t *start, uint32_t nseg, uint32_t tiles, int xcd_log2) {
+    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x, xcd_log2);
     uint32_t seg = 0, first = 0;
     for (uint32_t i = 1; i < nseg; ++i)
         if (bid >= start[i]) {
@@ -364,19 +366,18 @@ __device__ __forceinline__ BlockPos block_pos(const uint32_t *start, uint32_t ns
 // first forward level gains (195 -> 191 ms) and the second inverse level a
 // little (64.4 -> 63.0 ms); the second forward level loses 2 ms and the
 // first inverse level 13 ms (barriers over eight waves instead of four), so
-// only those two pair their tiles.  Bit 0: first forward level, bit 1:
-// second forward level, bit 2: second inverse level, bit 3: first inverse
-// level.
-#ifndef GPUPI_COL_PAIR_KINDS
-#define GPUPI_COL_PAIR_KINDS 5
-#endif
-constexpr bool pair_kind(int bit) { return ((GPUPI_COL_PAIR_KINDS) >> bit) & 1; }
+// by default only those two pair their tiles; with the last-step layout
+// (after the measurements above) the choice still holds: first inverse
+// level paired +11 ms per 10^9 step, the others within 2 ms.  The choice is
+// a launch parameter per level kind (gpupi_ntt_set_launch, the tuner's
+// --launch): bit 0 first forward level, bit 1 second forward level, bit 2
+// second inverse level, bit 3 first inverse level.
+constexpr uint32_t kPairMaskDefault = 5;
 constexpr int kPairFwdLimbs = 0, kPairFwd = 1, kPairInv = 2, kPairInvLimbs = 3;
-constexpr int pair_log2_for(int kind, int lm) { return pair_kind(kind) ? col_pair_log2(lm) : 0; }
 
-template <int M, int KIND>
+template <int M, bool PAIR>
 struct ColPair {
-    static constexpr int kLog2 = pair_log2_for(KIND, M);
+    static constexpr int kLog2 = PAIR ? col_pair_log2(M) : 0;
     static constexpr int kTiles = 1 << kLog2;
     static constexpr int kThreads = 256 * kTiles;
     static constexpr int LC = kTileLog2 - M;
@@ -453,13 +454,12 @@ __device__ __forceinline__ void col_fwd_prime(double *v, double *lds, const ColT
 // in-order counter, so a load issued after a store also waits for the
 // store.  Above level 0 the second prime's data is loaded after the first
 // prime's store, which keeps the kernel within 128 VGPRs.
-template <bool FROM_LIMBS, int M>
-__global__ void __launch_bounds__((ColPair<M, FROM_LIMBS ? kPairFwdLimbs : kPairFwd>::kThreads))
-    ntt_col_fwd_kernel(PassArgs a, int level) {
-    using CP = ColPair<M, FROM_LIMBS ? kPairFwdLimbs : kPairFwd>;
+template <bool FROM_LIMBS, int M, bool PAIR>
+__global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_kernel(PassArgs a, int level) {
+    using CP = ColPair<M, PAIR>;
     __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
     double *const lds = lds_tiles + CP::half() * kLdsSlots;
-    const BlockPos bp = block_pos(a.op_start, a.n_ops, tiles_per_product(a.plan) >> CP::kLog2);
+    const BlockPos bp = block_pos(a.op_start, a.n_ops, tiles_per_product(a.plan) >> CP::kLog2, a.xcd_log2);
     const NttOp o = a.ops[bp.seg];
     const ColThread<M> ct(a.plan, level, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
     double *T0 = a.T + o.base + (size_t)bp.row * plane_row_elems(a.plan);
@@ -510,7 +510,7 @@ template <int M>
 __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     __shared__ double lds[kLdsSlots];
     constexpr uint32_t kJ = 1u << (M - 4);
-    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan));
+    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan), a.xcd_log2);
     const NttProd pr = a.prods[bp.seg];
     const NttOp oa = a.ops[pr.a], ob = a.ops[pr.b];
     const uint64_t Kp = plane_row_elems(a.plan);
@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(256) ntt_row_pair_kernel(PassArgs a, uint32_t 
     __shared__ double lds[kLdsSlots];
     constexpr uint32_t kJ = 1u << (M - 4);
     const uint32_t tiles = tiles_per_product(a.plan);
-    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x, a.xcd_log2);
     const uint32_t row = bid / tiles, t = bid % tiles;
     const NttOp ob = a.ops[a.prods[0].b];
     const uint64_t Kp = plane_row_elems(a.plan);
@@ -630,12 +630,12 @@ __device__ __forceinline__ void col_inv_load(double *v0, double *v1, ColTw &tw0,
 
 // Inverse column level above the first: both primes through one tile,
 // stored together at the end.
-template <int M>
-__global__ void __launch_bounds__((ColPair<M, kPairInv>::kThreads)) ntt_col_inv_kernel(PassArgs a, int level) {
-    using CP = ColPair<M, kPairInv>;
+template <int M, bool PAIR>
+__global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_inv_kernel(PassArgs a, int level) {
+    using CP = ColPair<M, PAIR>;
     __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
     double *const lds = lds_tiles + CP::half() * kLdsSlots;
-    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan) >> CP::kLog2);
+    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan) >> CP::kLog2, a.xcd_log2);
     const NttOp oa = a.ops[a.prods[bp.seg].a];
     const ColThread<M> ct(a.plan, level, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
     double *T0 = a.T + oa.base + (size_t)bp.row * plane_row_elems(a.plan);
@@ -669,9 +669,9 @@ __device__ __forceinline__ uint32_t row_shr(uint32_t x) {
 // limb column sums s_j = w0_j + w1_(j-1) + w2_(j-2), which become limbs with
 // one neighbour shift and one ballot carry resolution per run; what passes
 // the run's top goes to the spill word (the carry engine adds it).
-template <int M>
-__global__ void __launch_bounds__((ColPair<M, kPairInvLimbs>::kThreads)) ntt_col_inv_limbs_kernel(PassArgs a) {
-    using CP = ColPair<M, kPairInvLimbs>;
+template <int M, bool PAIR>
+__global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_inv_limbs_kernel(PassArgs a) {
+    using CP = ColPair<M, PAIR>;
     __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
     double *const lds = lds_tiles + CP::half() * kLdsSlots;
     constexpr int LC = kTileLog2 - M;
@@ -681,7 +681,7 @@ __global__ void __launch_bounds__((ColPair<M, kPairInvLimbs>::kThreads)) ntt_col
     // adjacent lanes, which hold one row of one tile also in paired blocks,
     // and inside one DPP row of 16 lanes (the neighbour shifts)
     static_assert(kRun <= (1u << LC) && kRun <= 16u, "a run is one tile row inside one DPP row");
-    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan) >> CP::kLog2);
+    const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan) >> CP::kLog2, a.xcd_log2);
     const uint32_t prod = bp.row;
     const NttProd pr = a.prods[bp.seg];
     const NttOp oa = a.ops[pr.a];
@@ -791,7 +791,7 @@ __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
     constexpr uint32_t kJ = 1u << (M - 4);
     uint64_t *const lo_s = reinterpret_cast<uint64_t *>(work);
     uint32_t *const hi_s = reinterpret_cast<uint32_t *>(hi_w);
-    const uint32_t prod0 = xcd_tile(blockIdx.x, gridDim.x) * (kTile >> M);
+    const uint32_t prod0 = xcd_tile(blockIdx.x, gridDim.x, a.xcd_log2) * (kTile >> M);
     // prime 0's results: the first half stays in registers, the second
     // half waits in hi_w (8 x 256 doubles, lane-consecutive)
     double res0[16], res1[16];
@@ -997,6 +997,25 @@ std::atomic<uint32_t> g_split[kMaxLog2 + 1];
 Plan active_plan(int k) {
     const uint32_t s = g_split[k].load(std::memory_order_relaxed);
     return make_plan_split(k, (int)(s & 0xffu), (int)(s >> 8));
+}
+
+// Launch shape (gpupi_ntt_set_launch): which column level kinds pair their
+// tiles, and the XCD chunk of the block order.  Read once per product
+// launch sequence.
+std::atomic<uint32_t> g_pair_mask{kPairMaskDefault};
+std::atomic<int> g_xcd_log2{kXcdChunkLog2};
+
+// Calls f(std::integral_constant<bool, PAIR>) with PAIR = pair where a level
+// of 2^M points can pair its tiles, false otherwise.
+template <int M, typename F>
+void with_pair(bool pair, F &&f) {
+    if constexpr (col_pair_log2(M) > 0) {
+        if (pair) {
+            f(std::true_type{});
+            return;
+        }
+    }
+    f(std::false_type{});
 }
 
 // Spill words per product for any plan of size 2^k: runs are never shorter
@@ -1220,6 +1239,7 @@ hipError_t run_local(const NttJob &j, const NTTPrecomputedTables &tables, hipStr
         a.sC = pr.sC;
         a.N = pr.N;
         a.items = j.items;
+        a.xcd_log2 = g_xcd_log2.load(std::memory_order_relaxed);
         const uint32_t G = kTile >> k;
         const uint64_t blocks = ceil_div64(pr.N, G);
         if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
@@ -1259,6 +1279,8 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
     PassArgs a{};
     a.plan = p;
     a.tables = tables.roots;
+    a.xcd_log2 = g_xcd_log2.load(std::memory_order_relaxed);
+    const uint32_t pair_mask = g_pair_mask.load(std::memory_order_relaxed);
     a.T = reinterpret_cast<double *>(ws);
     a.spill = reinterpret_cast<uint64_t *>(ws + l.spill_off);
     a.n_ops = j.n_ops;
@@ -1311,6 +1333,9 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
         const double prod_tb = 16.0 * (double)prod_rows * (double)K;
         // a column level of 2^lm points runs 2^col_pair_log2(lm) tiles per
         // block: block starts and grid in units of such blocks
+        const auto pair_log2_for = [&](int kind, int lm) {
+            return ((pair_mask >> kind) & 1u) ? col_pair_log2(lm) : 0;
+        };
         const auto col_args = [&](int kind, int lm) {
             PassArgs c = a;
             const int pl = pair_log2_for(kind, lm);
@@ -1321,19 +1346,25 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
         if (fwd_blocks > 0) {
             GPUPI_RETURN_IF(timed(kColFwdLimbs, k, 4.0 * (double)limb_reads + fwd_tb, stream, [&] {
                 dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
-                    constexpr int pl = pair_log2_for(kPairFwdLimbs, decltype(mc)::value);
-                    hipLaunchKernelGGL((ntt_col_fwd_kernel<true, decltype(mc)::value>),
-                                       dim3((uint32_t)(fwd_blocks >> pl)), dim3(256 << pl), 0, stream,
-                                       col_args(kPairFwdLimbs, p.log2M[0]), 0);
+                    constexpr int M = decltype(mc)::value;
+                    with_pair<M>((pair_mask >> kPairFwdLimbs) & 1u, [&](auto pc) {
+                        constexpr int pl = ColPair<M, decltype(pc)::value>::kLog2;
+                        hipLaunchKernelGGL((ntt_col_fwd_kernel<true, M, decltype(pc)::value>),
+                                           dim3((uint32_t)(fwd_blocks >> pl)), dim3(256 << pl), 0, stream,
+                                           col_args(kPairFwdLimbs, M), 0);
+                    });
                 });
             }));
             if (p.levels == 3) {
                 GPUPI_RETURN_IF(timed(kColFwd, k, 2 * fwd_tb, stream, [&] {
                     dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
-                        constexpr int pl = pair_log2_for(kPairFwd, decltype(mc)::value);
-                        hipLaunchKernelGGL((ntt_col_fwd_kernel<false, decltype(mc)::value>),
-                                           dim3((uint32_t)(fwd_blocks >> pl)), dim3(256 << pl), 0, stream,
-                                           col_args(kPairFwd, p.log2M[1]), 1);
+                        constexpr int M = decltype(mc)::value;
+                        with_pair<M>((pair_mask >> kPairFwd) & 1u, [&](auto pc) {
+                            constexpr int pl = ColPair<M, decltype(pc)::value>::kLog2;
+                            hipLaunchKernelGGL((ntt_col_fwd_kernel<false, M, decltype(pc)::value>),
+                                               dim3((uint32_t)(fwd_blocks >> pl)), dim3(256 << pl), 0, stream,
+                                               col_args(kPairFwd, M), 1);
+                        });
                     });
                 }));
             }
@@ -1363,19 +1394,26 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
         if (p.levels == 3) {
             GPUPI_RETURN_IF(timed(kColInv, k, 2 * prod_tb, stream, [&] {
                 dispatch_m<kColLo, kColHi>(p.log2M[1], [&](auto mc) {
-                    constexpr int pl = pair_log2_for(kPairInv, decltype(mc)::value);
-                    hipLaunchKernelGGL((ntt_col_inv_kernel<decltype(mc)::value>), dim3((uint32_t)(prod_blocks >> pl)),
-                                       dim3(256 << pl), 0, stream, col_args(kPairInv, p.log2M[1]), 1);
+                    constexpr int M = decltype(mc)::value;
+                    with_pair<M>((pair_mask >> kPairInv) & 1u, [&](auto pc) {
+                        constexpr int pl = ColPair<M, decltype(pc)::value>::kLog2;
+                        hipLaunchKernelGGL((ntt_col_inv_kernel<M, decltype(pc)::value>),
+                                           dim3((uint32_t)(prod_blocks >> pl)), dim3(256 << pl), 0, stream,
+                                           col_args(kPairInv, M), 1);
+                    });
                 });
             }));
         }
         GPUPI_RETURN_IF(timed(kColInvLimbs, k, prod_tb + 4.0 * (double)limb_writes + 8.0 * prod_rows * runs, stream,
                               [&] {
                                   dispatch_m<kColLo, kColHi>(p.log2M[0], [&](auto mc) {
-                                      constexpr int pl = pair_log2_for(kPairInvLimbs, decltype(mc)::value);
-                                      hipLaunchKernelGGL((ntt_col_inv_limbs_kernel<decltype(mc)::value>),
-                                                         dim3((uint32_t)(prod_blocks >> pl)), dim3(256 << pl), 0,
-                                                         stream, col_args(kPairInvLimbs, p.log2M[0]));
+                                      constexpr int M = decltype(mc)::value;
+                                      with_pair<M>((pair_mask >> kPairInvLimbs) & 1u, [&](auto pc) {
+                                          constexpr int pl = ColPair<M, decltype(pc)::value>::kLog2;
+                                          hipLaunchKernelGGL((ntt_col_inv_limbs_kernel<M, decltype(pc)::value>),
+                                                             dim3((uint32_t)(prod_blocks >> pl)), dim3(256 << pl),
+                                                             0, stream, col_args(kPairInvLimbs, M));
+                                      });
                                   });
                               }));
         for (uint32_t q = 0; q < j.n_prods; ++q) {
@@ -1639,6 +1677,19 @@ hipError_t gpupi_ntt_set_split(int log2_k, int log2_m0, int log2_m1) {
     if (p.levels == 1 || !plan_ok(p) || (p.levels == 2 && log2_m1 != 0)) return hipErrorInvalidValue;
     g_split[log2_k].store((uint32_t)log2_m0 | (uint32_t)log2_m1 << 8);
     return hipSuccess;
+}
+
+hipError_t gpupi_ntt_set_launch(uint32_t pair_mask, int xcd_chunk_log2) {
+    using namespace gpupi::ntt;
+    if (pair_mask > 15u || xcd_chunk_log2 < 0 || xcd_chunk_log2 > kXcdChunkLog2Max) return hipErrorInvalidValue;
+    gpupi::g_pair_mask.store(pair_mask);
+    gpupi::g_xcd_log2.store(xcd_chunk_log2);
+    return hipSuccess;
+}
+
+void gpupi_ntt_get_launch(uint32_t *pair_mask, int *xcd_chunk_log2) {
+    if (pair_mask) *pair_mask = gpupi::g_pair_mask.load();
+    if (xcd_chunk_log2) *xcd_chunk_log2 = gpupi::g_xcd_log2.load();
 }
 
 int gpupi_ntt_get_split(int log2_k, int *log2_m) {

@@ -296,20 +296,22 @@ GPUPI_HD uint64_t col_plane_row_step(const Plan &p, int level) {
 }
 
 // Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin:
-// block b runs on XCD b mod 8.  xcd_tile(b, n) renumbers blocks in groups
-// of 8 * kXcdChunk: within a group each XCD gets kXcdChunk consecutive
-// tiles, so neighbouring tiles (which share 128-byte lines of limbs) meet
-// in one L2 while the blocks resident at one time still cover one compact
-// range of memory.  Blocks past the last whole group keep their number.  A
-// bijection on [0, n).  Measured at 10^9 digits: one contiguous range per
+// block b runs on XCD b mod 8.  xcd_tile(b, n, c) renumbers blocks in
+// groups of 8 * 2^c: within a group each XCD gets 2^c consecutive tiles, so
+// neighbouring tiles (which share 128-byte lines of limbs) meet in one L2
+// while the blocks resident at one time still cover one compact range of
+// memory.  Blocks past the last whole group keep their number; c = 0 is the
+// plain order.  A bijection on [0, n).  The chunk is a launch parameter
+// (gpupi_ntt_set_launch); measured at 10^9 digits: one contiguous range per
 // XCD 961.8 ms of kernels per step, chunks of 16 / 32 / 128 / 256 tiles
-// 940.7 / 940.5 / 941.0 / 944.0 ms, the plain round-robin order 983.2 ms.
+// 940.7 / 940.5 / 941.0 / 944.0 ms, the plain order 983.2 ms.
 constexpr uint32_t kXcds = 8;
-constexpr uint32_t kXcdChunk = 32;
-GPUPI_HD uint32_t xcd_tile(uint32_t b, uint32_t n) {
-    constexpr uint32_t kGroup = kXcds * kXcdChunk;
-    if (b >= (n / kGroup) * kGroup) return b;
-    return (b / kGroup) * kGroup + (b % kXcds) * kXcdChunk + (b / kXcds) % kXcdChunk;
+constexpr int kXcdChunkLog2 = 5;  // built-in chunk: 32 tiles
+constexpr int kXcdChunkLog2Max = 8;
+GPUPI_HD uint32_t xcd_tile(uint32_t b, uint32_t n, int chunk_log2) {
+    const int gl = 3 + chunk_log2;  // log2 of the group: kXcds << chunk_log2
+    if (b >= ((n >> gl) << gl)) return b;
+    return ((b >> gl) << gl) + ((b & (kXcds - 1u)) << chunk_log2) + ((b >> 3) & ((1u << chunk_log2) - 1u));
 }
 
 // Run length of the final inverse column level (limbs per run): the C

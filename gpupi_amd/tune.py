@@ -11,7 +11,16 @@ tuner on MI355X.
 
     python -m gpupi_amd.tune --ks 13-28 [--shared] [--apply] [--json table.json]
 
-Results are exact for every split; only speed changes."""
+With --launch it tunes the launch shapes instead (tile pairing per column
+level kind, the XCD chunk of the block order, the fused group stage's groups
+per block): each candidate times batched products at three transform sizes
+(2^22, 2^25, 2^28 points, 2^28 coefficients each) or the group stage of a
+10^7-digit run (65536 groups), and must leave the same products and rows
+as the built-in shape.
+
+    python -m gpupi_amd.tune --launch [--apply] [--json launch.json]
+
+Results are exact for every split and shape; only speed changes."""
 import argparse
 import json
 import sys
@@ -122,6 +131,82 @@ def tune_crossover(ctx, ops: Operands, totals=(128, 192, 256, 320, 384, 512, 768
     return best
 
 
+def _products_digest(ops: Operands, A, B, C) -> np.ndarray:
+    return ops.bc.download(np.uint32, count=min(1 << 18, C.length * C.batch_size))
+
+
+def tune_launch(ctx, ops: Operands, ks=(22, 25, 28), reps: int = 3, log=None) -> dict:
+    """Times every tile-pairing mask, XCD chunk and group-stage shape; each
+    must give the built-in shape's products / rows.  Restores the built-in
+    shapes and returns {"pair_mask": [...], "xcd_chunk_log2": [...],
+    "group_blocks": [...]} of (value, ms) fastest first, plus the built-ins."""
+    arrays = [ops.arrays(1 << (k - 1)) for k in ks]
+    builtin_launch, builtin_gpb = g.ntt_get_launch(), g.group_stage_get_shape()
+
+    def products_ms():
+        ms, digests = 0.0, []
+        for A, B, C in arrays:
+            ms += time_mul(ctx, A, B, C, reps)
+            digests.append(_products_digest(ops, A, B, C))
+        return ms, digests
+
+    def check(ref, got, what):
+        if any(not np.array_equal(a, b) for a, b in zip(ref, got)):
+            raise RuntimeError(f"{what} changed the products")
+
+    out = {"builtin": {"pair_mask": builtin_launch[0], "xcd_chunk_log2": builtin_launch[1],
+                       "group_blocks": builtin_gpb}}
+    try:
+        _, ref = products_ms()
+        res = []
+        for mask in range(16):
+            g.ntt_set_launch(mask, builtin_launch[1])
+            ms, got = products_ms()
+            check(ref, got, f"pair mask {mask}")
+            res.append((mask, ms))
+            if log:
+                log(f"pair mask {mask:2d}: {ms:.3f} ms")
+        out["pair_mask"] = sorted(res, key=lambda r: r[1])
+        g.ntt_set_launch(builtin_launch[0], builtin_launch[1])
+        res = []
+        for c in range(0, 9):
+            g.ntt_set_launch(builtin_launch[0], c)
+            ms, got = products_ms()
+            check(ref, got, f"XCD chunk 2^{c}")
+            res.append((c, ms))
+            if log:
+                log(f"XCD chunk 2^{c}: {ms:.3f} ms")
+        out["xcd_chunk_log2"] = sorted(res, key=lambda r: r[1])
+        g.ntt_set_launch(builtin_launch[0], builtin_launch[1])
+        res, ref_rows = [], None
+        stage = g.LeafStage(ctx, 65536, g.LEAF_FUSED)
+        try:
+            for gpb in (1, 2, 4, 8, 16):
+                g.group_stage_set_shape(gpb)
+                stage.run()
+                g.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps * 4):
+                    P, Q, R = stage.run()
+                g.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3 / (reps * 4)
+                rows = [g.rows_of(x) for x in (P, Q, R)]
+                if ref_rows is None:
+                    ref_rows = rows
+                elif any(not np.array_equal(a, b) for a, b in zip(ref_rows, rows)):
+                    raise RuntimeError(f"group shape {gpb} changed the rows")
+                res.append((gpb, ms))
+                if log:
+                    log(f"group stage {gpb} groups per block: {ms:.3f} ms")
+        finally:
+            stage.close()
+        out["group_blocks"] = sorted(res, key=lambda r: r[1])
+    finally:
+        g.ntt_set_launch(*builtin_launch)
+        g.group_stage_set_shape(builtin_gpb)
+    return out
+
+
 def apply(table: dict) -> None:
     """table: {k: [m0, m1, ...levels]} as written by main(); sets the splits."""
     for k, levels in table.items():
@@ -138,6 +223,8 @@ def main(argv=None) -> int:
     ap.add_argument("--shared", action="store_true",
                     help="time pairs of products sharing an operand (the tree's launches)")
     ap.add_argument("--crossover", action="store_true", help="also time the schoolbook/NTT crossover")
+    ap.add_argument("--launch", action="store_true",
+                    help="tune the launch shapes (tile pairing, XCD chunk, group-stage blocks) instead of splits")
     ap.add_argument("--apply", action="store_true", help="keep the fastest splits for this process")
     ap.add_argument("--json", help="write the table here")
     args = ap.parse_args(argv)
@@ -149,6 +236,22 @@ def main(argv=None) -> int:
     ks = [k for k in ks if 13 <= k <= min(30, args.total_log2)]
     ctx = g.Context()
     ops = Operands(args.total_log2, shared=args.shared)
+    if args.launch:
+        res = tune_launch(ctx, ops, reps=args.reps, log=lambda s: print(s, file=sys.stderr, flush=True))
+        ctx.close()
+        best = {"pair_mask": res["pair_mask"][0][0], "xcd_chunk_log2": res["xcd_chunk_log2"][0][0],
+                "group_blocks": res["group_blocks"][0][0]}
+        if args.apply:
+            g.ntt_set_launch(best["pair_mask"], best["xcd_chunk_log2"])
+            g.group_stage_set_shape(best["group_blocks"])
+        out = {"best": best, **{k: [[v, round(ms, 3)] for v, ms in res[k]]
+                                for k in ("pair_mask", "xcd_chunk_log2", "group_blocks")},
+               "builtin": res["builtin"]}
+        if args.json:
+            with open(args.json, "w") as f:
+                json.dump(out, f, indent=1)
+        print(json.dumps(out))
+        return 0
     table, report = {}, {}
     for k in ks:
         res = tune_splits(ctx, ops, k, args.reps, log=lambda s: print(s, file=sys.stderr, flush=True),

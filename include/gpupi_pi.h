@@ -105,6 +105,20 @@ int gpupi_kernel_stats_query_sizes(GpupiKernelStat *out, int max_entries);
  * (0 for an unsupported k). */
 hipError_t gpupi_ntt_set_split(int log2_k, int log2_m0, int log2_m1);
 int gpupi_ntt_get_split(int log2_k, int *log2_m);
+/* NTT launch shape (no reference counterpart; the reference fixes its block
+ * shapes, batch_mul_ntt.cu:1810-1915), process-wide, taking effect from the
+ * next multiply on:
+ *   pair_mask       column level kinds that run two neighbouring 4096-element
+ *                   tiles per 512-thread block (levels of 2^7 points or more):
+ *                   bit 0 first forward level, bit 1 second forward level,
+ *                   bit 2 second inverse level, bit 3 first inverse level
+ *                   (built-in 5);
+ *   xcd_chunk_log2  consecutive tiles each of the 8 XCDs takes in turn, as a
+ *                   log2 in 0..8 (0: the plain round-robin order; built-in 5).
+ * hipErrorInvalidValue outside those ranges.  Results are bit-identical for
+ * every shape; only speed changes.  gpupi_amd.tune --launch times them. */
+hipError_t gpupi_ntt_set_launch(uint32_t pair_mask, int xcd_chunk_log2);
+void gpupi_ntt_get_launch(uint32_t *pair_mask, int *xcd_chunk_log2);
 
 /* The 17-term group stage on its own (pi_bs_gpu.cu:555-779) for `groups`
  * groups (a power of two), in three forms that leave the same P, Q, R rows
@@ -127,6 +141,12 @@ typedef struct GpupiLeafStage GpupiLeafStage;
 hipError_t gpupi_leaf_stage_create(BatchMPContext *ctx, uint32_t groups, int mode, GpupiLeafStage **out);
 hipError_t gpupi_leaf_stage_run(GpupiLeafStage *stage, BatchMPArray *P, BatchMPArray *Q, BatchMPArray *R);
 void gpupi_leaf_stage_destroy(GpupiLeafStage *stage);
+/* Block shape of the fused group-stage kernel (no reference counterpart),
+ * process-wide, from the next launch on: groups per block, one wave each
+ * (1, 2, 4, 8 or 16; built-in 4; else hipErrorInvalidValue).  Rows are
+ * identical for every shape.  gpupi_amd.tune --launch times them. */
+hipError_t gpupi_group_stage_set_shape(uint32_t groups_per_block);
+uint32_t gpupi_group_stage_get_shape(void);
 
 /* Per-stage profile of the pipeline (the reference's StageProfiler,
  * pi_bs_gpu.cu:133-489, behind --profile-stages).  When enabled, every

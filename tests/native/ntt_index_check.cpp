@@ -216,19 +216,23 @@ static void check_col_pairs() {
 }
 
 static void check_xcd_tile() {
-    for (uint32_t n = 1; n < 3000; n += (n < 100 ? 1 : 37)) {
-        std::vector<int> seen(n, 0);
-        for (uint32_t b = 0; b < n; ++b) {
-            const uint32_t t = xcd_tile(b, n);
-            EXPECT(t < n, "xcd_tile range n=%u", n);
-            if (t < n) seen[t]++;
+    for (int c = 0; c <= kXcdChunkLog2Max; ++c)
+        for (uint32_t n = 1; n < 5000; n += (n < 100 ? 1 : 37)) {
+            const uint32_t chunk = 1u << c;
+            std::vector<int> seen(n, 0);
+            for (uint32_t b = 0; b < n; ++b) {
+                const uint32_t t = xcd_tile(b, n, c);
+                EXPECT(t < n, "xcd_tile range n=%u c=%d", n, c);
+                if (t < n) seen[t]++;
+            }
+            for (uint32_t t = 0; t < n; ++t) EXPECT(seen[t] == 1, "xcd_tile bijection n=%u c=%d", n, c);
+            // inside whole groups, each run of 2^c tiles runs on one XCD
+            const uint32_t whole = (n / (kXcds * chunk)) * (kXcds * chunk);
+            for (uint32_t b = 0; b < whole; ++b)
+                EXPECT((xcd_tile(b, n, c) / chunk) % kXcds == b % kXcds, "xcd_tile chunk on one XCD n=%u c=%d", n, c);
+            if (c == 0)
+                for (uint32_t b = 0; b < n; ++b) EXPECT(xcd_tile(b, n, 0) == b, "chunk 1 is the plain order");
         }
-        for (uint32_t t = 0; t < n; ++t) EXPECT(seen[t] == 1, "xcd_tile bijection n=%u", n);
-        // inside whole groups, each run of kXcdChunk tiles runs on one XCD
-        const uint32_t whole = (n / (kXcds * kXcdChunk)) * (kXcds * kXcdChunk);
-        for (uint32_t b = 0; b < whole; ++b)
-            EXPECT((xcd_tile(b, n) / kXcdChunk) % kXcds == b % kXcds, "xcd_tile chunk on one XCD n=%u", n);
-    }
 }
 
 int main() {

@@ -43,6 +43,27 @@ def test_leaf_stage_matches_oracle(ctx, groups, mode):
         st.close()
 
 
+@pytest.mark.parametrize("groups", [4, 4096])
+def test_fused_stage_every_block_shape(ctx, groups):
+    # groups per block of the fused kernel: the oracle's rows for every
+    # shape, including blocks with more waves than groups
+    want = oracle_stage(groups)
+    before = g.group_stage_get_shape()
+    st = g.LeafStage(ctx, groups, g.LEAF_FUSED)
+    try:
+        for gpb in (1, 2, 4, 8, 16):
+            g.group_stage_set_shape(gpb)
+            assert g.group_stage_get_shape() == gpb
+            for name, w, x in zip("PQR", want, stage_rows(st)):
+                assert np.array_equal(w, x), (name, gpb)
+        with pytest.raises(ValueError):
+            g.group_stage_set_shape(3)
+    finally:
+        g.group_stage_set_shape(before)
+        st.close()
+    assert g.group_stage_get_shape() == before
+
+
 def test_graph_replay_after_context_grows(ctx):
     # the captured graph uses the stage's own scratch: a large product that
     # regrows the context's workspace in between must not disturb replays

@@ -323,6 +323,59 @@ def test_ntt_every_split(ctx, k):
     assert g.ntt_get_split(k) == default
 
 
+LAUNCH_K = [10, 16, 20, 23, 24]
+
+
+@pytest.mark.parametrize("k", LAUNCH_K, ids=[f"K2^{k}" for k in LAUNCH_K])
+def test_ntt_every_launch_shape(ctx, k):
+    # every tile-pairing mask (each column level kind paired or not) and
+    # XCD chunks from the plain order to 256 tiles: bit-identical to GMP;
+    # K = 2^23 and 2^24 have 2^7- and 2^8-point second levels, so every kind
+    # meets both block shapes
+    g.check(ctx.set_ntt_threshold(2))
+    builtin = g.ntt_get_launch()
+    try:
+        n, la, lb = every_size_case(k)
+        n = min(n, 4)
+        rng = np.random.default_rng(300 + k)
+        A = rand_rows(rng, n, la)
+        B = rand_rows(rng, n, lb)
+        want = oracle_mul(A, B)
+        for mask in range(16):
+            for chunk in ((0, 3, 5, 8) if mask in (0, 15) else (5,)):
+                g.ntt_set_launch(mask, chunk)
+                assert g.ntt_get_launch() == (mask, chunk)
+                assert np.array_equal(device_mul(ctx, A, B), want), (mask, chunk)
+    finally:
+        g.ntt_set_launch(*builtin)
+        g.check(ctx.set_ntt_threshold(384))
+    assert g.ntt_get_launch() == builtin
+
+
+def test_shared_products_every_launch_shape(ctx):
+    # the paired last level (two products sharing B) under the extreme shapes
+    rng = np.random.default_rng(77)
+    A1, A2 = rand_rows(rng, 3, 70000), rand_rows(rng, 2, 65000)
+    B = rand_rows(rng, 3, 60000)
+    want1, want2 = oracle_mul(A1, B[:3]), oracle_mul(A2, B[:2])
+    builtin = g.ntt_get_launch()
+    try:
+        for mask, chunk in ((0, 0), (15, 8), (5, 5), (10, 2)):
+            g.ntt_set_launch(mask, chunk)
+            c1, c2 = device_mul_shared(ctx, A1, A2, B)
+            assert np.array_equal(c1, want1) and np.array_equal(c2, want2), (mask, chunk)
+    finally:
+        g.ntt_set_launch(*builtin)
+
+
+def test_launch_shape_rejects_bad_values():
+    builtin = g.ntt_get_launch()
+    for mask, chunk in ((16, 5), (5, 9), (5, -1)):
+        with pytest.raises(ValueError):
+            g.ntt_set_launch(mask, chunk)
+    assert g.ntt_get_launch() == builtin
+
+
 @pytest.mark.slow
 def test_ntt_three_level(ctx):
     # K = 2^23 and 2^25: two column levels + the contiguous level
@@ -419,6 +472,18 @@ def test_ragged_rejects_aliasing(ctx):
     buf = g.DeviceBuffer.from_array(np.ones(64, dtype=np.uint32))
     assert ctx.mul_ragged([(buf.ptr, buf.ptr + 64, buf.ptr, 8, 8)]) != 0
     assert ctx.mul_ragged([(None, buf.ptr, buf.ptr + 128, 8, 8)]) != 0
+
+
+def test_tuner_times_every_launch_shape_and_restores_it(ctx):
+    from gpupi_amd import tune
+    ops = tune.Operands(20, seed=8)
+    before = (g.ntt_get_launch(), g.group_stage_get_shape())
+    res = tune.tune_launch(ctx, ops, ks=(14, 17, 20), reps=1)
+    assert sorted(v for v, _ in res["pair_mask"]) == list(range(16))
+    assert sorted(v for v, _ in res["xcd_chunk_log2"]) == list(range(9))
+    assert sorted(v for v, _ in res["group_blocks"]) == [1, 2, 4, 8, 16]
+    assert all(ms > 0 for key in ("pair_mask", "xcd_chunk_log2", "group_blocks") for _, ms in res[key])
+    assert (g.ntt_get_launch(), g.group_stage_get_shape()) == before
 
 
 def test_tuner_times_every_split_and_restores_the_plan(ctx):

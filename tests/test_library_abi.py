@@ -43,6 +43,26 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_launch_shape_setters_validate_without_a_device():
+    # process-wide launch parameters: range checks and read-back (no device call)
+    lib = ctypes.CDLL(LIB)
+    lib.gpupi_ntt_set_launch.argtypes = [ctypes.c_uint32, ctypes.c_int]
+    lib.gpupi_group_stage_set_shape.argtypes = [ctypes.c_uint32]
+    lib.gpupi_group_stage_get_shape.restype = ctypes.c_uint32
+    m, c = ctypes.c_uint32(), ctypes.c_int()
+    lib.gpupi_ntt_get_launch(ctypes.byref(m), ctypes.byref(c))
+    assert (m.value, c.value) == (5, 5)
+    assert lib.gpupi_group_stage_get_shape() == 4
+    for mask, chunk in ((16, 5), (5, 9), (5, -1)):
+        assert lib.gpupi_ntt_set_launch(mask, chunk) != 0
+    for gpb in (0, 3, 32):
+        assert lib.gpupi_group_stage_set_shape(gpb) != 0
+    assert lib.gpupi_ntt_set_launch(15, 0) == 0 and lib.gpupi_group_stage_set_shape(16) == 0
+    lib.gpupi_ntt_get_launch(ctypes.byref(m), ctypes.byref(c))
+    assert (m.value, c.value) == (15, 0) and lib.gpupi_group_stage_get_shape() == 16
+    assert lib.gpupi_ntt_set_launch(5, 5) == 0 and lib.gpupi_group_stage_set_shape(4) == 0
+
+
 def test_library_does_not_link_the_oracle():
     out = subprocess.run(["readelf", "-d", LIB], capture_output=True, text=True, check=True).stdout
     needed = re.findall(r"\(NEEDED\).*\[(.*)\]", out)
