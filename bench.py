@@ -546,7 +546,7 @@ def bench_serve(args, g) -> dict:
     same = True
     first_submit = min(r.submit_ms for r, _ in results.values())
     last_finish = max(r.finish_ms for r, _ in results.values())
-    batch_max = 0
+    batch_max = pi_batch_max = 0
     for jid, (r, data) in results.items():
         i = id2job[jid]
         kind = jobs[i][0]
@@ -554,6 +554,8 @@ def bench_serve(args, g) -> dict:
         same = same and digest(kind, r, data) == alone[i]
         if kind == "mul":
             batch_max = max(batch_max, r.batch_size)
+        else:
+            pi_batch_max = max(pi_batch_max, r.batch_size)
     allv = np.array(lat["pi"] + lat["mul"])
     pct = lambda v, q: round(float(np.percentile(np.array(v), q)), 2) if v else None
 
@@ -584,12 +586,77 @@ def bench_serve(args, g) -> dict:
     return {"metric": f"serving p99 latency, {n} jobs, Poisson arrivals at 70% of sequential capacity",
             "value": pct(allv.tolist(), 99), "unit": "ms", "higher_is_better": False,
             "p50_ms": pct(allv.tolist(), 50), "p99_ms": pct(allv.tolist(), 99),
-            "pi": {"jobs": len(lat["pi"]), "p50_ms": pct(lat["pi"], 50), "p99_ms": pct(lat["pi"], 99)},
+            "pi": {"jobs": len(lat["pi"]), "p50_ms": pct(lat["pi"], 50), "p99_ms": pct(lat["pi"], 99),
+                   "max_jobs_per_shared_tree": pi_batch_max},
             "mul": {"jobs": len(lat["mul"]), "p50_ms": pct(lat["mul"], 50), "p99_ms": pct(lat["mul"], 99),
                     "max_jobs_per_shared_launch": batch_max},
             "throughput_jobs_per_s": round(n / ((last_finish - first_submit) / 1e3), 2),
             "arrival_rate_jobs_per_s": round(rate, 2), "mean_service_alone_ms": round(mean_service, 2),
             "identical_alone_and_in_mix": same, "hbm_budget_gb": 200,
+            "roofline": None, "cpu_baseline": cpu}
+
+
+def bench_serve_burst(args, g) -> dict:
+    """A burst of 16 pi jobs of one tree size (floor(10^U(6, 6.27)) digits,
+    seed 5: 8192 groups each) submitted at once to a server with 4 pi
+    workers, with shared trees (pi_batch_max 8) and without (1).  Each server
+    first serves the same burst once (contexts, memory pool), then the second
+    burst is timed; three servers per mode, alternating, median makespan.
+    Every result must be identical in both modes, and equal the oracle."""
+    import numpy as np
+    rng = np.random.default_rng(5)
+    digits = [int(10 ** rng.uniform(6.0, 6.27)) for _ in range(16)]
+    assert len({g.lib().gpupi_pi_groups_for_digits(d) for d in digits}) == 1
+
+    def burst(batch_max):
+        srv = g.Server(pi_workers=4, pi_batch_max=batch_max)
+        try:
+            for rep in range(2):
+                t0 = time.perf_counter()
+                ids = {srv.submit_pi(d, keep_digits=True): d for d in digits}
+                res = {}
+                while len(res) < len(ids):
+                    for r, data in srv.results(wait=True):
+                        assert r.status == 0
+                        res[r.id] = (r, bytes(data))
+                wall = (time.perf_counter() - t0) * 1e3
+        finally:
+            srv.close()
+        lat = [r.finish_ms - r.submit_ms for r, _ in res.values()]
+        return wall, float(np.mean(lat)), max(r.batch_size for r, _ in res.values()), \
+            {ids[i]: (r.ret_length, r.ret_hash, data) for i, (r, data) in res.items()}
+
+    runs = {1: [], 8: []}
+    outs = {}
+    for _ in range(3):
+        for bm in (1, 8):
+            wall, mean_lat, mb, out = burst(bm)
+            runs[bm].append((wall, mean_lat, mb))
+            outs.setdefault(bm, out)
+            assert out == outs[bm]
+    identical = outs[1] == outs[8]
+    med = {bm: sorted(v)[1] for bm, v in runs.items()}
+    cpu, exact = None, None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_ffi as of
+        t_cpu, exact = 0.0, True
+        for d in digits:
+            t1 = time.perf_counter()
+            _, odec, _ = of.pi_digits(d)
+            t_cpu += time.perf_counter() - t1
+            exact = exact and odec.encode() == outs[8][d][2]
+        cpu = {"value": round(len(digits) / t_cpu, 3), "unit": "pi jobs/s", "cores": 1, "kind": "port",
+               "sample": f"the whole burst: oracle/liboracle (GMP restatement of the pipeline) on the {len(digits)} "
+                         "jobs one after the other, one thread", "gpu_digits_checked_bit_exact": exact}
+    return {"metric": "pi jobs/s, burst of 16 jobs of one tree size (1.0-1.9 M digits), 4 pi workers",
+            "value": round(len(digits) / (med[8][0] / 1e3), 1), "unit": "jobs/s", "higher_is_better": True,
+            "shared_trees": {"makespan_ms": round(med[8][0], 2), "mean_latency_ms": round(med[8][1], 2),
+                             "max_jobs_per_tree": max(r[2] for r in runs[8])},
+            "one_tree_per_job": {"makespan_ms": round(med[1][0], 2), "mean_latency_ms": round(med[1][1], 2)},
+            "speedup_shared_vs_single": round(med[1][0] / med[8][0], 3),
+            "runs_ms": {str(bm): [round(r[0], 2) for r in v] for bm, v in runs.items()},
+            "digits": [min(digits), max(digits)], "identical_results": identical,
             "roofline": None, "cpu_baseline": cpu}
 
 
@@ -641,12 +708,14 @@ def main() -> int:
     ap.add_argument("--headline-only", action="store_true", help="skip the other workloads")
     ap.add_argument("--pairs", type=int, default=256)
     ap.add_argument("--jobs", type=int, default=100)
-    ap.add_argument("--workload", choices=["all", "pi", "pi1e6", "mul", "leaf", "ragged", "serve"], default="all",
+    ap.add_argument("--workload", choices=["all", "pi", "pi1e6", "mul", "leaf", "ragged", "serve", "burst"],
+                    default="all",
                     help="all: the headline run plus every other workload in one line; pi: the headline run; "
                          "pi1e6: pi to 10^6 digits, batched vs plain; mul: one batched product shape "
                          "(kernel study); leaf: the 17-term group stage of a 10^7-digit run in its four forms; "
                          "ragged: --pairs log-uniform pairs in one ragged call vs one call per pair; "
-                         "serve: the serving loop on a Poisson stream of --jobs pi and multiply jobs")
+                         "serve: the serving loop on a Poisson stream of --jobs pi and multiply jobs; "
+                         "burst: 16 pi jobs of one tree size at once, shared trees vs one tree per job")
     ap.add_argument("--la", type=int, default=1 << 27)
     ap.add_argument("--lb", type=int, default=1 << 27)
     ap.add_argument("--n", type=int, default=1)
@@ -659,7 +728,7 @@ def main() -> int:
     import gpupi_amd as g
 
     single = {"pi1e6": bench_pi_small, "mul": bench_mul, "leaf": bench_leaf, "ragged": bench_ragged,
-              "serve": bench_serve}
+              "serve": bench_serve, "burst": bench_serve_burst}
     if args.workload in single:
         r = single[args.workload](args, g)
         print(json.dumps(r))
@@ -669,7 +738,8 @@ def main() -> int:
     if args.workload == "all" and not args.headline_only:
         configs = {}
         for key, fn in (("pi_1e6", bench_pi_small), ("ragged_256", bench_ragged),
-                        ("group_stage_1e7", bench_leaf), ("serving", bench_serve)):
+                        ("group_stage_1e7", bench_leaf), ("serving", bench_serve),
+                        ("serving_burst", bench_serve_burst)):
             log(f"config {key}")
             t0 = time.perf_counter()
             configs[key] = fn(args, g)
@@ -681,10 +751,14 @@ def main() -> int:
             "ragged_256": configs["ragged_256"]["speedup_ragged_vs_plain"],
             "group_stage_1e7_fused": configs["group_stage_1e7"]["speedup_fused_vs_plain"],
             "group_stage_1e7_graph": configs["group_stage_1e7"]["speedup_graph_vs_plain"],
+            "serving_burst_shared_trees": configs["serving_burst"]["speedup_shared_vs_single"],
         }
         ok = ok and configs["pi_1e6"]["plain_identical"] and configs["pi_1e6"]["bit_exact_vs_oracle"] in (None, True)
         ok = ok and configs["ragged_256"]["identical_to_plain"] and configs["group_stage_1e7"]["identical_rows"]
         ok = ok and configs["serving"]["identical_alone_and_in_mix"]
+        ok = ok and configs["serving_burst"]["identical_results"]
+        burst_cpu = configs["serving_burst"]["cpu_baseline"]
+        ok = ok and (burst_cpu is None or burst_cpu["gpu_digits_checked_bit_exact"])
     print(json.dumps(line))
     return 0 if ok else 1
 

@@ -86,7 +86,7 @@ class GpupiMulItem(ctypes.Structure):
 
 class GpupiServeConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("pi_workers", ctypes.c_uint32),
-                ("mul_batch_max", ctypes.c_uint32)]
+                ("mul_batch_max", ctypes.c_uint32), ("pi_batch_max", ctypes.c_uint32)]
 
 
 class GpupiJobResult(ctypes.Structure):
@@ -162,6 +162,9 @@ def _declare(lib):
         "batch_mp_set_stream": (err, [c_ctx, ctypes.c_void_p]),
         "batch_mp_set_ntt_threshold": (err, [c_ctx, u32]),
         "gpupi_pi_run": (err, [c_ctx, u32, ctypes.c_int, ctypes.POINTER(GpupiPiResult)]),
+        "gpupi_pi_run_batch": (err, [c_ctx, ctypes.POINTER(u32), ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(GpupiPiResult)]),
+        "gpupi_pi_groups_for_digits": (u32, [u32]),
         "gpupi_pi_free": (None, [ctypes.POINTER(GpupiPiResult)]),
         "gpupi_binary_split": (err, [c_ctx, u32, ctypes.POINTER(A), ctypes.POINTER(A)]),
         "gpupi_sqrt_10005": (err, [c_ctx, ctypes.c_int, ctypes.POINTER(A), ctypes.POINTER(A)]),
@@ -387,6 +390,24 @@ class Context:
             self._lib.gpupi_pi_free(ctypes.byref(res))
         return ret, dec, res
 
+    def pi_batch(self, digits, convention: int = PREC_REQUESTED):
+        """Several pi jobs sharing one binary-splitting tree (gpupi_pi_run_batch);
+        returns [(RET limbs, digit string, result struct)] in input order."""
+        n = len(digits)
+        arr = (ctypes.c_uint32 * n)(*digits)
+        res = (GpupiPiResult * n)()
+        check(self._lib.gpupi_pi_run_batch(self.ptr, arr, n, convention, res), "gpupi_pi_run_batch")
+        out = []
+        try:
+            for i, d in enumerate(digits):
+                ret = download_words(res[i].ret, res[i].ret_length)
+                dec = download_bytes(res[i].digits, d).decode() if d else ""
+                out.append((ret, dec, res[i]))
+        finally:
+            for i in range(n):
+                self._lib.gpupi_pi_free(ctypes.byref(res[i]))
+        return out
+
     def set_form(self, form: int) -> None:
         """FORM_BATCHED (default) or FORM_PLAIN, the reference's per-op form."""
         check(self._lib.gpupi_set_pipeline_form(self.ptr, form), "gpupi_set_pipeline_form")
@@ -439,9 +460,9 @@ class Server:
     """The serving loop (gpupi_serve.h).  Host buffers passed to submit_*
     are kept alive here until their results are collected."""
 
-    def __init__(self, budget_bytes: int = 0, pi_workers: int = 0, mul_batch_max: int = 0):
+    def __init__(self, budget_bytes: int = 0, pi_workers: int = 0, mul_batch_max: int = 0, pi_batch_max: int = 0):
         self._lib = lib()
-        cfg = GpupiServeConfig(budget_bytes, pi_workers, mul_batch_max)
+        cfg = GpupiServeConfig(budget_bytes, pi_workers, mul_batch_max, pi_batch_max)
         self.ptr = self._lib.gpupi_server_create(ctypes.byref(cfg))
         if not self.ptr:
             raise RuntimeError("gpupi_server_create failed")

@@ -433,6 +433,9 @@ struct TreeBufs {
     void freeze(bool f) {
         for (Buf *b : {&bP0, &bQ0, &bR0, &bP, &bQ, &bR, &bPn, &bQn, &bRn, &bR1, &bR2}) b->frozen = f;
     }
+    void drop() {
+        for (Buf *b : {&bP0, &bQ0, &bR0, &bP, &bQ, &bR, &bPn, &bQn, &bRn, &bR1, &bR2}) b->drop();
+    }
 };
 
 // How a tree level fixes the lengths of its outputs.
@@ -1060,16 +1063,25 @@ hipError_t gpupi_recursive_inverse(BatchMPContext *ctx, BatchMPArray x, uint64_t
     return hipSuccess;
 }
 
-// pi_bs_gpu.cu:1524-1685
-hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPArray *ret_out, uint32_t *groups_out,
-                               uint64_t *inverse_bits_out) {
-    if (ctx == nullptr || ret_out == nullptr) return hipErrorInvalidValue;
-    int N = 1;
-    while (N * 17 * 14.3 < target_digits) N *= 2;
-    const int prec_limbs = (int)(((long)(target_digits * 3.322) + 31) / 32 + 1);
+}  // extern "C"
 
-    BatchMPArray Q_bs{}, R_bs{};
-    GPUPI_RETURN_IF(gpupi_binary_split(ctx, (uint32_t)N, &Q_bs, &R_bs));
+namespace {
+
+// Groups of the binary-splitting tree for target_digits (pi_bs_gpu.cu:1537-1540:
+// the smallest power of two with 17 N 14.3 >= target_digits).
+uint32_t pi_groups(int target_digits) {
+    uint32_t N = 1;
+    while (N * 17 * 14.3 < target_digits) N *= 2;
+    return N;
+}
+
+// pi_fractional after the binary splitting (pi_bs_gpu.cu:1557-1685): Q_bs,
+// R_bs are the root rows of the tree over N groups, handed over (blocks of
+// ctx's cache, or of its memory pool).
+hipError_t pi_fractional_from_tree(BatchMPContext *ctx, int target_digits, int N, BatchMPArray Q_bs,
+                                   BatchMPArray R_bs, BatchMPArray *ret_out, uint32_t *groups_out,
+                                   uint64_t *inverse_bits_out) {
+    const int prec_limbs = (int)(((long)(target_digits * 3.322) + 31) / 32 + 1);
     Buf bQbs(ctx), bRbs(ctx);
     bQbs.adopt(Q_bs.data, Q_bs.stride);
     bRbs.adopt(R_bs.data, R_bs.stride);
@@ -1138,6 +1150,20 @@ hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPAr
     return hipSuccess;
 }
 
+}  // namespace
+
+extern "C" {
+
+// pi_bs_gpu.cu:1524-1685
+hipError_t gpupi_pi_fractional(BatchMPContext *ctx, int target_digits, BatchMPArray *ret_out, uint32_t *groups_out,
+                               uint64_t *inverse_bits_out) {
+    if (ctx == nullptr || ret_out == nullptr) return hipErrorInvalidValue;
+    const uint32_t N = pi_groups(target_digits);
+    BatchMPArray Q_bs{}, R_bs{};
+    GPUPI_RETURN_IF(gpupi_binary_split(ctx, N, &Q_bs, &R_bs));
+    return pi_fractional_from_tree(ctx, target_digits, (int)N, Q_bs, R_bs, ret_out, groups_out, inverse_bits_out);
+}
+
 // pi_bs_gpu.cu:924-939
 uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits) {
     if (requested == 0) {
@@ -1153,6 +1179,9 @@ uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits
     if (leaf_digits) *leaf_digits = digits;
     return digits << levels;
 }
+
+// pi_bs_gpu.cu:1537-1540
+uint32_t gpupi_pi_groups_for_digits(uint32_t prec_digits) { return pi_groups((int)prec_digits); }
 
 // pi_bs_gpu.cu:941-944
 uint32_t gpupi_prec_limbs_for_digits(uint32_t digits) {
@@ -1364,40 +1393,60 @@ hipError_t convert_decimal_equal_split(BatchMPContext *ctx, BatchMPArray input, 
     return hipStreamSynchronize(ctx->stream);
 }
 
-// pi_bs_gpu.cu:2073-2193: the reference's timed region.
-hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec_convention, GpupiPiResult *result) {
-    if (ctx == nullptr || result == nullptr) return hipErrorInvalidValue;
-    std::memset(result, 0, sizeof(*result));
-    uint32_t leaf = 477;
-    const uint32_t convert_digits = gpupi_round_up_decimal_digits(requested_digits, &leaf);
-    const uint32_t prec_digits = prec_convention == GPUPI_PREC_ROUNDED ? convert_digits : requested_digits;
-    const uint32_t prec_limbs = gpupi_prec_limbs_for_digits(prec_digits);
-    const uint32_t levels = convert_digits ? (uint32_t)__builtin_ctz(convert_digits / leaf) : 0u;
-    hipEvent_t start = nullptr, stop = nullptr;
-    GPUPI_RETURN_IF(hipEventCreate(&start));
-    GPUPI_RETURN_IF(hipEventCreate(&stop));
+}  // extern "C"
+
+namespace {
+
+// One pi job's sizes (pi_bs_gpu.cu:2096-2136): digits converted, leaf digits
+// of the conversion, precision in digits and limbs, conversion levels, and
+// the binary-splitting tree's groups.
+struct PiPlan {
+    uint32_t requested, leaf, convert_digits, prec_digits, prec_limbs, levels, groups;
+};
+PiPlan pi_plan(uint32_t requested_digits, int prec_convention) {
+    PiPlan pl{};
+    pl.requested = requested_digits;
+    pl.leaf = 477;
+    pl.convert_digits = gpupi_round_up_decimal_digits(requested_digits, &pl.leaf);
+    pl.prec_digits = prec_convention == GPUPI_PREC_ROUNDED ? pl.convert_digits : requested_digits;
+    pl.prec_limbs = gpupi_prec_limbs_for_digits(pl.prec_digits);
+    pl.levels = pl.convert_digits ? (uint32_t)__builtin_ctz(pl.convert_digits / pl.leaf) : 0u;
+    pl.groups = pi_groups((int)pl.prec_digits);
+    return pl;
+}
+
+// Everything of a pi job after its binary splitting: the rest of
+// pi_fractional, the power table and the radix conversion
+// (pi_bs_gpu.cu:2137-2193).  Q_bs, R_bs: the root rows of the tree over
+// pl.groups groups, handed over.  elapsed_ms runs from `start`.
+hipError_t pi_finish(BatchMPContext *ctx, const PiPlan &pl, BatchMPArray Q_bs, BatchMPArray R_bs, hipEvent_t start,
+                     GpupiPiResult *result) {
+    hipEvent_t stop = nullptr;
+    if (const hipError_t e = hipEventCreate(&stop); e != hipSuccess) {
+        gpupi::ctx_free(ctx, Q_bs.data);
+        gpupi::ctx_free(ctx, R_bs.data);
+        return e;
+    }
     struct EventGuard {
-        hipEvent_t a, b;
-        ~EventGuard() {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-        }
-    } guard{start, stop};
-    GPUPI_RETURN_IF(hipEventRecord(start, ctx->stream));
+        hipEvent_t e;
+        ~EventGuard() { (void)hipEventDestroy(e); }
+    } guard{stop};
     BatchMPArray ret{};
     uint32_t groups = 0;
     uint64_t inverse_bits = 0;
-    GPUPI_RETURN_IF(gpupi_pi_fractional(ctx, (int)prec_digits, &ret, &groups, &inverse_bits));
+    GPUPI_RETURN_IF(pi_fractional_from_tree(ctx, (int)pl.prec_digits, (int)pl.groups, Q_bs, R_bs, &ret, &groups,
+                                            &inverse_bits));
     CachedPowers powers;
-    hipError_t err = stage(ctx, "decimal_powers", "precompute_5pow", convert_digits, levels, leaf, 0, false,
-                           [&] { return powers.fill(ctx, leaf, levels); });
+    hipError_t err = stage(ctx, "decimal_powers", "precompute_5pow", pl.convert_digits, pl.levels, pl.leaf, 0, false,
+                           [&] { return powers.fill(ctx, pl.leaf, pl.levels); });
     char *digits = nullptr;
-    if (err == hipSuccess) err = gpupi::ctx_alloc(ctx, reinterpret_cast<void **>(&digits), std::max<uint32_t>(convert_digits, 1));
-    if (err == hipSuccess && requested_digits > 0) {
-        const BatchMPArray frac = make_subview(ret, 0, std::min(prec_limbs, ret.length));
-        err = stage(ctx, "decimal_convert", "equal_split", convert_digits, 1, frac.length, 0, false, [&] {
-            return convert_decimal_equal_split(ctx, frac, prec_limbs * 32u, convert_digits, leaf, &powers.table,
-                                               digits);
+    if (err == hipSuccess)
+        err = gpupi::ctx_alloc(ctx, reinterpret_cast<void **>(&digits), std::max<uint32_t>(pl.convert_digits, 1));
+    if (err == hipSuccess && pl.requested > 0) {
+        const BatchMPArray frac = make_subview(ret, 0, std::min(pl.prec_limbs, ret.length));
+        err = stage(ctx, "decimal_convert", "equal_split", pl.convert_digits, 1, frac.length, 0, false, [&] {
+            return convert_decimal_equal_split(ctx, frac, pl.prec_limbs * 32u, pl.convert_digits, pl.leaf,
+                                               &powers.table, digits);
         });
     }
     if (err == hipSuccess) err = hipEventRecord(stop, ctx->stream);
@@ -1409,10 +1458,10 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
         gpupi::ctx_free(ctx, digits);
         return err;
     }
-    result->requested_digits = requested_digits;
-    result->convert_digits = convert_digits;
-    result->leaf_digits = leaf;
-    result->prec_limbs = prec_limbs;
+    result->requested_digits = pl.requested;
+    result->convert_digits = pl.convert_digits;
+    result->leaf_digits = pl.leaf;
+    result->prec_limbs = pl.prec_limbs;
     result->groups = groups;
     result->ret_length = ret.length;
     result->ret = ret.data;
@@ -1420,6 +1469,123 @@ hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec
     result->elapsed_ms = ms;
     result->inverse_bits = inverse_bits;
     return hipSuccess;
+}
+
+struct StartEvent {
+    hipEvent_t e = nullptr;
+    ~StartEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// pi_bs_gpu.cu:2073-2193: the reference's timed region.
+hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec_convention, GpupiPiResult *result) {
+    if (ctx == nullptr || result == nullptr) return hipErrorInvalidValue;
+    std::memset(result, 0, sizeof(*result));
+    const PiPlan pl = pi_plan(requested_digits, prec_convention);
+    StartEvent start;
+    GPUPI_RETURN_IF(hipEventCreate(&start.e));
+    GPUPI_RETURN_IF(hipEventRecord(start.e, ctx->stream));
+    BatchMPArray Q_bs{}, R_bs{};
+    GPUPI_RETURN_IF(gpupi_binary_split(ctx, pl.groups, &Q_bs, &R_bs));
+    return pi_finish(ctx, pl, Q_bs, R_bs, start.e, result);
+}
+
+// Several pi jobs with one binary-splitting tree (no counterpart in the
+// reference, which runs one job per process).  The tree over N groups is the
+// tree over every smaller power of two of groups continued: the leaf terms
+// depend only on the term index, and a level's sign and shift only on how
+// many groups its nodes cover, so the root of a job over N_i groups is row
+// 0 of the level whose nodes cover N_i groups in the tree over the largest
+// N.  That tree runs once; each job's root is copied out (and compacted to
+// its own length) when its level is reached, and the job's remaining stages
+// run at once, smallest jobs first.  Results equal gpupi_pi_run's for each
+// job.  In the plain form the jobs run one after the other.
+hipError_t gpupi_pi_run_batch(BatchMPContext *ctx, const uint32_t *requested_digits, int count, int prec_convention,
+                              GpupiPiResult *results) {
+    if (ctx == nullptr || requested_digits == nullptr || results == nullptr || count <= 0) return hipErrorInvalidValue;
+    for (int i = 0; i < count; ++i) std::memset(&results[i], 0, sizeof(results[i]));
+    const auto fail = [&](hipError_t e) {
+        for (int i = 0; i < count; ++i) {
+            gpupi::ctx_free(ctx, results[i].ret);
+            gpupi::ctx_free(ctx, results[i].digits);
+            std::memset(&results[i], 0, sizeof(results[i]));
+        }
+        return e;
+    };
+    if (ctx->pipeline_form == GPUPI_FORM_PLAIN) {
+        for (int i = 0; i < count; ++i) {
+            const hipError_t e = gpupi_pi_run(ctx, requested_digits[i], prec_convention, &results[i]);
+            if (e != hipSuccess) return fail(e);
+        }
+        return hipSuccess;
+    }
+    std::vector<PiPlan> plans(count);
+    uint32_t n_max = 1;
+    for (int i = 0; i < count; ++i) {
+        plans[i] = pi_plan(requested_digits[i], prec_convention);
+        n_max = std::max(n_max, plans[i].groups);
+    }
+    StartEvent start;
+    hipError_t err = hipEventCreate(&start.e);
+    if (err == hipSuccess) err = hipEventRecord(start.e, ctx->stream);
+    if (err != hipSuccess) return fail(err);
+    TreeBufs t(ctx);
+    BatchMPArray P, Q, R;
+    // the jobs whose tree covers `groups` groups: root = row 0 of Q and R,
+    // copied out for each (at the top level the last one takes the tree's
+    // own buffers, and the tree's other buffers go back before the jobs'
+    // remaining stages run)
+    const auto finish_level = [&](uint32_t groups, bool top) -> hipError_t {
+        int last = -1;
+        for (int i = 0; i < count; ++i)
+            if (plans[i].groups == groups) last = i;
+        if (last < 0) return hipSuccess;
+        struct Root {
+            int job;
+            BatchMPArray Q, R;
+            std::unique_ptr<Buf> bq, br;  // own the rows until the job takes them
+        };
+        std::vector<Root> roots;
+        for (int i = 0; i <= last; ++i) {
+            if (plans[i].groups != groups) continue;
+            Root r{i, {}, {}, std::unique_ptr<Buf>(new Buf(ctx)), std::unique_ptr<Buf>(new Buf(ctx))};
+            if (top && i == last) {
+                r.Q = view_of(t.bQ.ptr, Q.length, 1, Q.stride);
+                r.R = view_of(t.bR.ptr, R.length, 1, R.stride);
+                r.bq->swap(t.bQ);
+                r.br->swap(t.bR);
+            } else {
+                GPUPI_RETURN_IF(r.bq->view(1, Q.length, r.Q));
+                GPUPI_RETURN_IF(r.br->view(1, R.length, r.R));
+                GPUPI_RETURN_IF(hipMemcpyAsync(r.Q.data, Q.data, (size_t)Q.length * 4, hipMemcpyDeviceToDevice,
+                                               ctx->stream));
+                GPUPI_RETURN_IF(hipMemcpyAsync(r.R.data, R.data, (size_t)R.length * 4, hipMemcpyDeviceToDevice,
+                                               ctx->stream));
+                GPUPI_RETURN_IF(compact_many(ctx, {&r.Q, &r.R}, "Q,R", groups));
+            }
+            roots.push_back(std::move(r));
+        }
+        if (top) t.drop();
+        for (Root &r : roots) {
+            r.bq->release();
+            r.br->release();
+            GPUPI_RETURN_IF(pi_finish(ctx, plans[r.job], r.Q, r.R, start.e, &results[r.job]));
+        }
+        return hipSuccess;
+    };
+    err = leaf_stage_fused(ctx, t, n_max, P, Q, R);
+    if (err == hipSuccess) err = compact_many(ctx, {&P, &Q, &R}, "P,Q,R", 2 * n_max);
+    if (err == hipSuccess) err = finish_level(1, n_max == 1);
+    for (uint32_t n1 = n_max; err == hipSuccess && n1 > 1; n1 >>= 1) {
+        err = tree_level(ctx, t, n_max, n1, Lengths::kShared, P, Q, R);
+        if (err == hipSuccess) err = finish_level(2 * n_max / n1, n1 == 2);
+    }
+    return err == hipSuccess ? hipSuccess : fail(err);
 }
 
 hipError_t gpupi_set_memory_pool(BatchMPContext *ctx, hipMemPool_t pool) {

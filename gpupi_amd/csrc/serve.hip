@@ -50,6 +50,7 @@ struct Job {
     uint32_t La = 0, Lb = 0;
     uint32_t *C_out = nullptr;
     uint64_t bytes = 0;
+    bool worker_slot = false;  // a pi job admitted with a worker of its own
     GpupiJobResult res{};
 };
 
@@ -99,6 +100,7 @@ struct GpupiServer {
             j->res.start_ms = t;
             if (j->kind == GPUPI_JOB_PI) {
                 --free_pi;
+                j->worker_slot = true;
                 pi_ready.push_back(std::move(j));
             } else {
                 mul_ready.push_back(std::move(j));
@@ -116,7 +118,7 @@ struct GpupiServer {
         j->res.status = status;
         j->res.finish_ms = now_ms();
         in_use -= j->bytes;
-        if (j->kind == GPUPI_JOB_PI) ++free_pi;
+        if (j->worker_slot) ++free_pi;
         done.push_back(j->res);
         schedule();
         cv_done.notify_all();
@@ -133,43 +135,79 @@ void GpupiServer::pi_worker() {
         (void)gpupi_set_memory_pool(ctx, pool);
     }
     for (;;) {
-        std::unique_ptr<Job> j;
+        // the front job and the admitted pi jobs waiting for a worker whose
+        // tree has as many groups: one shared tree
+        std::vector<std::unique_ptr<Job>> batch;
         {
             std::unique_lock<std::mutex> lk(mu);
             cv_pi.wait(lk, [&] { return stop || !pi_ready.empty(); });
             if (pi_ready.empty()) break;
-            j = std::move(pi_ready.front());
+            batch.push_back(std::move(pi_ready.front()));
             pi_ready.pop_front();
+            // jobs not yet admitted because every worker is busy join it, on
+            // the budget alone: same tree size, in arrival order up to the
+            // first job that does not fit, a fair share (one worker's part of
+            // those jobs) so that the other workers' pipelines still overlap
+            const uint32_t groups = gpupi_pi_groups_for_digits(batch[0]->digits);
+            uint64_t room = cfg.hbm_budget_bytes - in_use;
+            size_t same = 0;
+            for (const auto &w : waiting) {
+                if (w->bytes > room) break;
+                room -= w->bytes;
+                if (w->kind == GPUPI_JOB_PI && gpupi_pi_groups_for_digits(w->digits) == groups) ++same;
+            }
+            const size_t share = (same + cfg.pi_workers - 1) / cfg.pi_workers;
+            const size_t want = std::min<size_t>(cfg.pi_batch_max, 1 + share);
+            const double t = now_ms();
+            for (auto it = waiting.begin(); it != waiting.end() && batch.size() < want;) {
+                Job &w = **it;
+                if (in_use + w.bytes > cfg.hbm_budget_bytes) break;
+                if (w.kind == GPUPI_JOB_PI && gpupi_pi_groups_for_digits(w.digits) == groups) {
+                    in_use += w.bytes;
+                    w.res.start_ms = t;
+                    batch.push_back(std::move(*it));
+                    it = waiting.erase(it);
+                } else {
+                    ++it;
+                }
+            }
         }
+        const size_t n = batch.size();
+        std::vector<uint32_t> digits(n);
+        for (size_t i = 0; i < n; ++i) digits[i] = batch[i]->digits;
+        std::vector<GpupiPiResult> r(n);
         int status = ctx && stream ? hipSuccess : hipErrorNotInitialized;
-        if (status == hipSuccess) {
-            GpupiPiResult r{};
-            status = gpupi_pi_run(ctx, j->digits, GPUPI_PREC_REQUESTED, &r);
-            if (status == hipSuccess) {
-                std::vector<uint32_t> ret(r.ret_length);
+        if (status == hipSuccess)
+            status = gpupi_pi_run_batch(ctx, digits.data(), (int)n, GPUPI_PREC_REQUESTED, r.data());
+        for (size_t i = 0; i < n; ++i) {
+            Job &j = *batch[i];
+            int st = status;
+            if (st == hipSuccess) {
+                std::vector<uint32_t> ret(r[i].ret_length);
                 std::vector<char> own;
-                char *dig = j->digits_out;
+                char *dig = j.digits_out;
                 if (dig == nullptr) {
-                    own.resize(j->digits);
+                    own.resize(j.digits);
                     dig = own.data();
                 }
                 // copies and frees in this worker's stream only: nothing
                 // waits for the other workers' work
-                status = hipMemcpyAsync(ret.data(), r.ret, ret.size() * 4, hipMemcpyDeviceToHost, stream);
-                if (status == hipSuccess && j->digits)
-                    status = hipMemcpyAsync(dig, r.digits, j->digits, hipMemcpyDeviceToHost, stream);
-                (void)hipFreeAsync(r.ret, stream);
-                (void)hipFreeAsync(r.digits, stream);
+                st = hipMemcpyAsync(ret.data(), r[i].ret, ret.size() * 4, hipMemcpyDeviceToHost, stream);
+                if (st == hipSuccess && j.digits)
+                    st = hipMemcpyAsync(dig, r[i].digits, j.digits, hipMemcpyDeviceToHost, stream);
+                (void)hipFreeAsync(r[i].ret, stream);
+                (void)hipFreeAsync(r[i].digits, stream);
                 const hipError_t synced = hipStreamSynchronize(stream);
-                if (status == hipSuccess) status = synced;
-                if (status == hipSuccess) {
-                    j->res.ret_length = r.ret_length;
-                    j->res.ret_hash = ret_hash(ret.data(), r.ret_length);
-                    j->res.dec_hash = dec_hash(dig, j->digits);
+                if (st == hipSuccess) st = synced;
+                if (st == hipSuccess) {
+                    j.res.ret_length = r[i].ret_length;
+                    j.res.ret_hash = ret_hash(ret.data(), r[i].ret_length);
+                    j.res.dec_hash = dec_hash(dig, j.digits);
                 }
             }
+            j.res.batch_size = (uint32_t)n;
+            finish(std::move(batch[i]), st);
         }
-        finish(std::move(j), status);
     }
     if (ctx) batch_mp_destroy(ctx);
     if (stream) (void)hipStreamDestroy(stream);
@@ -272,6 +310,7 @@ GpupiServer *gpupi_server_create(const GpupiServeConfig *cfg) {
     if (s->cfg.hbm_budget_bytes == 0) s->cfg.hbm_budget_bytes = 200ull * 1000 * 1000 * 1000;
     if (s->cfg.pi_workers == 0) s->cfg.pi_workers = 4;
     if (s->cfg.mul_batch_max == 0) s->cfg.mul_batch_max = 512;
+    if (s->cfg.pi_batch_max == 0) s->cfg.pi_batch_max = 8;
     s->t0 = std::chrono::steady_clock::now();
     int device = 0;
     hipMemPoolProps props{};

@@ -45,6 +45,17 @@ typedef struct GpupiPiResult {
 hipError_t gpupi_pi_run(BatchMPContext *ctx, uint32_t requested_digits, int prec_convention,
                         GpupiPiResult *result);
 void gpupi_pi_free(GpupiPiResult *result);
+/* Several pi jobs with one binary-splitting tree (no reference counterpart;
+ * its CLI runs one job per process).  The tree over the largest job's groups
+ * runs once: a job over N groups takes as its root row 0 of the level whose
+ * nodes cover N groups, as soon as that level is reached, and runs its
+ * remaining stages (sqrt, inverse, division, conversion) at once, smallest
+ * jobs first.  results[i] equals gpupi_pi_run(requested_digits[i]) (free
+ * each with gpupi_pi_free); elapsed_ms runs from the call's start to that
+ * job's end.  On an error every result is released and zeroed.  In the
+ * plain pipeline form the jobs run one after the other. */
+hipError_t gpupi_pi_run_batch(BatchMPContext *ctx, const uint32_t *requested_digits, int count, int prec_convention,
+                              GpupiPiResult *results);
 
 /* How the pipeline issues its work (no reference counterpart).
  *   GPUPI_FORM_BATCHED (default): the 17-term group stage as one fused
@@ -191,6 +202,9 @@ hipError_t gpupi_set_memory_pool(BatchMPContext *ctx, hipMemPool_t pool);
 /* Sizing helpers (pi_bs_gpu.cu:924-944) */
 uint32_t gpupi_round_up_decimal_digits(uint32_t requested, uint32_t *leaf_digits);
 uint32_t gpupi_prec_limbs_for_digits(uint32_t digits);
+/* 17-term groups of the binary-splitting tree for a precision of
+ * `prec_digits` digits (pi_bs_gpu.cu:1537-1540). */
+uint32_t gpupi_pi_groups_for_digits(uint32_t prec_digits);
 
 /* Measurement helper (no reference counterpart): the current device's
  * compute units, peak engine clock (kHz) and architecture name, for the

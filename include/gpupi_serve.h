@@ -9,7 +9,13 @@
  *     every pi worker busy lets later jobs go ahead.  A job larger than the
  *     whole budget is refused at submission.
  *   - Pi jobs run on `pi_workers` host threads, each with its own context and
- *     stream, so several pipelines are in flight on the device at once.
+ *     stream, so several pipelines are in flight on the device at once.  A
+ *     worker starting a pi job also takes pi jobs not yet admitted because
+ *     every worker was busy whose binary-splitting tree has the same number
+ *     of groups: in arrival order up to the first job that does not fit the
+ *     budget, at most its share of them (their count over pi_workers) and
+ *     pi_batch_max in all.  They share one tree (gpupi_pi_run_batch) and one
+ *     worker, and report batch_size.
  *   - Multiply jobs are batched continuously: the multiply lane takes every
  *     admitted multiply job (up to mul_batch_max) into one shared ragged
  *     product launch; jobs arriving meanwhile form the next batch.
@@ -34,6 +40,7 @@ typedef struct GpupiServeConfig {
     uint64_t hbm_budget_bytes; /* admission budget; 0 = 200 GB */
     uint32_t pi_workers;       /* pi jobs in flight at once; 0 = 4 */
     uint32_t mul_batch_max;    /* multiply jobs per shared launch; 0 = 512 */
+    uint32_t pi_batch_max;     /* pi jobs of one tree size sharing a tree; 0 = 8, 1 = none */
 } GpupiServeConfig;
 
 typedef struct GpupiJobResult {
@@ -44,7 +51,7 @@ typedef struct GpupiJobResult {
     double start_ms;         /* admitted and started */
     double finish_ms;        /* result in the caller's buffer */
     uint64_t reserved_bytes; /* admission estimate it held */
-    uint32_t batch_size;     /* multiply jobs: jobs in its shared launch */
+    uint32_t batch_size;     /* jobs in its shared launch (multiply) or tree (pi) */
     uint32_t ret_length;     /* pi jobs: RET limbs */
     uint32_t ret_hash;       /* pi jobs: RET mod 1e9+7 (pi_bs_gpu.cu:852-858) */
     uint32_t dec_hash;       /* pi jobs: hash of all digits (pi_bs_gpu.cu:860-866) */

@@ -95,6 +95,22 @@ hipError_t gpupi_pi_run(BatchMPContext *, uint32_t digits, int, GpupiPiResult *r
     return hipSuccess;
 }
 void gpupi_pi_free(GpupiPiResult *r) { free(r->ret); free(r->digits); }
+uint32_t gpupi_pi_groups_for_digits(uint32_t d) {
+    uint32_t n = 1;
+    while (n * 17 * 14.3 < d) n *= 2;
+    return n;
+}
+static std::atomic<int> g_max_pi_batch{0};
+hipError_t gpupi_pi_run_batch(BatchMPContext *c, const uint32_t *digits, int count, int prec, GpupiPiResult *r) {
+    for (int i = 0; i < count; ++i) {
+        if (gpupi_pi_groups_for_digits(digits[i]) != gpupi_pi_groups_for_digits(digits[0])) return hipErrorInvalidValue;
+        gpupi_pi_run(c, digits[i], prec, &r[i]);
+    }
+    int seen = g_max_pi_batch.load();
+    while (count > seen && !g_max_pi_batch.compare_exchange_weak(seen, count)) {
+    }
+    return hipSuccess;
+}
 hipError_t gpupi_cache_trim(BatchMPContext *) { jitter(200); return hipSuccess; }
 hipError_t gpupi_mul_ragged(BatchMPContext *, const GpupiMulItem *items, uint32_t count) {
     jitter(2000);
@@ -138,7 +154,7 @@ static void serialized(int pi_workers) {
 
 static void scenario(int seed, uint64_t budget, int pi_workers, int jobs) {
     std::mt19937 rng(seed);
-    GpupiServeConfig cfg{budget, (uint32_t)pi_workers, 8};
+    GpupiServeConfig cfg{budget, (uint32_t)pi_workers, 8, 1 + (uint32_t)(seed % 5)};
     GpupiServer *s = gpupi_server_create(&cfg);
     struct Mul {
         std::vector<uint32_t> A, B, C;
@@ -200,6 +216,7 @@ static void scenario(int seed, uint64_t budget, int pi_workers, int jobs) {
     for (auto &r : results) {
         seen[r.id]++;
         EXPECT(r.status == 0, "status");
+        EXPECT(r.kind != 0 || (r.batch_size >= 1 && r.batch_size <= cfg.pi_batch_max), "pi batch size");
         EXPECT(r.submit_ms <= r.start_ms && r.start_ms <= r.finish_ms, "timestamps");
         EXPECT(r.kind == kind[r.id], "kind");
         if (r.kind == 0) {
@@ -241,6 +258,8 @@ int main() {
         scenario(seed, 200ull << 30, 1 + seed % 4, 60);
         scenario(1000 + seed, tight, 1 + seed % 3, 30);
     }
+    // pi jobs of one tree size did share trees somewhere in the mixes
+    EXPECT(g_max_pi_batch.load() > 1, "no pi batch formed");
     finished = true;
     watchdog.join();
     printf(fails ? "serve sim: %d failures\n" : "serve sim: ok\n", fails);

@@ -92,6 +92,50 @@ def test_pi_matches_oracle(ctx, digits):
     assert dec == odec
 
 
+BATCHES = [
+    [1000, 5000, 5100, 30000, 100000, 100000, 250000],  # nested trees, equal ones, one size twice
+    [0, 1, 9, 100, 575, 576],                           # one-group trees and the empty job
+    [200000, 210000, 220000],                           # one tree size
+]
+
+
+@pytest.mark.parametrize("digits", BATCHES, ids=["nested", "tiny", "same_tree"])
+@pytest.mark.parametrize("convention", [g.PREC_REQUESTED, g.PREC_ROUNDED], ids=["requested", "rounded"])
+def test_pi_batch_equals_single_runs(ctx, digits, convention):
+    # jobs sharing one binary-splitting tree: each result equals the job run
+    # alone (RET limbs, digits, sizes), in input order
+    batch = ctx.pi_batch(digits, convention)
+    groups = [g.lib().gpupi_pi_groups_for_digits(r.convert_digits if convention == g.PREC_ROUNDED else d)
+              for d, (_, _, r) in zip(digits, batch)]
+    for d, (ret, dec, res), n in zip(digits, batch, groups):
+        oret, odec, ores = ctx.pi(d, convention)
+        assert (res.convert_digits, res.ret_length, res.groups, res.inverse_bits) == \
+            (ores.convert_digits, ores.ret_length, ores.groups, ores.inverse_bits), d
+        assert res.groups == n
+        assert np.array_equal(ret, oret), d
+        assert dec == odec, d
+
+
+def test_pi_batch_matches_oracle_and_plain_form(ctx):
+    digits = [3000, 40000, 41000]
+    batch = ctx.pi_batch(digits)
+    for d, (ret, dec, _) in zip(digits, batch):
+        oret, odec, _ = of.pi_digits(d)
+        assert np.array_equal(ret, oret) and dec == odec, d
+    with g.Context() as plain:
+        plain.set_form(g.FORM_PLAIN)
+        for (ret, dec, _), (pret, pdec, _) in zip(batch, plain.pi_batch(digits)):
+            assert np.array_equal(ret, pret) and dec == pdec
+
+
+def test_pi_batch_rejects_bad_arguments(ctx):
+    res = (g.GpupiPiResult * 1)()
+    arr = (ctypes.c_uint32 * 1)(100)
+    assert g.lib().gpupi_pi_run_batch(ctx.ptr, arr, 0, 0, res) != 0
+    assert g.lib().gpupi_pi_run_batch(ctx.ptr, None, 1, 0, res) != 0
+    assert g.lib().gpupi_pi_run_batch(ctx.ptr, arr, 1, 0, None) != 0
+
+
 def test_repeat_run_reuses_cached_blocks():
     # the second run of one size takes its buffers from the context's block
     # cache (RET's block leaves the cache with the result; best fit may pair

@@ -49,6 +49,33 @@ def test_mix_equals_alone_and_oracle():
     assert max(r.batch_size for k, (r, _) in mix.items() if k[0] == "mul") > 1
 
 
+def test_pi_jobs_of_one_tree_size_share_a_tree():
+    # one pi worker: the jobs queued behind the first, all of one tree size,
+    # join one batch (one shared tree); every result equals the job served
+    # with batching off and the oracle
+    sizes = [40000, 45000, 50000, 55000, 60000]
+    assert len({g.lib().gpupi_pi_groups_for_digits(d) for d in sizes}) == 1
+    srv = g.Server(pi_workers=1)
+    try:
+        batched = run_jobs(srv, sizes, [])
+    finally:
+        srv.close()
+    srv = g.Server(pi_workers=1, pi_batch_max=1)
+    try:
+        single = run_jobs(srv, sizes, [])
+    finally:
+        srv.close()
+    assert max(r.batch_size for r, _ in batched.values()) > 1
+    assert all(r.batch_size == 1 for r, _ in single.values())
+    for d in sizes:
+        r, data = batched[("pi", d)]
+        rs, ds = single[("pi", d)]
+        assert (r.ret_hash, r.dec_hash, r.ret_length) == (rs.ret_hash, rs.dec_hash, rs.ret_length), d
+        assert data == ds
+        _, odec, _ = of.pi_digits(d)
+        assert data.decode() == odec
+
+
 def test_pool_stays_within_budget_under_load():
     # many jobs against a budget of about three jobs: the pool's peak (all
     # workers' pipeline buffers, workspaces and arenas) stays within it
