@@ -9,6 +9,7 @@
 #include <new>
 
 #include "common.hpp"
+#include "gpupi_pi.h"
 
 namespace {
 
@@ -325,6 +326,45 @@ hipError_t batch_mp_set_ntt_threshold(BatchMPContext *ctx, uint32_t min_total_li
     return hipSuccess;
 }
 
+namespace {
+
+// A large one-row product whose transform is mostly padding: with L the
+// longer operand, S the shorter and K = 2^k >= L + S its transform, when
+// L + 2 S <= K the pieces L[0, s) and L[s, L) with s = K/2 - S each fit
+// K/2 points with S (L - s <= s) and share S's transform
+// (batch_mp_mul_shared): five transforms of K/2 points instead of three of
+// K.  C[0, s + S) = L[0, s) S directly, the top of C is cleared, and
+// C[s, L + S) += L[s, L) S.  Transforms of 2^22 points and up (smaller
+// products are bound by launches, not by transform size).
+constexpr int kSplitMinLog2 = 22;
+
+bool split_point(uint32_t L, uint32_t S, uint32_t *s) {
+    const int k = gpupi::ntt_transform_log2(L, S);
+    if (k < kSplitMinLog2) return false;
+    const uint64_t half = 1ull << (k - 1);
+    if ((uint64_t)L + 2ull * S > 2 * half || (uint64_t)S >= half) return false;
+    *s = (uint32_t)(half - S);
+    return L > *s;
+}
+
+hipError_t mul_split(BatchMPContext *ctx, const BatchMPArray &L, const BatchMPArray &S, uint32_t s,
+                     const BatchMPArray &C) {
+    const uint32_t hi = L.length - s, lt = hi + S.length;
+    const BatchMPArray L0{L.data, s, 1, s}, L1{L.data + s, hi, 1, hi};
+    const BatchMPArray C0{C.data, s + S.length, 1, s + S.length}, Ct{C.data + s, lt, 1, lt};
+    void *t = nullptr;
+    size_t got = 0;
+    GPUPI_RETURN_IF(ctx->cache->acquire(ctx->stream, (size_t)lt * sizeof(uint32_t), &t, &got));
+    const BatchMPArray T{static_cast<uint32_t *>(t), lt, 1, lt};
+    hipError_t err = gpupi::launch_fill_u32(C.data + s + S.length, hi, 0u, ctx->stream);
+    if (err == hipSuccess) err = batch_mp_mul_shared(ctx, L0, L1, S, C0, T);
+    if (err == hipSuccess) err = batch_mp_add(ctx, Ct, T, Ct);
+    ctx->cache->give_back(t, got, ctx->stream);
+    return err;
+}
+
+}  // namespace
+
 hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, BatchMPArray C) {
     if (ctx == nullptr || !valid_array(A) || !valid_array(B) || !valid_output(C)) return hipErrorInvalidValue;
     if (!same_batch(A, B) || !same_batch(A, C)) return hipErrorInvalidValue;
@@ -338,6 +378,12 @@ hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, Bat
         if (!gpupi::mul_naive_supports(A.length, B.length)) return hipErrorInvalidValue;
         return gpupi::launch_mul_naive(A.data, B.data, C.data, C.batch_size, A.length, B.length, A.stride,
                                        B.stride, C.stride, ctx->stream);
+    }
+    if (C.batch_size == 1 && ctx->pipeline_form != GPUPI_FORM_PLAIN) {
+        const bool a_long = A.length >= B.length;
+        uint32_t s = 0;
+        if (split_point(a_long ? A.length : B.length, a_long ? B.length : A.length, &s))
+            return mul_split(ctx, a_long ? A : B, a_long ? B : A, s, C);
     }
     GPUPI_RETURN_IF(ensure(ctx, gpupi::ntt_workspace_bytes(C.batch_size, A.length, B.length)));
     return gpupi::launch_mul_ntt(A.data, B.data, C.data, ctx->workspace, ctx->workspace_size_bytes,

@@ -323,6 +323,35 @@ def test_ntt_every_split(ctx, k):
     assert g.ntt_get_split(k) == default
 
 
+SPLIT = [  # one row with L + 2 S <= K: the longer operand runs as two pieces at K / 2
+    (3000000, 500000, "random"),    # K = 2^22
+    (500000, 3000000, "random"),    # the longer operand second
+    (2200000, 900000, "ones"),      # carries through the seam
+    (6000000, 1000001, "top"),      # K = 2^23
+]
+
+
+@pytest.mark.parametrize("la,lb,mode", SPLIT, ids=[f"{a}x{b}_{m}" for a, b, m in SPLIT])
+def test_unbalanced_product_splits_and_stays_exact(ctx, la, lb, mode):
+    rng = np.random.default_rng(la ^ lb)
+    A = rand_rows(rng, 1, la, mode)
+    B = rand_rows(rng, 1, lb, mode)
+    want = oracle_mul(A, B)
+    g.kernel_stats_reset(True)
+    try:
+        got = device_mul(ctx, A, B, sa=la + 3, sb=lb + 1)
+        sizes = {name.split("@2^")[1] for name, *_ in g.kernel_stats(by_size=True)}
+    finally:
+        g.kernel_stats_reset(False)
+    k = (la + lb - 1).bit_length()
+    assert str(k - 1) in sizes and str(k) not in sizes, sizes
+    assert np.array_equal(got, want)
+    # the plain form keeps the reference's one product at K points
+    with g.Context() as plain:
+        plain.set_form(g.FORM_PLAIN)
+        assert np.array_equal(device_mul(plain, A, B), want)
+
+
 LAUNCH_K = [10, 16, 20, 23, 24]
 
 
