@@ -328,27 +328,54 @@ hipError_t batch_mp_set_ntt_threshold(BatchMPContext *ctx, uint32_t min_total_li
 
 namespace {
 
-// A large one-row product whose transform is mostly padding: with L the
-// longer operand, S the shorter and K = 2^k >= L + S its transform, when
-// L + 2 S <= K the pieces L[0, s) and L[s, L) with s = K/2 - S each fit
-// K/2 points with S (L - s <= s) and share S's transform
-// (batch_mp_mul_shared): five transforms of K/2 points instead of three of
-// K.  C[0, s + S) = L[0, s) S directly, the top of C is cleared, and
-// C[s, L + S) += L[s, L) S.  Transforms of 2^22 points and up (smaller
+// One-row products whose transform is mostly padding, or whose shorter
+// operand is far shorter than the longer (L the longer, S the shorter,
+// K = 2^k >= L + S the transform), run in pieces of the longer operand that
+// share the shorter one's transform.  Transform work counts 3 K for the
+// product as it is, and (2 m + 1) K' for m pieces of K' points (m forward,
+// m inverse, one of S):
+//   two pieces at K' = K/2 when L + 2 S <= K (2.5 K): C[0, s + S) =
+//     L[0, s) S directly, C[s, L + S) += L[s, L) S, s = K/2 - S;
+//   m >= 3 pieces of s = K' - S >= S limbs at a smaller K' when that costs
+//     at most 0.8 of 3 K: the pieces' products (m - 1 rows against S
+//     broadcast, one batched product, and the last piece) go to even and odd
+//     accumulators (pieces of one parity never overlap since s >= S), which
+//     one full-length add sums.
+// Transforms of 2^22 points and up for two pieces, 2^20 for more (smaller
 // products are bound by launches, not by transform size).
-constexpr int kSplitMinLog2 = 22;
+constexpr int kSplitMinLog2 = 22, kSplitManyMinLog2 = 20;
 
-bool split_point(uint32_t L, uint32_t S, uint32_t *s) {
+struct SplitPlan {
+    uint32_t m = 0;  // pieces: 0 = none
+    uint32_t s = 0;  // piece length (the last may be shorter)
+};
+
+SplitPlan split_plan(uint32_t L, uint32_t S) {
+    SplitPlan p;
     const int k = gpupi::ntt_transform_log2(L, S);
-    if (k < kSplitMinLog2) return false;
-    const uint64_t half = 1ull << (k - 1);
-    if ((uint64_t)L + 2ull * S > 2 * half || (uint64_t)S >= half) return false;
-    *s = (uint32_t)(half - S);
-    return L > *s;
+    const uint64_t K = 1ull << k;
+    double best = 3.0 * (double)K;
+    if (k >= kSplitMinLog2 && (uint64_t)L + 2ull * S <= K && (uint64_t)S < K / 2) {
+        p.m = 2;
+        p.s = (uint32_t)(K / 2 - S);
+        best = 2.5 * (double)K;
+    }
+    if (k < kSplitManyMinLog2) return p;
+    for (uint64_t Kp = K / 2; Kp >= 2ull * S && Kp >= (1ull << 13); Kp /= 2) {
+        const uint64_t s = Kp - S;
+        const uint64_t m = ((uint64_t)L + s - 1) / s;
+        const double cost = (2.0 * (double)m + 1.0) * (double)Kp;
+        if (m >= 3 && cost <= 0.8 * 3.0 * (double)K && cost < best) {
+            best = cost;
+            p.m = (uint32_t)m;
+            p.s = (uint32_t)s;
+        }
+    }
+    return p;
 }
 
-hipError_t mul_split(BatchMPContext *ctx, const BatchMPArray &L, const BatchMPArray &S, uint32_t s,
-                     const BatchMPArray &C) {
+hipError_t mul_split_two(BatchMPContext *ctx, const BatchMPArray &L, const BatchMPArray &S, uint32_t s,
+                         const BatchMPArray &C) {
     const uint32_t hi = L.length - s, lt = hi + S.length;
     const BatchMPArray L0{L.data, s, 1, s}, L1{L.data + s, hi, 1, hi};
     const BatchMPArray C0{C.data, s + S.length, 1, s + S.length}, Ct{C.data + s, lt, 1, lt};
@@ -360,6 +387,48 @@ hipError_t mul_split(BatchMPContext *ctx, const BatchMPArray &L, const BatchMPAr
     if (err == hipSuccess) err = batch_mp_mul_shared(ctx, L0, L1, S, C0, T);
     if (err == hipSuccess) err = batch_mp_add(ctx, Ct, T, Ct);
     ctx->cache->give_back(t, got, ctx->stream);
+    return err;
+}
+
+hipError_t mul_split_many(BatchMPContext *ctx, const BatchMPArray &L, const BatchMPArray &S, uint32_t m,
+                          uint32_t s, const BatchMPArray &C) {
+    const uint32_t w = s + S.length;                  // a piece's product
+    const uint32_t last = L.length - (m - 1) * s;     // the last piece, 1..s limbs
+    const size_t p_words = (size_t)m * w, c_words = C.length;
+    void *pp = nullptr, *op = nullptr;
+    size_t pgot = 0, ogot = 0;
+    GPUPI_RETURN_IF(ctx->cache->acquire(ctx->stream, p_words * sizeof(uint32_t), &pp, &pgot));
+    hipError_t err = ctx->cache->acquire(ctx->stream, c_words * sizeof(uint32_t), &op, &ogot);
+    if (err != hipSuccess) {
+        ctx->cache->give_back(pp, pgot, ctx->stream);
+        return err;
+    }
+    uint32_t *P = static_cast<uint32_t *>(pp), *O = static_cast<uint32_t *>(op);
+    // pieces 0..m-2 against S broadcast (S transformed once), then the last
+    const BatchMPArray Ar{L.data, s, m - 1, s}, Sb{S.data, S.length, m - 1, 0}, Pr{P, w, m - 1, w};
+    const BatchMPArray Al{L.data + (size_t)(m - 1) * s, last, 1, last};
+    const BatchMPArray Pl{P + (size_t)(m - 1) * w, last + S.length, 1, last + S.length};
+    err = batch_mp_mul(ctx, Ar, Sb, Pr);
+    if (err == hipSuccess) err = batch_mp_mul(ctx, Al, S, Pl);
+    // even pieces into C, odd ones into O, each at 32 s i bits
+    if (err == hipSuccess) err = gpupi::launch_fill_u32(C.data, c_words, 0u, ctx->stream);
+    if (err == hipSuccess) err = gpupi::launch_fill_u32(O, c_words, 0u, ctx->stream);
+    const uint32_t full = m - 1;  // pieces copied as whole rows of w limbs
+    for (uint32_t parity = 0; parity < 2 && err == hipSuccess; ++parity) {
+        const uint32_t rows = full > parity ? (full - parity + 1) / 2 : 0;
+        uint32_t *dst = parity ? O : C.data;
+        if (rows > 0)
+            err = hipMemcpy2DAsync(dst + (size_t)parity * s, (size_t)2 * s * sizeof(uint32_t), P + (size_t)parity * w,
+                                   (size_t)2 * w * sizeof(uint32_t), (size_t)w * sizeof(uint32_t), rows,
+                                   hipMemcpyDeviceToDevice, ctx->stream);
+    }
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(((m - 1) & 1 ? O : C.data) + (size_t)(m - 1) * s, Pl.data,
+                             (size_t)Pl.length * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream);
+    const BatchMPArray Cf{C.data, C.length, 1, C.length}, Of{O, C.length, 1, C.length};
+    if (err == hipSuccess) err = batch_mp_add(ctx, Cf, Of, Cf);
+    ctx->cache->give_back(op, ogot, ctx->stream);
+    ctx->cache->give_back(pp, pgot, ctx->stream);
     return err;
 }
 
@@ -381,9 +450,10 @@ hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, Bat
     }
     if (C.batch_size == 1 && ctx->pipeline_form != GPUPI_FORM_PLAIN) {
         const bool a_long = A.length >= B.length;
-        uint32_t s = 0;
-        if (split_point(a_long ? A.length : B.length, a_long ? B.length : A.length, &s))
-            return mul_split(ctx, a_long ? A : B, a_long ? B : A, s, C);
+        const BatchMPArray &L = a_long ? A : B, &S = a_long ? B : A;
+        const SplitPlan sp = split_plan(L.length, S.length);
+        if (sp.m == 2) return mul_split_two(ctx, L, S, sp.s, C);
+        if (sp.m >= 3) return mul_split_many(ctx, L, S, sp.m, sp.s, C);
     }
     GPUPI_RETURN_IF(ensure(ctx, gpupi::ntt_workspace_bytes(C.batch_size, A.length, B.length)));
     return gpupi::launch_mul_ntt(A.data, B.data, C.data, ctx->workspace, ctx->workspace_size_bytes,
@@ -480,6 +550,23 @@ hipError_t gpupi_mul_ragged(BatchMPContext *ctx, const GpupiMulItem *items, uint
         if (small)
             GPUPI_RETURN_IF(gpupi::launch_mul_naive(it.A, it.B, it.C, 1, it.L_a, it.L_b, it.L_a, it.L_b,
                                                     (uint32_t)total, ctx->stream));
+    }
+    // products that gain from running in pieces (batch_mp_mul splits them),
+    // before the grouped launches below take the workspace
+    if (ctx->pipeline_form != GPUPI_FORM_PLAIN) {
+        std::vector<std::pair<int, uint32_t>> rest;
+        for (const auto &b : big) {
+            const GpupiMulItem &it = items[b.second];
+            if (split_plan(std::max(it.L_a, it.L_b), std::min(it.L_a, it.L_b)).m >= 2) {
+                const uint32_t total = it.L_a + it.L_b;
+                GPUPI_RETURN_IF(batch_mp_mul(ctx, BatchMPArray{const_cast<uint32_t *>(it.A), it.L_a, 1, it.L_a},
+                                             BatchMPArray{const_cast<uint32_t *>(it.B), it.L_b, 1, it.L_b},
+                                             BatchMPArray{it.C, total, 1, total}));
+            } else {
+                rest.push_back(b);
+            }
+        }
+        big.swap(rest);
     }
     if (big.empty()) return hipSuccess;
     std::stable_sort(big.begin(), big.end(),

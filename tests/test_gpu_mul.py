@@ -329,6 +329,13 @@ SPLIT = [  # one row with L + 2 S <= K: the longer operand runs as two pieces at
     (2200000, 900000, "ones"),      # carries through the seam
     (6000000, 1000001, "top"),      # K = 2^23
 ]
+SPLIT_MANY = [  # one row with S far shorter than L: many pieces at a small transform
+    (4194309, 3000, "random"),      # K = 2^23, pieces of 2^16 - 2^17 points
+    (3000, 4194309, "ones"),        # the longer operand second, carries through every seam
+    (1500000, 40000, "top"),        # K = 2^21
+    (1048600, 200000, "random"),    # K = 2^21, a few pieces
+    (2500001, 70001, "sparse"),
+]
 
 
 @pytest.mark.parametrize("la,lb,mode", SPLIT, ids=[f"{a}x{b}_{m}" for a, b, m in SPLIT])
@@ -350,6 +357,36 @@ def test_unbalanced_product_splits_and_stays_exact(ctx, la, lb, mode):
     with g.Context() as plain:
         plain.set_form(g.FORM_PLAIN)
         assert np.array_equal(device_mul(plain, A, B), want)
+
+
+@pytest.mark.parametrize("la,lb,mode", SPLIT_MANY, ids=[f"{a}x{b}_{m}" for a, b, m in SPLIT_MANY])
+def test_very_unbalanced_product_runs_in_pieces_and_stays_exact(ctx, la, lb, mode):
+    rng = np.random.default_rng(la + 7 * lb)
+    A = rand_rows(rng, 1, la, mode)
+    B = rand_rows(rng, 1, lb, mode)
+    want = oracle_mul(A, B)
+    g.kernel_stats_reset(True)
+    try:
+        got = device_mul(ctx, A, B, sa=la + 1, sb=lb + 2)
+        sizes = {int(name.split("@2^")[1]) for name, *_ in g.kernel_stats(by_size=True)}
+    finally:
+        g.kernel_stats_reset(False)
+    k = (la + lb - 1).bit_length()
+    assert sizes and max(sizes) <= k - 2, sizes
+    assert np.array_equal(got, want)
+    # the same pairs through the ragged call (pieces too) and mixed with a
+    # pair that stays whole
+    C = rand_rows(rng, 1, 5000)
+    D = rand_rows(rng, 1, 4000)
+    bufs = [g.DeviceBuffer.from_array(x[0]) for x in (A, B, C, D)]
+    outs = [g.DeviceBuffer.from_array(np.full(n + 1, PAD, dtype=np.uint32)) for n in (la + lb, 9000)]
+    g.check(ctx.mul_ragged([(bufs[0].ptr, bufs[1].ptr, outs[0].ptr, la, lb),
+                            (bufs[2].ptr, bufs[3].ptr, outs[1].ptr, 5000, 4000)]), "mul_ragged")
+    g.synchronize()
+    o0, o1 = outs[0].download(np.uint32), outs[1].download(np.uint32)
+    assert o0[-1] == PAD and o1[-1] == PAD
+    assert np.array_equal(o0[:-1], want[0])
+    assert np.array_equal(o1[:-1], oracle_mul(C, D)[0])
 
 
 LAUNCH_K = [10, 16, 20, 23, 24]
