@@ -395,12 +395,16 @@ def ragged_pairs(seed: int, count: int, lo: float, hi: float):
 def bench_ragged(args, g) -> dict:
     """256 pairs, lengths log-uniform in [2^10, 2^24] limbs, uniform random
     limbs (seed 1234): one ragged call (gpupi_mul_ragged) against one
-    batch_mp_mul call per pair, with inputs resident in HBM.  Reports ms for
-    both forms, per-pass NTT GB/s of the ragged form, and the reference's
-    compulsory figure 4*sum(L_a+L_b+L_c)/t (batch_mul_ntt_test.cu:491-492)."""
+    batch_mp_mul call per pair in the reference's plain form (each pair one
+    whole product; GPUPI_FORM_PLAIN) and in the batched form (unbalanced
+    pairs in pieces), with inputs resident in HBM.  Reports ms for each,
+    per-pass NTT GB/s of the ragged form, and the reference's compulsory
+    figure 4*sum(L_a+L_b+L_c)/t (batch_mul_ntt_test.cu:491-492)."""
     import numpy as np
     rng, lengths = ragged_pairs(1234, args.pairs, 10.0, 24.0)
     ctx = g.Context()
+    ctx_plain = g.Context()
+    ctx_plain.set_form(g.FORM_PLAIN)
     bufs, items, uni = [], [], []
     for la, lb in lengths:
         A = rng.integers(0, 1 << 32, size=la, dtype=np.uint64).astype(np.uint32)
@@ -416,13 +420,18 @@ def bench_ragged(args, g) -> dict:
 
     def plain():
         for a_, b_, c_ in uni:
+            g.check(ctx_plain.mul(a_, b_, c_), "mul")
+
+    def single():
+        for a_, b_, c_ in uni:
             g.check(ctx.mul(a_, b_, c_), "mul")
 
     def ragged():
         g.check(ctx.mul_ragged(items), "gpupi_mul_ragged")
 
     res = {}
-    for name, fn in (("plain", plain), ("ragged", ragged)):
+    same_single = True
+    for name, fn in (("plain", plain), ("single", single), ("ragged", ragged)):
         for _ in range(max(args.warmup, 1)):
             fn()
         g.synchronize()
@@ -431,6 +440,11 @@ def bench_ragged(args, g) -> dict:
             fn()
         g.synchronize()
         res[name] = (time.perf_counter() - t0) * 1000.0 / steps
+        if name == "plain":
+            plain_out = [bu.download(np.uint32) for _, _, _, bu in bufs]
+        elif name == "single":
+            same_single = all(np.array_equal(bu.download(np.uint32), p_) for (_, _, _, bu), p_ in zip(bufs, plain_out))
+            del plain_out
     g.kernel_stats_reset(True)
     ragged()
     g.synchronize()
@@ -466,9 +480,13 @@ def bench_ragged(args, g) -> dict:
                                          f"({limbs_cpu} limbs in+out) in {t_cpu:.1f} s, one thread",
                "gpu_products_checked_bit_exact": exact}
     ctx.close()
+    ctx_plain.close()
+    same = same and same_single
     return {"metric": f"ms per batched product of {len(lengths)} ragged pairs (one ragged call)",
             "value": round(res["ragged"], 3), "unit": "ms", "higher_is_better": False, "steps": steps,
             "plain_ms": round(res["plain"], 3), "speedup_ragged_vs_plain": round(res["plain"] / res["ragged"], 3),
+            "single_calls_batched_form_ms": round(res["single"], 3),
+            "speedup_ragged_vs_single_calls": round(res["single"] / res["ragged"], 3),
             "a_b_ret_GBps": round(4.0 * limbs / (res["ragged"] / 1e3) / 1e9, 1),
             "total_limbs_in_out": limbs, "identical_to_plain": same, "ntt_passes": ntt_table(stats, 1),
             "roofline": hbm_roofline(stats), "cpu_baseline": cpu}
