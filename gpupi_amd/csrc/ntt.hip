@@ -343,8 +343,7 @@ struct PassArgs {
 struct BlockPos {
     uint32_t seg, row, t;
 };
-__device__ __forceinline__ BlockPos block_pos(const uint32_t *start, uint32_t nseg, uint32_t tiles, int xcd_log2) {
-    const uint32_t bid = xcd_tile(blockIdx.x, gridDim.x, xcd_log2);
+__device__ __forceinline__ BlockPos block_pos_of(uint32_t bid, const uint32_t *start, uint32_t nseg, uint32_t tiles) {
     uint32_t seg = 0, first = 0;
     for (uint32_t i = 1; i < nseg; ++i)
         if (bid >= start[i]) {
@@ -353,6 +352,9 @@ __device__ __forceinline__ BlockPos block_pos(const uint32_t *start, uint32_t ns
         }
     const uint32_t local = bid - first;
     return BlockPos{seg, local / tiles, local % tiles};
+}
+__device__ __forceinline__ BlockPos block_pos(const uint32_t *start, uint32_t nseg, uint32_t tiles, int xcd_log2) {
+    return block_pos_of(xcd_tile(blockIdx.x, gridDim.x, xcd_log2), start, nseg, tiles);
 }
 
 // Column kernels of DFT size 2^M >= 2^7 (tiles of 32 or fewer adjacent
@@ -446,15 +448,13 @@ __device__ __forceinline__ void col_fwd_prime(double *v, double *lds, const ColT
     if (BARRIER) __syncthreads();
 }
 
-// Forward column level: blocks cover (product, operand, tile); both primes
-// pass through the tile in turn; the inter-level twiddle is applied on
-// store.  Level 0 reads limbs.  At level 0 every global load (the limbs and
-// both primes' column twiddles) is issued before the first store and both
-// primes' results are stored at the end: loads and stores share one
-// in-order counter, so a load issued after a store also waits for the
-// store.  Above level 0 the second prime's data is loaded after the first
-// prime's store, which keeps the kernel within 128 VGPRs.
-template <bool FROM_LIMBS, int M, bool PAIR>
+// Forward column level above the first: blocks cover (product, operand,
+// tile); both primes pass through the tile in turn; the inter-level twiddle
+// is applied on store.  Every global load (both primes' residues and column
+// twiddles) is issued before the first store and both primes' results are
+// stored at the end: loads and stores share one in-order counter, so a
+// load issued after a store also waits for the store.
+template <int M, bool PAIR>
 __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_kernel(PassArgs a, int level) {
     using CP = ColPair<M, PAIR>;
     __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
@@ -465,27 +465,10 @@ __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_kern
     double *T0 = a.T + o.base + (size_t)bp.row * plane_row_elems(a.plan);
     double *T1 = T0 + o.pstride;
     double v0[16], v1[16];
-    if (FROM_LIMBS) {
-        const uint32_t *X = o.X + (size_t)bp.row * o.s;
-        uint32_t LX = o.L;
-        if (a.items) {
-            const GpupiMulItem it = a.items[bp.row];
-            X = bp.seg ? it.B : it.A;
-            LX = bp.seg ? it.L_b : it.L_a;
-        }
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const uint64_t o = ct.off(u);
-            const uint32_t x = o < LX ? X[o] : 0u;
-            v0[u] = (double)x;
-            v1[u] = (double)x;
-        }
-    } else {
+    for (int u = 0; u < 16; ++u) v0[u] = T0[ct.poff(u)];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v0[u] = T0[ct.poff(u)];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) v1[u] = T1[ct.poff(u)];
-    }
+    for (int u = 0; u < 16; ++u) v1[u] = T1[ct.poff(u)];
     const ColTw tw0 = col_tw_load<M>(a.tables, ct.cg, ct.B0, false);
     const ColTw tw1 = col_tw_load<M>(a.tables + kPrimeTable, ct.cg, ct.B0, false);
     col_fwd_prime<M, 0, true>(v0, lds, ct, a.tables, tw0);
@@ -494,6 +477,78 @@ __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_kern
     for (int k = 0; k < 16; ++k) T0[ct.plast(k)] = v0[k];
 #pragma unroll
     for (int k = 0; k < 16; ++k) T1[ct.plast(k)] = v1[k];
+}
+
+// Limbs of one tile block for the first forward level: element u of the
+// thread's column (zero past the operand's length).
+template <int M>
+__device__ __forceinline__ void load_limbs(uint32_t *x, const PassArgs &a, const BlockPos &bp, const ColThread<M> &ct) {
+    const NttOp o = a.ops[bp.seg];
+    const uint32_t *X = o.X + (size_t)bp.row * o.s;
+    uint32_t LX = o.L;
+    if (a.items) {
+        const GpupiMulItem it = a.items[bp.row];
+        X = bp.seg ? it.B : it.A;
+        LX = bp.seg ? it.L_b : it.L_a;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const uint64_t off = ct.off(u);
+        x[u] = off < LX ? X[off] : 0u;
+    }
+}
+
+// First forward column level, persistent: block b works through the tile
+// blocks w = b, b + gridDim.x, ... (numbered and XCD-ordered as one block per
+// tile block would be; gridDim.x is a multiple of the XCD count).  The limbs
+// stay in registers as 32-bit words and each prime converts them; once the
+// second prime has, the next tile block's limbs are loaded into the same
+// registers, so that their latency (and the completion of the first
+// prime's stores before them) overlaps the second prime's transform and
+// stores.  It needs 126-135 VGPRs (three waves per SIMD); held to 128 (four
+// waves) it spills 4-8 and, measured at 10^9 digits, loses 6 ms per step to
+// the three-wave build (the non-persistent kernel: 185 -> 176 ms).
+template <int M, bool PAIR>
+__global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_limbs_kernel(PassArgs a, uint32_t total) {
+    using CP = ColPair<M, PAIR>;
+    __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
+    double *const lds = lds_tiles + CP::half() * kLdsSlots;
+    const uint32_t tiles = tiles_per_product(a.plan) >> CP::kLog2;
+    uint32_t w = blockIdx.x;
+    if (w >= total) return;  // uniform over the block
+    BlockPos bp = block_pos_of(xcd_tile(w, total, a.xcd_log2), a.op_start, a.n_ops, tiles);
+    uint32_t x[16];
+    load_limbs<M>(x, a, bp, ColThread<M>(a.plan, 0, (bp.t << CP::kLog2) + CP::half(), CP::vtid()));
+    for (;;) {
+        const ColThread<M> ct(a.plan, 0, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
+        const NttOp o = a.ops[bp.seg];
+        double *T0 = a.T + o.base + (size_t)bp.row * plane_row_elems(a.plan);
+        double *T1 = T0 + o.pstride;
+        const ColTw tw0 = col_tw_load<M>(a.tables, ct.cg, ct.B0, false);
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = (double)x[u];
+        col_fwd_prime<M, 0, true>(v, lds, ct, a.tables, tw0);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) T0[ct.plast(k)] = v[k];
+        // prime 1's column twiddles: loaded here (not held through prime 0),
+        // used after its transform
+        const ColTw tw1 = col_tw_load<M>(a.tables + kPrimeTable, ct.cg, ct.B0, false);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = (double)x[u];
+        const uint32_t wn = w + gridDim.x;
+        const bool more = wn < total;  // uniform over the block
+        if (more) {
+            bp = block_pos_of(xcd_tile(wn, total, a.xcd_log2), a.op_start, a.n_ops, tiles);
+            load_limbs<M>(x, a, bp, ColThread<M>(a.plan, 0, (bp.t << CP::kLog2) + CP::half(), CP::vtid()));
+        }
+        // the barrier at its end lets the next tile block reuse `lds`
+        col_fwd_prime<M, 1, true>(v, lds, ct, a.tables, tw1);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) T1[ct.plast(k)] = v[k];
+        if (!more) break;
+        w = wn;
+    }
 }
 
 // Step-0 elements of a row tile (DFT size 2^M): v[j] is tile index
@@ -1005,6 +1060,37 @@ Plan active_plan(int k) {
 std::atomic<uint32_t> g_pair_mask{kPairMaskDefault};
 std::atomic<int> g_xcd_log2{kXcdChunkLog2};
 
+// Blocks of the persistent first-level kernel: as many as fit on the device
+// at once (a multiple of the XCD count), or one per tile block when there
+// are fewer.
+uint32_t device_cus() {
+    static std::atomic<int> cus{0};
+    int n = cus.load(std::memory_order_relaxed);
+    if (n == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+            n = 1;
+        cus.store(n, std::memory_order_relaxed);
+    }
+    return (uint32_t)n;
+}
+template <int M, bool PAIR>
+__global__ void ntt_col_fwd_limbs_kernel(PassArgs a, uint32_t total);
+template <int M, bool PAIR>
+uint32_t persistent_grid(uint32_t total) {
+    static std::atomic<int> per_cu{0};
+    int b = per_cu.load(std::memory_order_relaxed);
+    if (b == 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ntt_col_fwd_limbs_kernel<M, PAIR>,
+                                                         ColPair<M, PAIR>::kThreads, 0) != hipSuccess || b < 1)
+            b = 1;
+        per_cu.store(b, std::memory_order_relaxed);
+    }
+    const uint64_t fit = ((uint64_t)b * device_cus()) / kXcds * kXcds;
+    return (uint32_t)std::min<uint64_t>(total, fit < kXcds ? kXcds : fit);
+}
+
 // Calls f(std::integral_constant<bool, PAIR>) with PAIR = pair where a level
 // of 2^M points can pair its tiles, false otherwise.
 template <int M, typename F>
@@ -1349,9 +1435,10 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
                     constexpr int M = decltype(mc)::value;
                     with_pair<M>((pair_mask >> kPairFwdLimbs) & 1u, [&](auto pc) {
                         constexpr int pl = ColPair<M, decltype(pc)::value>::kLog2;
-                        hipLaunchKernelGGL((ntt_col_fwd_kernel<true, M, decltype(pc)::value>),
-                                           dim3((uint32_t)(fwd_blocks >> pl)), dim3(256 << pl), 0, stream,
-                                           col_args(kPairFwdLimbs, M), 0);
+                        const uint32_t total = (uint32_t)(fwd_blocks >> pl);
+                        hipLaunchKernelGGL((ntt_col_fwd_limbs_kernel<M, decltype(pc)::value>),
+                                           dim3(persistent_grid<M, decltype(pc)::value>(total)), dim3(256 << pl), 0,
+                                           stream, col_args(kPairFwdLimbs, M), total);
                     });
                 });
             }));
@@ -1361,7 +1448,7 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
                         constexpr int M = decltype(mc)::value;
                         with_pair<M>((pair_mask >> kPairFwd) & 1u, [&](auto pc) {
                             constexpr int pl = ColPair<M, decltype(pc)::value>::kLog2;
-                            hipLaunchKernelGGL((ntt_col_fwd_kernel<false, M, decltype(pc)::value>),
+                            hipLaunchKernelGGL((ntt_col_fwd_kernel<M, decltype(pc)::value>),
                                                dim3((uint32_t)(fwd_blocks >> pl)), dim3(256 << pl), 0, stream,
                                                col_args(kPairFwd, M), 1);
                         });

@@ -11,8 +11,9 @@
         DIR: a rocprofv3 --kernel-trace --stats --output-format csv run;
         writes calls, total and average time per kernel.
 
-Kernel names are shortened to the kernel template's name (the first forward
-NTT level, which reads limbs, is 'ntt_col_fwd_kernel<limbs>')."""
+Kernel names are shortened to the kernel template's name; the first
+forward NTT level, which reads limbs (ntt_col_fwd_limbs_kernel), keeps the
+label of the live kernel statistics, 'ntt_col_fwd_kernel<limbs>'."""
 import collections
 import csv
 import glob
@@ -29,8 +30,8 @@ WRITE_SCALE = 1.0
 def short(name: str) -> str:
     m = re.search(r"(\w+_kernel)\b", name)
     base = m.group(1) if m else name.split("(")[0][-48:]
-    if base == "ntt_col_fwd_kernel" and "<true" in name:
-        base += "<limbs>"
+    if base == "ntt_col_fwd_limbs_kernel" or (base == "ntt_col_fwd_kernel" and "<true" in name):
+        base = "ntt_col_fwd_kernel<limbs>"
     return base
 
 
