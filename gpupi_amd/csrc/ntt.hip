@@ -189,7 +189,7 @@ __device__ __forceinline__ void fwd_dft(double *v, double *lds, const double *ta
 // Inverse transform of the tile.  On entry v holds this thread's last-step
 // elements (as left by fwd_dft, so the first LDS store touches only slots
 // this thread read); on exit its step-0 elements.
-template <int M, bool COL>
+template <int M, bool COL, int FINAL = kDitLast>
 __device__ __forceinline__ void inv_dft(double *v, double *lds, const double *tab, int P, Mod m,
                                         uint32_t tid = threadIdx.x) {
     constexpr int kLast = step_count(M) - 1;
@@ -201,7 +201,7 @@ __device__ __forceinline__ void inv_dft(double *v, double *lds, const double *ta
         lds_steps<M, true, COL, kLast - 1, 1>(lds, tab, P, m, tid);
         const StepMap<M, 0, COL> s0(tid);
         lds_load(v, lds, s0);
-        step_regs<M, 0, true, COL, kDitLast>(v, s0, tab, P, m);
+        step_regs<M, 0, true, COL, FINAL>(v, s0, tab, P, m);
     }
 }
 
@@ -661,13 +661,13 @@ __global__ void __launch_bounds__(256) ntt_row_pair_kernel(PassArgs a, uint32_t 
 // loaded from ct.plast(k)): inverse twiddle, inverse DFT; leaves the
 // thread's elements tid + 256 u in v.  The caller must make sure no thread
 // still reads `lds`.
-template <int M, int P>
+template <int M, int P, int FINAL = kDitLast>
 __device__ __forceinline__ void col_inv_prime(double *v, double *lds, const ColThread<M> &ct, const double *tables,
                                               const ColTw &tw) {
     const Mod m = mod_of(P);
     const double *tab = tables + P * kPrimeTable;
     column_twiddles<M>(v, tw, m);
-    inv_dft<M, true>(v, lds, tab, P, m, ct.tid);
+    inv_dft<M, true, FINAL>(v, lds, tab, P, m, ct.tid);
 }
 
 // Loads both primes' values of a column tile and their inverse column
@@ -746,16 +746,20 @@ __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_inv_limb
     double v0[16], v1[16];
     ColTw tw0, tw1;
     col_inv_load<M>(v0, v1, tw0, tw1, a, ct, T0, T0 + oa.pstride);
-    col_inv_prime<M, 0>(v0, lds, ct, a.tables, tw0);
+    // 1/K rides on the column twiddles (the inverse transform is linear):
+    // one product per prime instead of one per value; the last inverse step
+    // re-centres its outputs (|v| < p/2 + p/1024) for the recombination
+    tw0.lb = mulp(tw0.lb, c_ntt.kinv[0][log2K], mod_of(0));
+    tw1.lb = mulp(tw1.lb, c_ntt.kinv[1][log2K], mod_of(1));
+    col_inv_prime<M, 0, kDitToGarner>(v0, lds, ct, a.tables, tw0);
     __syncthreads();
-    col_inv_prime<M, 1>(v1, lds, ct, a.tables, tw1);
+    col_inv_prime<M, 1, kDitToGarner>(v1, lds, ct, a.tables, tw1);
     // prime 1's elements wait in the thread's own LDS slots while the limbs
     // are formed (registers for the carry resolution)
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < 16; ++u) lds[ct.lds0 + 272u * u] = v1[u];
 
-    const double kinv0 = c_ntt.kinv[0][log2K], kinv1 = c_ntt.kinv[1][log2K];
     const uint32_t lane = threadIdx.x & (kWave - 1u);
     const uint32_t jr = lane & (kRun - 1u);  // position inside the run
     uint64_t last_mask = 0;                   // last lanes of runs pass no carry on
@@ -767,8 +771,7 @@ __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_inv_limb
     const uint64_t n_u0 = ct.off0;  // coefficient index of element u = 0
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-        const unsigned __int128 c =
-            recombine(mulp(v0[u], kinv0, mod_of(0)), mulp(lds[ct.lds0 + 272u * u], kinv1, mod_of(1)));
+        const unsigned __int128 c = recombine(v0[u], lds[ct.lds0 + 272u * u]);
         const uint32_t w0 = (uint32_t)c, w1 = (uint32_t)(c >> 32), w2 = (uint32_t)(c >> 64);
         const uint32_t w1a = row_shr<1>(w1), w2a = row_shr<1>(w2), w2b = row_shr<2>(w2);
         const uint64_t s = (uint64_t)w0 + (jr >= 1 ? w1a : 0u) + (jr >= 2 ? w2b : 0u);

@@ -163,6 +163,7 @@ enum BflyKind : int {
     kDitMid,         // inverse step between two others
     kDitLast,        // last inverse step after another: all -> twiddle or 1/K
     kDitOnly,        // single inverse step (M <= 4)
+    kDitToGarner,    // last inverse step after another, outputs re-centred: -> prime recombination
     kBflyKinds
 };
 constexpr double kBoundPointA = 0.51;   // A's transform
@@ -179,6 +180,8 @@ constexpr RedPlan bfly_plan_t(int kind, int R, double T) {
             return dit_plan_t(R, kBoundInvMid, mul_bound(kBoundInvMid, kTwBound), kBoundInvMid, T);
         case kDitLast:
             return dit_plan_t(R, kBoundInvMid, mul_bound(kBoundInvMid, kTwBound), kBoundToTwiddle, T);
+        case kDitToGarner:
+            return dit_plan_t(R, kBoundInvMid, mul_bound(kBoundInvMid, kTwBound), kRedBound, T);
         default: return dit_plan_t(R, kBoundIn, kBoundIn, kBoundToTwiddle, T);
     }
 }
@@ -207,6 +210,9 @@ static_assert(mul_bound(7.9, kTwBound) <= kBoundIn, "step twiddle output feeds t
 static_assert(kBoundPointA * kBoundPointB <= kMulLimit, "pointwise product");
 static_assert(mul_bound(kBoundPointA, kBoundPointB) <= kBoundIn, "pointwise output feeds the inverse");
 static_assert(kBoundInvMid * kTwBound <= kMulLimit, "inverse step twiddle");
+static_assert(mul_bound(mul_bound(kTwBound, kTwBound), kTwBound) <= kColTwBound,
+              "a column twiddle base carrying 1/K (a table entry times 1/K, times the other entry)");
+static_assert(kRedBound < 1.0, "re-centred outputs leave |v| < p for the prime recombination");
 
 template <int KIND, int R>
 struct PlanOf {

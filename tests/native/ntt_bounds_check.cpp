@@ -136,6 +136,7 @@ static void check_radix(const Ctx &c) {
     check_kind<kDitMid, R>(c, true, kBoundInvMid, mul_bound(kBoundInvMid, kTwBound));
     check_kind<kDitLast, R>(c, true, kBoundInvMid, mul_bound(kBoundInvMid, kTwBound));
     check_kind<kDitOnly, R>(c, true, kBoundIn, kBoundIn);
+    check_kind<kDitToGarner, R>(c, true, kBoundInvMid, mul_bound(kBoundInvMid, kTwBound));
 }
 
 // a * b at the stated bounds: exact mod p and within mul_bound
@@ -167,6 +168,8 @@ int main() {
         check_product(c, kBoundToTwiddle, kTwBound, "1/K");
         check_product(c, 7.9, kTwBound, "step twiddle");
         check_product(c, kColTwBound, kColTwBound, "twiddle recurrence");
+        check_product(c, kTwBound, kTwBound, "table entry times 1/K");
+        check_product(c, mul_bound(kTwBound, kTwBound), kTwBound, "column twiddle base with 1/K");
     }
     printf("plans (re-centrings per butterfly, R=16):");
     for (int k = 0; k < kBflyKinds; ++k) printf(" %d", bfly_plan(k, 16).count);
