@@ -796,9 +796,12 @@ __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_inv_limb
 }
 
 // K = 2^M <= 4096, one prime: transforms of A (kept in registers) and B,
-// pointwise, inverse; res[j] is the result (times 1/K) at tile element
-// row_step0_base<M>() + j 2^(M-4), the element the last inverse step read
-// from `lds` (so it may be written back there without a barrier).
+// pointwise, inverse without 1/K; res[j] is K times the result, re-centred
+// (|res| < p/2 + p/1024), at tile element row_step0_base<M>() + j 2^(M-4),
+// the element the last inverse step read from `lds` (so it may be written
+// back there without a barrier).  The coefficients are below 2^76, so K
+// times them stays below 2^88 < p0 p1: the recombination gives K c exactly
+// and a shift by M bits divides it out.
 template <int M>
 __device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, uint32_t prod0, int P, double *res) {
     constexpr uint32_t kJ = 1u << (M - 4);
@@ -828,10 +831,9 @@ __device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, uint
     fwd_dft<M, false, kDifToPointB>(v, lds, tab, P, m);
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = mulp(v[j], ra[j], m);
-    inv_dft<M, false>(v, lds, tab, P, m);
-    const double kinv = c_ntt.kinv[P][M];
+    inv_dft<M, false, kDitToGarner>(v, lds, tab, P, m);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) res[j] = mulp(v[j], kinv, m);
+    for (int j = 0; j < 16; ++j) res[j] = step_count(M) == 1 ? redp(v[j], m) : v[j];
 }
 
 // K = 2^M <= 4096: whole products inside one tile (4096 / K per block).
@@ -867,7 +869,9 @@ __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const uint32_t e = lds_index(is0 + j * kJ);
-        const unsigned __int128 c = recombine(res0[j], res1[j]);
+        static_assert((unsigned __int128)kPrime0 * kPrime1 > ((unsigned __int128)1 << (75 + 12)),
+                      "K times a one-tile coefficient stays below p0 p1");
+        const unsigned __int128 c = recombine(res0[j], res1[j]) >> M;
         lo_s[e] = (uint64_t)c;
         hi_s[e] = (uint32_t)(c >> 64);
     }
