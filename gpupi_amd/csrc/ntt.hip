@@ -383,6 +383,13 @@ struct ColPair {
     static constexpr int kTiles = 1 << kLog2;
     static constexpr int kThreads = 256 * kTiles;
     static constexpr int LC = kTileLog2 - M;
+    // the tiles of a block start 16 slots (32 four-byte banks) off a
+    // multiple of the bank count, so that a wave's two tile halves, which
+    // address the same slots of their tiles, fall on different banks (the
+    // paired second inverse level: 25 % -> 0 % of LDS cycles in bank
+    // conflicts, its time unchanged)
+    static constexpr uint32_t kTileStride = kLdsSlots + 16;
+    static constexpr uint32_t kLdsWords = kTiles * kTileStride - 16;
     __device__ __forceinline__ static uint32_t half() { return kLog2 ? col_pair_half(threadIdx.x, M) : 0u; }
     __device__ __forceinline__ static uint32_t vtid() { return kLog2 ? col_pair_vtid(threadIdx.x, M) : threadIdx.x; }
 };
@@ -457,8 +464,8 @@ __device__ __forceinline__ void col_fwd_prime(double *v, double *lds, const ColT
 template <int M, bool PAIR>
 __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_kernel(PassArgs a, int level) {
     using CP = ColPair<M, PAIR>;
-    __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
-    double *const lds = lds_tiles + CP::half() * kLdsSlots;
+    __shared__ double lds_tiles[CP::kLdsWords];
+    double *const lds = lds_tiles + CP::half() * CP::kTileStride;
     const BlockPos bp = block_pos(a.op_start, a.n_ops, tiles_per_product(a.plan) >> CP::kLog2, a.xcd_log2);
     const NttOp o = a.ops[bp.seg];
     const ColThread<M> ct(a.plan, level, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
@@ -511,8 +518,8 @@ __device__ __forceinline__ void load_limbs(uint32_t *x, const PassArgs &a, const
 template <int M, bool PAIR>
 __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_limbs_kernel(PassArgs a, uint32_t total) {
     using CP = ColPair<M, PAIR>;
-    __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
-    double *const lds = lds_tiles + CP::half() * kLdsSlots;
+    __shared__ double lds_tiles[CP::kLdsWords];
+    double *const lds = lds_tiles + CP::half() * CP::kTileStride;
     const uint32_t tiles = tiles_per_product(a.plan) >> CP::kLog2;
     uint32_t w = blockIdx.x;
     if (w >= total) return;  // uniform over the block
@@ -688,8 +695,8 @@ __device__ __forceinline__ void col_inv_load(double *v0, double *v1, ColTw &tw0,
 template <int M, bool PAIR>
 __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_inv_kernel(PassArgs a, int level) {
     using CP = ColPair<M, PAIR>;
-    __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
-    double *const lds = lds_tiles + CP::half() * kLdsSlots;
+    __shared__ double lds_tiles[CP::kLdsWords];
+    double *const lds = lds_tiles + CP::half() * CP::kTileStride;
     const BlockPos bp = block_pos(a.prod_start, a.n_prods, tiles_per_product(a.plan) >> CP::kLog2, a.xcd_log2);
     const NttOp oa = a.ops[a.prods[bp.seg].a];
     const ColThread<M> ct(a.plan, level, (bp.t << CP::kLog2) + CP::half(), CP::vtid());
@@ -727,8 +734,8 @@ __device__ __forceinline__ uint32_t row_shr(uint32_t x) {
 template <int M, bool PAIR>
 __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_inv_limbs_kernel(PassArgs a) {
     using CP = ColPair<M, PAIR>;
-    __shared__ double lds_tiles[CP::kTiles * kLdsSlots];
-    double *const lds = lds_tiles + CP::half() * kLdsSlots;
+    __shared__ double lds_tiles[CP::kLdsWords];
+    double *const lds = lds_tiles + CP::half() * CP::kTileStride;
     constexpr int LC = kTileLog2 - M;
     constexpr int RL = run_log2_for(M);
     constexpr uint32_t kRun = 1u << RL;
