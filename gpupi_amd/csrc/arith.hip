@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -334,15 +335,15 @@ namespace {
 // share the shorter one's transform.  Transform work counts 3 K for the
 // product as it is, and (2 m + 1) K' for m pieces of K' points (m forward,
 // m inverse, one of S):
-//   two pieces at K' = K/2 when L + 2 S <= K (2.5 K): C[0, s + S) =
-//     L[0, s) S directly, C[s, L + S) += L[s, L) S, s = K/2 - S;
+//   two pieces at K' = K/2 when L + 2 S <= K (2.5 K), s = K/2 - S;
 //   m >= 3 pieces of s = K' - S >= S limbs at a smaller K' when that costs
-//     at most 0.8 of 3 K: the pieces' products (m - 1 rows against S
-//     broadcast, one batched product, and the last piece) go to even and odd
-//     accumulators (pieces of one parity never overlap since s >= S), which
-//     one full-length add sums.
-// Transforms of 2^22 points and up for two pieces, 2^20 for more (smaller
-// products are bound by launches, not by transform size).
+//     at most 0.8 of 3 K.
+// Two pieces are two products sharing S's transform, the first written in
+// place.  More are the rows of one ragged launch sequence (piece_row): even
+// pieces' products land in C and odd ones in an accumulator, each at its
+// offset with zeros to the next piece of its parity.  Either way one add
+// finishes.  Transforms of 2^22 points and up for two pieces, 2^20 for more
+// (smaller products are bound by launches, not by transform size).
 constexpr int kSplitMinLog2 = 22, kSplitManyMinLog2 = 20;
 
 struct SplitPlan {
@@ -374,6 +375,8 @@ SplitPlan split_plan(uint32_t L, uint32_t S) {
     return p;
 }
 
+// Two pieces: C[0, s + S) = L[0, s) S directly, T = L[s, L) S, both one
+// launch sequence sharing S's transform, then C[s..) += T.
 hipError_t mul_split_two(BatchMPContext *ctx, const BatchMPArray &L, const BatchMPArray &S, uint32_t s,
                          const BatchMPArray &C) {
     const uint32_t hi = L.length - s, lt = hi + S.length;
@@ -390,45 +393,22 @@ hipError_t mul_split_two(BatchMPContext *ctx, const BatchMPArray &L, const Batch
     return err;
 }
 
-hipError_t mul_split_many(BatchMPContext *ctx, const BatchMPArray &L, const BatchMPArray &S, uint32_t m,
-                          uint32_t s, const BatchMPArray &C) {
-    const uint32_t w = s + S.length;                  // a piece's product
-    const uint32_t last = L.length - (m - 1) * s;     // the last piece, 1..s limbs
-    const size_t p_words = (size_t)m * w, c_words = C.length;
-    void *pp = nullptr, *op = nullptr;
-    size_t pgot = 0, ogot = 0;
-    GPUPI_RETURN_IF(ctx->cache->acquire(ctx->stream, p_words * sizeof(uint32_t), &pp, &pgot));
-    hipError_t err = ctx->cache->acquire(ctx->stream, c_words * sizeof(uint32_t), &op, &ogot);
-    if (err != hipSuccess) {
-        ctx->cache->give_back(pp, pgot, ctx->stream);
-        return err;
-    }
-    uint32_t *P = static_cast<uint32_t *>(pp), *O = static_cast<uint32_t *>(op);
-    // pieces 0..m-2 against S broadcast (S transformed once), then the last
-    const BatchMPArray Ar{L.data, s, m - 1, s}, Sb{S.data, S.length, m - 1, 0}, Pr{P, w, m - 1, w};
-    const BatchMPArray Al{L.data + (size_t)(m - 1) * s, last, 1, last};
-    const BatchMPArray Pl{P + (size_t)(m - 1) * w, last + S.length, 1, last + S.length};
-    err = batch_mp_mul(ctx, Ar, Sb, Pr);
-    if (err == hipSuccess) err = batch_mp_mul(ctx, Al, S, Pl);
-    // even pieces into C, odd ones into O, each at 32 s i bits
-    if (err == hipSuccess) err = gpupi::launch_fill_u32(C.data, c_words, 0u, ctx->stream);
-    if (err == hipSuccess) err = gpupi::launch_fill_u32(O, c_words, 0u, ctx->stream);
-    const uint32_t full = m - 1;  // pieces copied as whole rows of w limbs
-    for (uint32_t parity = 0; parity < 2 && err == hipSuccess; ++parity) {
-        const uint32_t rows = full > parity ? (full - parity + 1) / 2 : 0;
-        uint32_t *dst = parity ? O : C.data;
-        if (rows > 0)
-            err = hipMemcpy2DAsync(dst + (size_t)parity * s, (size_t)2 * s * sizeof(uint32_t), P + (size_t)parity * w,
-                                   (size_t)2 * w * sizeof(uint32_t), (size_t)w * sizeof(uint32_t), rows,
-                                   hipMemcpyDeviceToDevice, ctx->stream);
-    }
+// L * S in the pieces of `sp` (one launch sequence, S transformed once),
+// then C[s..) += T.
+hipError_t mul_pieces(BatchMPContext *ctx, const BatchMPArray &L, const BatchMPArray &S, const SplitPlan &sp,
+                      const BatchMPArray &C) {
+    const uint32_t t_len = C.length - sp.s;
+    void *tp = nullptr;
+    size_t tgot = 0;
+    GPUPI_RETURN_IF(ctx->cache->acquire(ctx->stream, (size_t)t_len * sizeof(uint32_t), &tp, &tgot));
+    uint32_t *T = static_cast<uint32_t *>(tp);
+    hipError_t err = ensure(ctx, gpupi::ntt_pieces_workspace_bytes(S.length, sp.m, sp.s));
     if (err == hipSuccess)
-        err = hipMemcpyAsync(((m - 1) & 1 ? O : C.data) + (size_t)(m - 1) * s, Pl.data,
-                             (size_t)Pl.length * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream);
-    const BatchMPArray Cf{C.data, C.length, 1, C.length}, Of{O, C.length, 1, C.length};
-    if (err == hipSuccess) err = batch_mp_add(ctx, Cf, Of, Cf);
-    ctx->cache->give_back(op, ogot, ctx->stream);
-    ctx->cache->give_back(pp, pgot, ctx->stream);
+        err = gpupi::launch_mul_ntt_pieces(L.data, L.length, S.data, S.length, C.data, T, sp.m, sp.s,
+                                           ctx->workspace, ctx->workspace_size_bytes, ctx->ntt_tables, ctx->stream);
+    const BatchMPArray Ch{C.data + sp.s, t_len, 1, t_len}, Tf{T, t_len, 1, t_len};
+    if (err == hipSuccess) err = batch_mp_add(ctx, Ch, Tf, Ch);
+    ctx->cache->give_back(tp, tgot, ctx->stream);
     return err;
 }
 
@@ -453,7 +433,7 @@ hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, Bat
         const BatchMPArray &L = a_long ? A : B, &S = a_long ? B : A;
         const SplitPlan sp = split_plan(L.length, S.length);
         if (sp.m == 2) return mul_split_two(ctx, L, S, sp.s, C);
-        if (sp.m >= 3) return mul_split_many(ctx, L, S, sp.m, sp.s, C);
+        if (sp.m >= 3) return mul_pieces(ctx, L, S, sp, C);
     }
     GPUPI_RETURN_IF(ensure(ctx, gpupi::ntt_workspace_bytes(C.batch_size, A.length, B.length)));
     return gpupi::launch_mul_ntt(A.data, B.data, C.data, ctx->workspace, ctx->workspace_size_bytes,
@@ -521,6 +501,89 @@ hipError_t batch_mp_mul_shared(BatchMPContext *ctx, BatchMPArray A1, BatchMPArra
     return batch_mp_mul(ctx, A2, B2, C2);
 }
 
+namespace {
+// The NTT launches of a ragged call: rows of whole products and of the
+// pieces of split ones (plans[q].m >= 2; the odd pieces' accumulators at
+// T + t_offs[i], i counting the split products in order), grouped by
+// transform size into launches of at most ntt_ragged_max_rows rows, each
+// with the set of distinct B operands its rows use.
+hipError_t ragged_launches(BatchMPContext *ctx, const GpupiMulItem *items,
+                           const std::vector<std::pair<int, uint32_t>> &big, const std::vector<SplitPlan> &plans,
+                           uint32_t *T, const std::vector<size_t> &t_offs) {
+    struct Row {
+        int k;
+        gpupi::RaggedRow r;
+    };
+    std::vector<Row> rows;
+    for (size_t q = 0, sp = 0; q < big.size(); ++q) {
+        const GpupiMulItem &it = items[big[q].second];
+        const uint32_t m = plans[q].m;
+        if (m < 2) {
+            rows.push_back(Row{big[q].first, gpupi::RaggedRow{it.A, it.B, it.C, it.L_a, it.L_b, 0, 0}});
+            continue;
+        }
+        const bool a_long = it.L_a >= it.L_b;
+        const uint32_t *Lp = a_long ? it.A : it.B, *Sp = a_long ? it.B : it.A;
+        const uint32_t L = a_long ? it.L_a : it.L_b, S = a_long ? it.L_b : it.L_a;
+        const int kp = gpupi::ntt_transform_log2(plans[q].s, S);
+        uint32_t *Tq = T + t_offs[sp++];
+        for (uint32_t i = 0; i < m; ++i)
+            rows.push_back(Row{kp, gpupi::piece_row(i, Lp, L, Sp, S, it.C, Tq, m, plans[q].s, 0)});
+    }
+    std::stable_sort(rows.begin(), rows.end(), [](const Row &x, const Row &y) { return x.k < y.k; });
+    struct Launch {
+        int k;
+        size_t r0, n, b0, nb;
+    };
+    std::vector<Launch> launches;
+    std::vector<gpupi::RaggedRow> host_rows;
+    std::vector<gpupi::RaggedB> host_bset;
+    host_rows.reserve(rows.size());
+    std::map<std::pair<const uint32_t *, uint32_t>, uint32_t> bidx;
+    size_t ntt_bytes = 0;
+    for (size_t g0 = 0; g0 < rows.size();) {
+        const int k = rows[g0].k;
+        const size_t cap = gpupi::ntt_ragged_max_rows(k);
+        size_t g1 = g0;
+        while (g1 < rows.size() && rows[g1].k == k && g1 - g0 < cap) ++g1;
+        Launch l{k, host_rows.size(), g1 - g0, host_bset.size(), 0};
+        bidx.clear();
+        for (size_t i = g0; i < g1; ++i) {
+            gpupi::RaggedRow r = rows[i].r;
+            auto f = bidx.find(std::make_pair(r.B, r.L_b));
+            if (f == bidx.end()) {
+                f = bidx.emplace(std::make_pair(r.B, r.L_b), (uint32_t)l.nb++).first;
+                host_bset.push_back(gpupi::RaggedB{r.B, r.L_b});
+            }
+            r.b = f->second;
+            host_rows.push_back(r);
+        }
+        ntt_bytes = std::max(ntt_bytes, gpupi::ntt_ragged_workspace_bytes((uint32_t)l.n, (uint32_t)l.nb, k));
+        launches.push_back(l);
+        g0 = g1;
+    }
+    // the descriptors at the head of the workspace, then the largest
+    // launch's NTT scratch
+    const size_t row_bytes = (host_rows.size() * sizeof(gpupi::RaggedRow) + 255) & ~(size_t)255;
+    const size_t b_bytes = (host_bset.size() * sizeof(gpupi::RaggedB) + 255) & ~(size_t)255;
+    GPUPI_RETURN_IF(ensure(ctx, row_bytes + b_bytes + ntt_bytes));
+    char *ws = reinterpret_cast<char *>(ctx->workspace);
+    auto *d_rows = reinterpret_cast<gpupi::RaggedRow *>(ws);
+    auto *d_bset = reinterpret_cast<gpupi::RaggedB *>(ws + row_bytes);
+    char *ntt_ws = ws + row_bytes + b_bytes;
+    GPUPI_RETURN_IF(hipMemcpyAsync(d_rows, host_rows.data(), host_rows.size() * sizeof(gpupi::RaggedRow),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    GPUPI_RETURN_IF(hipMemcpyAsync(d_bset, host_bset.data(), host_bset.size() * sizeof(gpupi::RaggedB),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
+    for (const Launch &l : launches)
+        GPUPI_RETURN_IF(gpupi::launch_mul_ntt_ragged(d_rows + l.r0, host_rows.data() + l.r0, (uint32_t)l.n,
+                                                     d_bset + l.b0, host_bset.data() + l.b0, (uint32_t)l.nb, l.k,
+                                                     ntt_ws, ntt_bytes, ctx->ntt_tables, ctx->stream));
+    return hipSuccess;
+}
+}  // namespace
+
 hipError_t gpupi_mul_ragged(BatchMPContext *ctx, const GpupiMulItem *items, uint32_t count) {
     if (ctx == nullptr || (count > 0 && items == nullptr)) return hipErrorInvalidValue;
     // validate everything before launching anything
@@ -551,51 +614,45 @@ hipError_t gpupi_mul_ragged(BatchMPContext *ctx, const GpupiMulItem *items, uint
             GPUPI_RETURN_IF(gpupi::launch_mul_naive(it.A, it.B, it.C, 1, it.L_a, it.L_b, it.L_a, it.L_b,
                                                     (uint32_t)total, ctx->stream));
     }
-    // products that gain from running in pieces (batch_mp_mul splits them),
-    // before the grouped launches below take the workspace
-    if (ctx->pipeline_form != GPUPI_FORM_PLAIN) {
-        std::vector<std::pair<int, uint32_t>> rest;
-        for (const auto &b : big) {
-            const GpupiMulItem &it = items[b.second];
-            if (split_plan(std::max(it.L_a, it.L_b), std::min(it.L_a, it.L_b)).m >= 2) {
-                const uint32_t total = it.L_a + it.L_b;
-                GPUPI_RETURN_IF(batch_mp_mul(ctx, BatchMPArray{const_cast<uint32_t *>(it.A), it.L_a, 1, it.L_a},
-                                             BatchMPArray{const_cast<uint32_t *>(it.B), it.L_b, 1, it.L_b},
-                                             BatchMPArray{it.C, total, 1, total}));
-            } else {
-                rest.push_back(b);
-            }
-        }
-        big.swap(rest);
-    }
     if (big.empty()) return hipSuccess;
-    std::stable_sort(big.begin(), big.end(),
-                     [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) { return x.first < y.first; });
-    std::vector<GpupiMulItem> sorted(big.size());
-    for (size_t i = 0; i < big.size(); ++i) sorted[i] = items[big[i].second];
-    // workspace: the sorted descriptors, then the largest group's NTT scratch
-    const size_t desc_bytes = (sorted.size() * sizeof(GpupiMulItem) + 255) & ~(size_t)255;
-    size_t ntt_bytes = 0;
-    for (size_t g0 = 0; g0 < big.size();) {
-        size_t g1 = g0;
-        while (g1 < big.size() && big[g1].first == big[g0].first) ++g1;
-        ntt_bytes = std::max(ntt_bytes, gpupi::ntt_ragged_workspace_bytes((uint32_t)(g1 - g0), big[g0].first));
-        g0 = g1;
+    // NTT rows, one launch sequence per transform size (and row chunk):
+    // whole products, and in the batched form the pieces of the products
+    // that gain from running in pieces (split_plan).  A split product's
+    // pieces share the shorter operand's transform; even pieces land in C,
+    // odd ones in an accumulator T (T word 0 = C word s), each followed by
+    // zeros up to the next piece of its parity, and C[s..) += T finishes.
+    struct Split {
+        uint32_t *C;
+        uint32_t s, t_len;
+        size_t t_off;
+    };
+    std::vector<Split> splits;
+    std::vector<SplitPlan> plans(big.size());
+    size_t t_words = 0;
+    if (ctx->pipeline_form != GPUPI_FORM_PLAIN) {
+        for (size_t q = 0; q < big.size(); ++q) {
+            const GpupiMulItem &it = items[big[q].second];
+            plans[q] = split_plan(std::max(it.L_a, it.L_b), std::min(it.L_a, it.L_b));
+            if (plans[q].m < 2) continue;
+            const uint32_t t_len = it.L_a + it.L_b - plans[q].s;
+            splits.push_back(Split{it.C, plans[q].s, t_len, t_words});
+            t_words += ((size_t)t_len + 63) & ~(size_t)63;
+        }
     }
-    GPUPI_RETURN_IF(ensure(ctx, desc_bytes + ntt_bytes));
-    GpupiMulItem *desc = reinterpret_cast<GpupiMulItem *>(ctx->workspace);
-    char *ntt_ws = reinterpret_cast<char *>(ctx->workspace) + desc_bytes;
-    GPUPI_RETURN_IF(hipMemcpyAsync(desc, sorted.data(), sorted.size() * sizeof(GpupiMulItem), hipMemcpyHostToDevice,
-                                   ctx->stream));
-    GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
-    for (size_t g0 = 0; g0 < big.size();) {
-        size_t g1 = g0;
-        while (g1 < big.size() && big[g1].first == big[g0].first) ++g1;
-        GPUPI_RETURN_IF(gpupi::launch_mul_ntt_ragged(desc + g0, sorted.data() + g0, (uint32_t)(g1 - g0), big[g0].first,
-                                                     ntt_ws, ntt_bytes, ctx->ntt_tables, ctx->stream));
-        g0 = g1;
+    void *tp = nullptr;
+    size_t tgot = 0;
+    if (t_words > 0) GPUPI_RETURN_IF(ctx->cache->acquire(ctx->stream, t_words * sizeof(uint32_t), &tp, &tgot));
+    uint32_t *T = static_cast<uint32_t *>(tp);
+    std::vector<size_t> t_offs;
+    for (const Split &sp : splits) t_offs.push_back(sp.t_off);
+    hipError_t err = ragged_launches(ctx, items, big, plans, T, t_offs);
+    for (size_t i = 0; i < splits.size() && err == hipSuccess; ++i) {
+        const Split &sp = splits[i];
+        const BatchMPArray Ch{sp.C + sp.s, sp.t_len, 1, sp.t_len}, Tf{T + sp.t_off, sp.t_len, 1, sp.t_len};
+        err = batch_mp_add(ctx, Ch, Tf, Ch);
     }
-    return hipSuccess;
+    if (tp != nullptr) ctx->cache->give_back(tp, tgot, ctx->stream);
+    return err;
 }
 
 hipError_t batch_mp_mul_small(BatchMPContext *ctx, BatchMPArray A, uint32_t B, BatchMPArray C) {

@@ -101,10 +101,52 @@ hipError_t launch_mul_ntt_bounded(const uint32_t *A, const uint32_t *B, uint32_t
 int ntt_transform_log2(uint32_t L_a, uint32_t L_b);
 // The fused 17-term group stage: N rows each of P (51 limbs), Q (68), R (83).
 hipError_t launch_group_stage(uint32_t N, uint32_t *P, uint32_t *Q, uint32_t *R, hipStream_t stream);
-size_t ntt_ragged_workspace_bytes(uint32_t N, int log2K);
-hipError_t launch_mul_ntt_ragged(const GpupiMulItem *items, const GpupiMulItem *host_items, uint32_t N, int log2K,
-                                 void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
-                                 hipStream_t stream);
+// One product row of a ragged NTT launch: A (L_a limbs) times row b of the
+// launch's set of B operands (B, L_b limbs), written to C as L_a + L_b limbs
+// followed by `tail` zero limbs.  Rows sharing a B operand (the pieces of
+// one long product) have it transformed once.
+struct RaggedRow {
+    const uint32_t *A;
+    const uint32_t *B;
+    uint32_t *C;
+    uint32_t L_a, L_b, b, tail;
+};
+struct RaggedB {
+    const uint32_t *B;
+    uint64_t L;
+};
+// Row i of a product L * S (LL and LS limbs) run in m pieces of s limbs of
+// L (the last one LL - (m - 1) s limbs), all against S, row b of the
+// launch's B set: piece i goes to C (even i) or to an accumulator T (odd i,
+// T word 0 = C word s) at (i - i % 2) s, and its zero tail reaches the next
+// piece of its parity (2 s on; pieces of one parity do not overlap when
+// s >= LS, or when m = 2), or the end of C or T for the last two pieces.
+// C[s..) += T then gives L * S.
+__host__ __device__ inline RaggedRow piece_row(uint32_t i, const uint32_t *L, uint32_t LL, const uint32_t *S,
+                                               uint32_t LS, uint32_t *C, uint32_t *T, uint32_t m, uint32_t s,
+                                               uint32_t b) {
+    const uint32_t last = LL - (m - 1) * s, par = i & 1u;
+    const uint32_t len = i + 1 < m ? s : last;
+    const uint32_t tail = i + 1 == m ? 0u : i + 2 == m ? last : s - LS;
+    return RaggedRow{L + (size_t)i * s, S, (par ? T : C) + (size_t)(i - par) * s, len, LS, b, tail};
+}
+// Rows of one launch at most (N rows of 2^log2K points in one launch chunk).
+uint32_t ntt_ragged_max_rows(int log2K);
+// L * S in m >= 2 pieces of s limbs (piece_row; s >= LS when m > 2, every
+// piece's product above one tile): the pieces go to C and T, ready for
+// C[s..) += T.  The rows are generated on the device; the workspace holds
+// them and the NTT scratch.
+size_t ntt_pieces_workspace_bytes(uint32_t LS, uint32_t m, uint32_t s);
+hipError_t launch_mul_ntt_pieces(const uint32_t *L, uint32_t LL, const uint32_t *S, uint32_t LS, uint32_t *C,
+                                 uint32_t *T, uint32_t m, uint32_t s, void *workspace, size_t workspace_bytes,
+                                 const NTTPrecomputedTables &tables, hipStream_t stream);
+size_t ntt_ragged_workspace_bytes(uint32_t N, uint32_t nb, int log2K);
+// N rows over nb B operands, both as device arrays with host copies: every
+// row's L_a + L_b <= 2^log2K and b < nb, N <= ntt_ragged_max_rows(log2K),
+// and tails only above one tile (2^log2K > 4096).
+hipError_t launch_mul_ntt_ragged(const RaggedRow *rows, const RaggedRow *host_rows, uint32_t N, const RaggedB *bset,
+                                 const RaggedB *host_bset, uint32_t nb, int log2K, void *workspace,
+                                 size_t workspace_bytes, const NTTPrecomputedTables &tables, hipStream_t stream);
 bool ntt_supports(uint32_t L_a, uint32_t L_b);
 // Two NTT products sharing operand B (C1 = A1 B over N1 rows, C2 = A2 B over
 // N2 rows; B has max(N1, N2) rows, or one row at stride 0): B is

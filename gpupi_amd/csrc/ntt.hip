@@ -331,9 +331,10 @@ struct PassArgs {
     uint32_t n_ops, n_prods;
     Plan plan;
     const double *tables;
-    // ragged launches: per-product operands and lengths (device array), or
-    // null when the operands above describe every product
-    const GpupiMulItem *items;
+    // ragged launches: per-product rows and the set of B operands they use
+    // (device arrays), or null when the operands above describe every product
+    const RaggedRow *rows;
+    const RaggedB *bset;
     // log2 of the tiles each XCD takes in turn (xcd_tile)
     int xcd_log2;
 };
@@ -493,10 +494,16 @@ __device__ __forceinline__ void load_limbs(uint32_t *x, const PassArgs &a, const
     const NttOp o = a.ops[bp.seg];
     const uint32_t *X = o.X + (size_t)bp.row * o.s;
     uint32_t LX = o.L;
-    if (a.items) {
-        const GpupiMulItem it = a.items[bp.row];
-        X = bp.seg ? it.B : it.A;
-        LX = bp.seg ? it.L_b : it.L_a;
+    if (a.rows) {
+        if (bp.seg) {
+            const RaggedB rb = a.bset[bp.row];
+            X = rb.B;
+            LX = (uint32_t)rb.L;
+        } else {
+            const RaggedRow rr = a.rows[bp.row];
+            X = rr.A;
+            LX = rr.L_a;
+        }
     }
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
@@ -578,7 +585,8 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     const uint64_t Kp = plane_row_elems(a.plan);
     const size_t off = plane_index(a.plan, (size_t)bp.t * kTile) + row_step0_base<M>();
     double *TA0 = a.T + oa.base + (size_t)bp.row * Kp + off;
-    const double *TB0 = a.T + ob.base + (size_t)(pr.b_bcast ? 0u : bp.row) * Kp + off;
+    const uint32_t brow = a.rows ? a.rows[bp.row].b : pr.b_bcast ? 0u : bp.row;
+    const double *TB0 = a.T + ob.base + (size_t)brow * Kp + off;
 #pragma clang loop unroll(disable)
     for (int P = 0; P < 2; ++P) {
         const Mod m = mod_of(P);
@@ -823,8 +831,8 @@ __device__ __forceinline__ void local_prime(const PassArgs &a, double *lds, uint
         ra[j] = 0.0;
         v[j] = 0.0;
         if (prod < a.N) {
-            if (a.items) {
-                const GpupiMulItem it = a.items[prod];
+            if (a.rows) {
+                const RaggedRow it = a.rows[prod];
                 if (r < it.L_a) ra[j] = (double)it.A[r];
                 if (r < it.L_b) v[j] = (double)it.B[r];
             } else {
@@ -937,8 +945,8 @@ __global__ void __launch_bounds__(256) ntt_local_kernel(PassArgs a) {
     if (prod < a.N) {
         uint32_t *out = a.C + (size_t)prod * a.sC;
         uint32_t Lc = a.Lc;
-        if (a.items) {
-            const GpupiMulItem it = a.items[prod];
+        if (a.rows) {
+            const RaggedRow it = a.rows[prod];
             out = it.C;
             Lc = it.L_a + it.L_b;
         }
@@ -995,6 +1003,9 @@ size_t chunk_products(uint32_t N, int log2K) {
     if (n > N) n = N;
     return n;
 }
+
+// A ragged launch runs as one chunk (its B set stays resident throughout).
+uint32_t ragged_max_rows(int log2K) { return (uint32_t)std::max<size_t>(1, kChunkElems >> log2K); }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -1199,22 +1210,23 @@ void ntt_tables_release(NTTPrecomputedTables *tables) {
 namespace {
 
 // Copies limbs [0, L_a + L_b) of each staged product row (stride K) to its
-// item's C.  Blocks: (item, 1024-limb slice).
+// row's C, followed by the row's zero tail.  Blocks: (row, 1024-limb slice).
 __global__ void __launch_bounds__(256) ragged_copy_kernel(const uint32_t *staging, uint64_t K,
-                                                          const GpupiMulItem *items, uint32_t slices) {
-    const uint32_t item = blockIdx.x / slices, slice = blockIdx.x % slices;
-    const GpupiMulItem it = items[item];
-    const uint32_t Lc = it.L_a + it.L_b;
-    const uint32_t *src = staging + (size_t)item * K;
-    for (uint32_t i = slice * 1024u + threadIdx.x; i < Lc && i < (slice + 1) * 1024u; i += 256u) it.C[i] = src[i];
+                                                          const RaggedRow *rows, uint32_t slices) {
+    const uint32_t row = blockIdx.x / slices, slice = blockIdx.x % slices;
+    const RaggedRow it = rows[row];
+    const uint32_t Lc = it.L_a + it.L_b, end = Lc + it.tail;
+    const uint32_t *src = staging + (size_t)row * K;
+    for (uint32_t i = slice * 1024u + threadIdx.x; i < end && i < (slice + 1) * 1024u; i += 256u)
+        it.C[i] = i < Lc ? src[i] : 0u;
 }
 
 // One NTT launch sequence over products of one transform size 2^k: up to
 // three forward operands and two products with rows at strides (a product's
 // A operand receives the product and is used by nothing else; a B operand
 // may be shared by both products, or be one row at stride 0 that every row
-// multiplies), or ragged items (operands 0 = A and 1 = B of each item, one
-// product).
+// multiplies), or ragged rows (operand 0: each row's A, operand 1: the
+// launch's B set, one product; one launch chunk).
 struct NttJob {
     struct Op {
         const uint32_t *X = nullptr;
@@ -1229,8 +1241,11 @@ struct NttJob {
     Prod prods[2];
     uint32_t n_ops = 0, n_prods = 0;
     bool ragged = false;
-    const GpupiMulItem *items = nullptr;  // ragged: device items
-    const GpupiMulItem *host_items = nullptr;
+    const RaggedRow *rows = nullptr;  // ragged: device rows and B set, with host copies
+    const RaggedRow *host_rows = nullptr;
+    const RaggedB *bset = nullptr;
+    const RaggedB *host_bset = nullptr;
+    uint32_t nb = 0;
     int k = 0;
     size_t max_chunk = 0;  // rows per launch chunk at most (0: the default)
 };
@@ -1260,13 +1275,17 @@ JobShape shape_of(const NttJob &j) {
         sh.op_bcast[o] = !j.ragged && !used_as_a[o] && j.ops[o].s == 0 && sh.op_rows[o] > 1;
         if (sh.op_bcast[o]) sh.op_rows[o] = 1;
     }
+    if (j.ragged) sh.op_rows[1] = j.nb;
     return sh;
 }
 
 // Whether the operands and products describe a job run_ntt can do.
 bool job_ok(const NttJob &j) {
     if (j.n_ops < 1 || j.n_ops > 3 || j.n_prods < 1 || j.n_prods > 2) return false;
-    if (j.ragged && (j.items == nullptr || j.host_items == nullptr || j.n_ops != 2 || j.n_prods != 1)) return false;
+    if (j.ragged && (j.rows == nullptr || j.host_rows == nullptr || j.bset == nullptr || j.host_bset == nullptr ||
+                     j.nb == 0 || j.nb > j.prods[0].N || j.n_ops != 2 || j.n_prods != 1 ||
+                     j.prods[0].N > ragged_max_rows(j.k)))
+        return false;
     for (uint32_t q = 0; q < j.n_prods; ++q) {
         const NttJob::Prod &pr = j.prods[q];
         if (pr.a >= j.n_ops || pr.b >= j.n_ops) return false;
@@ -1338,14 +1357,14 @@ hipError_t run_local(const NttJob &j, const NTTPrecomputedTables &tables, hipStr
         a.Lc = a.La + a.Lb;
         a.sC = pr.sC;
         a.N = pr.N;
-        a.items = j.items;
+        a.rows = j.rows;
         a.xcd_log2 = g_xcd_log2.load(std::memory_order_relaxed);
         const uint32_t G = kTile >> k;
         const uint64_t blocks = ceil_div64(pr.N, G);
         if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
         double limbs = 0;
         if (j.ragged) {
-            for (uint32_t i = 0; i < pr.N; ++i) limbs += (double)j.host_items[i].L_a + j.host_items[i].L_b;
+            for (uint32_t i = 0; i < pr.N; ++i) limbs += (double)j.host_rows[i].L_a + j.host_rows[i].L_b;
         } else {
             limbs = (double)pr.N * ((double)a.La + a.Lb);
         }
@@ -1423,9 +1442,10 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
         }
         if (fwd_blocks > 0x7fffffffull || prod_blocks > 0x7fffffffull) return hipErrorInvalidValue;
         if (j.ragged) {
-            a.items = j.items + r0;
-            for (uint32_t i = 0; i < rows[0]; ++i)
-                limb_reads += (uint64_t)j.host_items[r0 + i].L_a + j.host_items[r0 + i].L_b;
+            a.rows = j.rows;
+            a.bset = j.bset;
+            for (uint32_t i = 0; i < rows[0]; ++i) limb_reads += j.host_rows[i].L_a;
+            for (uint32_t b = 0; b < j.nb; ++b) limb_reads += j.host_bset[b].L;
         }
         // algorithmic bytes: 16 B per residue pair (one binary64 per prime),
         // 4 B per limb
@@ -1524,9 +1544,13 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
                                               1u << final_run_log2(p), (uint32_t)runs, flags, stream));
         }
         if (j.ragged) {
-            const uint32_t slices = (uint32_t)ceil_div64(K, 1024);
+            uint64_t span = K;  // longest row with its zero tail
+            for (uint32_t i = 0; i < rows[0]; ++i)
+                span = std::max<uint64_t>(span, (uint64_t)j.host_rows[i].L_a + j.host_rows[i].L_b + j.host_rows[i].tail);
+            const uint32_t slices = (uint32_t)ceil_div64(span, 1024);
+            if ((uint64_t)rows[0] * slices > 0x7fffffffull) return hipErrorInvalidValue;
             hipLaunchKernelGGL(ragged_copy_kernel, dim3(rows[0] * slices), dim3(256), 0, stream, staging, (uint64_t)K,
-                               a.items, slices);
+                               a.rows, slices);
             GPUPI_RETURN_IF(hipGetLastError());
         }
     }
@@ -1661,30 +1685,102 @@ hipError_t launch_mul_ntt_shared(const uint32_t *A1, uint32_t L1, uint32_t s1, u
 int ntt_transform_log2(uint32_t L_a, uint32_t L_b) { return transform_log2(L_a, L_b); }
 
 namespace {
-NttJob ragged_job(const GpupiMulItem *items, const GpupiMulItem *host_items, uint32_t N, int log2K) {
+NttJob ragged_job(const RaggedRow *rows, const RaggedRow *host_rows, uint32_t N, const RaggedB *bset,
+                  const RaggedB *host_bset, uint32_t nb, int log2K) {
     NttJob j;
     j.n_ops = 2;
     j.n_prods = 1;
     j.prods[0].N = N;
     j.ragged = true;
-    j.items = items;
-    j.host_items = host_items;
+    j.rows = rows;
+    j.host_rows = host_rows;
+    j.bset = bset;
+    j.host_bset = host_bset;
+    j.nb = nb;
     j.k = log2K;
     return j;
 }
 }  // namespace
 
-size_t ntt_ragged_workspace_bytes(uint32_t N, int log2K) {
-    return job_workspace_bytes(ragged_job(nullptr, nullptr, N, log2K));
+uint32_t ntt_ragged_max_rows(int log2K) { return ragged_max_rows(log2K); }
+
+namespace {
+// The rows of a product in pieces (piece_row) and its one B entry, written
+// on the device (no host upload).
+__global__ void __launch_bounds__(256) piece_rows_kernel(RaggedRow *rows, RaggedB *bset, const uint32_t *L,
+                                                         uint32_t LL, const uint32_t *S, uint32_t LS, uint32_t *C,
+                                                         uint32_t *T, uint32_t m, uint32_t s) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i == 0) bset[0] = RaggedB{S, LS};
+    if (i < m) rows[i] = piece_row(i, L, LL, S, LS, C, T, m, s, 0);
 }
 
-hipError_t launch_mul_ntt_ragged(const GpupiMulItem *items, const GpupiMulItem *host_items, uint32_t N, int log2K,
-                                 void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
-                                 hipStream_t stream) {
-    if (N == 0) return hipSuccess;
-    if (items == nullptr || host_items == nullptr || tables.roots == nullptr || log2K < kMinLog2 || log2K > kMaxLog2)
+struct PiecesLayout {
+    size_t rows_bytes, bset_bytes, ntt_bytes;
+    uint32_t per_launch;
+    int k;
+};
+PiecesLayout pieces_layout(uint32_t LS, uint32_t m, uint32_t s) {
+    PiecesLayout l;
+    l.k = transform_log2(s, LS);
+    l.per_launch = std::min(m, ragged_max_rows(l.k));
+    l.rows_bytes = align256((size_t)m * sizeof(RaggedRow));
+    l.bset_bytes = align256(sizeof(RaggedB));
+    l.ntt_bytes = job_workspace_bytes(ragged_job(nullptr, nullptr, l.per_launch, nullptr, nullptr, 1, l.k));
+    return l;
+}
+}  // namespace
+
+size_t ntt_pieces_workspace_bytes(uint32_t LS, uint32_t m, uint32_t s) {
+    const PiecesLayout l = pieces_layout(LS, m, s);
+    return l.rows_bytes + l.bset_bytes + l.ntt_bytes;
+}
+
+hipError_t launch_mul_ntt_pieces(const uint32_t *L, uint32_t LL, const uint32_t *S, uint32_t LS, uint32_t *C,
+                                 uint32_t *T, uint32_t m, uint32_t s, void *workspace, size_t workspace_bytes,
+                                 const NTTPrecomputedTables &tables, hipStream_t stream) {
+    if (m < 2 || s == 0 || LS == 0 || (m > 2 && s < LS) || (uint64_t)(m - 1) * s >= LL ||
+        (uint64_t)LL - (uint64_t)(m - 1) * s > s || tables.roots == nullptr)
         return hipErrorInvalidValue;
-    return run_ntt(ragged_job(items, host_items, N, log2K), workspace, workspace_bytes, tables, stream);
+    const PiecesLayout l = pieces_layout(LS, m, s);
+    if (l.k <= kTileLog2 || l.k > kMaxLog2 || workspace == nullptr ||
+        workspace_bytes < l.rows_bytes + l.bset_bytes + l.ntt_bytes)
+        return hipErrorInvalidValue;
+    char *ws = static_cast<char *>(workspace);
+    auto *rows = reinterpret_cast<RaggedRow *>(ws);
+    auto *bset = reinterpret_cast<RaggedB *>(ws + l.rows_bytes);
+    hipLaunchKernelGGL(piece_rows_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, rows, bset, L, LL, S, LS, C, T,
+                       m, s);
+    GPUPI_RETURN_IF(hipGetLastError());
+    // the same rows on the host, for the launch's bookkeeping
+    std::vector<RaggedRow> host_rows(m);
+    for (uint32_t i = 0; i < m; ++i) host_rows[i] = piece_row(i, L, LL, S, LS, C, T, m, s, 0);
+    const RaggedB host_b{S, LS};
+    for (uint32_t r0 = 0; r0 < m; r0 += l.per_launch) {
+        const uint32_t n = std::min(l.per_launch, m - r0);
+        GPUPI_RETURN_IF(run_ntt(ragged_job(rows + r0, host_rows.data() + r0, n, bset, &host_b, 1, l.k),
+                                ws + l.rows_bytes + l.bset_bytes, l.ntt_bytes, tables, stream));
+    }
+    return hipSuccess;
+}
+
+size_t ntt_ragged_workspace_bytes(uint32_t N, uint32_t nb, int log2K) {
+    return job_workspace_bytes(ragged_job(nullptr, nullptr, N, nullptr, nullptr, nb, log2K));
+}
+
+hipError_t launch_mul_ntt_ragged(const RaggedRow *rows, const RaggedRow *host_rows, uint32_t N, const RaggedB *bset,
+                                 const RaggedB *host_bset, uint32_t nb, int log2K, void *workspace,
+                                 size_t workspace_bytes, const NTTPrecomputedTables &tables, hipStream_t stream) {
+    if (N == 0) return hipSuccess;
+    if (tables.roots == nullptr || log2K < kMinLog2 || log2K > kMaxLog2) return hipErrorInvalidValue;
+    for (uint32_t i = 0; i < N && host_rows != nullptr; ++i) {
+        const RaggedRow &r = host_rows[i];
+        if (r.b >= nb || (uint64_t)r.L_a + r.L_b > (1ull << log2K) || (r.tail != 0 && log2K <= kTileLog2) ||
+            (host_bset != nullptr && host_bset[r.b].L != r.L_b))
+            return hipErrorInvalidValue;
+    }
+    return run_ntt(ragged_job(rows, host_rows, N, bset, host_bset, nb, log2K), workspace, workspace_bytes, tables,
+                   stream);
 }
 
 }  // namespace gpupi

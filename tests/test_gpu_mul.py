@@ -148,12 +148,14 @@ def test_random_shapes_and_strides(ctx):
 
 
 def test_broadcast_operand(ctx):
-    # convert_decimal multiplies every row by one power (stride 0)
+    # convert_decimal multiplies every row by one power (stride 0); above
+    # one tile the power's last level is transformed once for every row
     rng = np.random.default_rng(3)
-    for la, lb in [(10, 7), (300, 300), (1500, 2500)]:
-        A = rand_rows(rng, 33, la)
-        B = np.repeat(rand_rows(rng, 1, lb), 33, axis=0)
-        assert np.array_equal(device_mul(ctx, A, B, b_broadcast=True), oracle_mul(A, B))
+    for n, la, lb, mode in [(33, 10, 7, "random"), (33, 300, 300, "random"), (33, 1500, 2500, "random"),
+                            (33, 9000, 7000, "random"), (5, 70000, 60000, "ones"), (3, 600000, 400000, "random")]:
+        A = rand_rows(rng, n, la, mode)
+        B = np.repeat(rand_rows(rng, 1, lb, mode), n, axis=0)
+        assert np.array_equal(device_mul(ctx, A, B, b_broadcast=True), oracle_mul(A, B)), (n, la, lb)
 
 
 def every_size_case(k):
@@ -491,6 +493,46 @@ def test_ragged_products(ctx):
             assert np.array_equal(C[:-1], oracle_mul(A, B)[0]), (la, lb)
         else:
             assert not np.any(C[:-1]), (la, lb)
+
+
+def test_ragged_pieces_and_shared_operands(ctx):
+    # one ragged call where rows share B operands: items naming one buffer
+    # (whole products of one transform size, a prefix of it, the buffer on
+    # the A side) and split products (two pieces, many pieces, the longer
+    # operand on either side, two products with one shorter operand, all-ones
+    # carries through every seam); every C ends at its length
+    rng = np.random.default_rng(41)
+    ops = {
+        "b": rand_rows(rng, 1, 30000)[0], "a1": rand_rows(rng, 1, 40000)[0], "a2": rand_rows(rng, 1, 50000)[0],
+        "a3": rand_rows(rng, 1, 35000)[0], "s": rand_rows(rng, 1, 3000)[0], "l1": rand_rows(rng, 1, 4194309)[0],
+        "l2": rand_rows(rng, 1, 1500000)[0], "t1": rand_rows(rng, 1, 3000000)[0],
+        "t2": rand_rows(rng, 1, 500000)[0], "o1": rand_rows(rng, 1, 2500001, "ones")[0],
+        "o2": rand_rows(rng, 1, 70001, "ones")[0],
+    }
+    bufs = {k: g.DeviceBuffer.from_array(v) for k, v in ops.items()}
+    # (A, B, A length, B length): lengths up to the buffer's
+    pairs = [("a1", "b", 40000, 30000), ("a2", "b", 50000, 30000), ("a3", "b", 35000, 30000),
+             ("a1", "b", 40000, 20000), ("b", "a3", 30000, 35000), ("l1", "s", 4194309, 3000),
+             ("s", "l2", 3000, 1500000), ("t1", "t2", 3000000, 500000), ("t2", "t1", 500000, 3000000),
+             ("o1", "o2", 2500001, 70001), ("a2", "b", 50000, 30000)]
+    outs, items = [], []
+    for a, b, la, lb in pairs:
+        c = g.DeviceBuffer.from_array(np.full(la + lb + 1, PAD, dtype=np.uint32))
+        outs.append(c)
+        items.append((bufs[a].ptr, bufs[b].ptr, c.ptr, la, lb))
+    g.kernel_stats_reset(True)
+    try:
+        g.check(ctx.mul_ragged(items), "mul_ragged")
+        g.synchronize()
+        sizes = {int(name.split("@2^")[1]) for name, *_ in g.kernel_stats(by_size=True)}
+    finally:
+        g.kernel_stats_reset(False)
+    assert max(sizes) <= 21, sizes  # every long product ran in pieces
+    for (a, b, la, lb), c in zip(pairs, outs):
+        got = c.download(np.uint32, count=la + lb + 1)
+        assert got[-1] == PAD, (a, b, la, lb)
+        want = oracle_mul(ops[a][:la].reshape(1, -1), ops[b][:lb].reshape(1, -1))[0]
+        assert np.array_equal(got[:-1], want), (a, b, la, lb)
 
 
 def test_ragged_one_tile_products_of_all_ones(ctx):
