@@ -393,23 +393,13 @@ hipError_t mul_split_two(BatchMPContext *ctx, const BatchMPArray &L, const Batch
     return err;
 }
 
-// L * S in the pieces of `sp` (one launch sequence, S transformed once),
-// then C[s..) += T.
+// C = L * S in the pieces of `sp`: one launch sequence, S transformed once,
+// the pieces' products assembled into C.
 hipError_t mul_pieces(BatchMPContext *ctx, const BatchMPArray &L, const BatchMPArray &S, const SplitPlan &sp,
                       const BatchMPArray &C) {
-    const uint32_t t_len = C.length - sp.s;
-    void *tp = nullptr;
-    size_t tgot = 0;
-    GPUPI_RETURN_IF(ctx->cache->acquire(ctx->stream, (size_t)t_len * sizeof(uint32_t), &tp, &tgot));
-    uint32_t *T = static_cast<uint32_t *>(tp);
-    hipError_t err = ensure(ctx, gpupi::ntt_pieces_workspace_bytes(S.length, sp.m, sp.s));
-    if (err == hipSuccess)
-        err = gpupi::launch_mul_ntt_pieces(L.data, L.length, S.data, S.length, C.data, T, sp.m, sp.s,
-                                           ctx->workspace, ctx->workspace_size_bytes, ctx->ntt_tables, ctx->stream);
-    const BatchMPArray Ch{C.data + sp.s, t_len, 1, t_len}, Tf{T, t_len, 1, t_len};
-    if (err == hipSuccess) err = batch_mp_add(ctx, Ch, Tf, Ch);
-    ctx->cache->give_back(tp, tgot, ctx->stream);
-    return err;
+    GPUPI_RETURN_IF(ensure(ctx, gpupi::ntt_pieces_workspace_bytes(L.length, S.length, sp.m, sp.s)));
+    return gpupi::launch_mul_ntt_pieces(L.data, L.length, S.data, S.length, C.data, sp.m, sp.s, ctx->workspace,
+                                        ctx->workspace_size_bytes, ctx->ntt_tables, ctx->stream);
 }
 
 }  // namespace
@@ -502,63 +492,71 @@ hipError_t batch_mp_mul_shared(BatchMPContext *ctx, BatchMPArray A1, BatchMPArra
 }
 
 namespace {
-// The NTT launches of a ragged call: rows of whole products and of the
-// pieces of split ones (plans[q].m >= 2; the odd pieces' accumulators at
-// T + t_offs[i], i counting the split products in order), grouped by
-// transform size into launches of at most ntt_ragged_max_rows rows, each
-// with the set of distinct B operands its rows use.
+// The NTT launches of a ragged call: rows of whole products, and the
+// pieces of split ones (plans[q].m >= 2) with their assemblies, grouped by
+// transform size into launches filled to ntt_ragged_max_rows rows (a
+// product's pieces stay in one launch), each with the set of distinct B
+// operands its rows use.
 hipError_t ragged_launches(BatchMPContext *ctx, const GpupiMulItem *items,
-                           const std::vector<std::pair<int, uint32_t>> &big, const std::vector<SplitPlan> &plans,
-                           uint32_t *T, const std::vector<size_t> &t_offs) {
-    struct Row {
+                           const std::vector<std::pair<int, uint32_t>> &big, const std::vector<SplitPlan> &plans) {
+    // groups of rows: a whole product (one row) or a split one (its pieces)
+    struct Group {
         int k;
-        gpupi::RaggedRow r;
+        size_t q;
     };
-    std::vector<Row> rows;
-    for (size_t q = 0, sp = 0; q < big.size(); ++q) {
+    std::vector<Group> groups;
+    for (size_t q = 0; q < big.size(); ++q) {
         const GpupiMulItem &it = items[big[q].second];
-        const uint32_t m = plans[q].m;
-        if (m < 2) {
-            rows.push_back(Row{big[q].first, gpupi::RaggedRow{it.A, it.B, it.C, it.L_a, it.L_b, 0, 0}});
-            continue;
-        }
-        const bool a_long = it.L_a >= it.L_b;
-        const uint32_t *Lp = a_long ? it.A : it.B, *Sp = a_long ? it.B : it.A;
-        const uint32_t L = a_long ? it.L_a : it.L_b, S = a_long ? it.L_b : it.L_a;
-        const int kp = gpupi::ntt_transform_log2(plans[q].s, S);
-        uint32_t *Tq = T + t_offs[sp++];
-        for (uint32_t i = 0; i < m; ++i)
-            rows.push_back(Row{kp, gpupi::piece_row(i, Lp, L, Sp, S, it.C, Tq, m, plans[q].s, 0)});
+        const int k = plans[q].m < 2 ? big[q].first
+                                     : gpupi::ntt_transform_log2(plans[q].s, std::min(it.L_a, it.L_b));
+        groups.push_back(Group{k, q});
     }
-    std::stable_sort(rows.begin(), rows.end(), [](const Row &x, const Row &y) { return x.k < y.k; });
+    std::stable_sort(groups.begin(), groups.end(), [](const Group &x, const Group &y) { return x.k < y.k; });
     struct Launch {
         int k;
-        size_t r0, n, b0, nb;
+        size_t r0, n, b0, nb, a0, na;
+        uint32_t chunks;
     };
     std::vector<Launch> launches;
     std::vector<gpupi::RaggedRow> host_rows;
     std::vector<gpupi::RaggedB> host_bset;
-    host_rows.reserve(rows.size());
+    std::vector<gpupi::RaggedAssembly> host_asms;
     std::map<std::pair<const uint32_t *, uint32_t>, uint32_t> bidx;
-    size_t ntt_bytes = 0;
-    for (size_t g0 = 0; g0 < rows.size();) {
-        const int k = rows[g0].k;
-        const size_t cap = gpupi::ntt_ragged_max_rows(k);
-        size_t g1 = g0;
-        while (g1 < rows.size() && rows[g1].k == k && g1 - g0 < cap) ++g1;
-        Launch l{k, host_rows.size(), g1 - g0, host_bset.size(), 0};
-        bidx.clear();
-        for (size_t i = g0; i < g1; ++i) {
-            gpupi::RaggedRow r = rows[i].r;
-            auto f = bidx.find(std::make_pair(r.B, r.L_b));
-            if (f == bidx.end()) {
-                f = bidx.emplace(std::make_pair(r.B, r.L_b), (uint32_t)l.nb++).first;
-                host_bset.push_back(gpupi::RaggedB{r.B, r.L_b});
-            }
-            r.b = f->second;
-            host_rows.push_back(r);
+    const auto b_of = [&](Launch &l, const uint32_t *B, uint32_t LB) {
+        auto f = bidx.find(std::make_pair(B, LB));
+        if (f == bidx.end()) {
+            f = bidx.emplace(std::make_pair(B, LB), (uint32_t)l.nb++).first;
+            host_bset.push_back(gpupi::RaggedB{B, LB});
         }
-        ntt_bytes = std::max(ntt_bytes, gpupi::ntt_ragged_workspace_bytes((uint32_t)l.n, (uint32_t)l.nb, k));
+        return f->second;
+    };
+    size_t ntt_bytes = 0;
+    for (size_t g0 = 0; g0 < groups.size();) {
+        const int k = groups[g0].k;
+        const size_t cap = gpupi::ntt_ragged_max_rows(k);
+        Launch l{k, host_rows.size(), 0, host_bset.size(), 0, host_asms.size(), 0, 0};
+        bidx.clear();
+        size_t g1 = g0;
+        for (; g1 < groups.size() && groups[g1].k == k; ++g1) {
+            const size_t q = groups[g1].q;
+            const GpupiMulItem &it = items[big[q].second];
+            const uint32_t m = plans[q].m < 2 ? 1u : plans[q].m;
+            if (l.n > 0 && l.n + m > cap) break;
+            if (m == 1) {
+                host_rows.push_back(gpupi::RaggedRow{it.A, it.B, it.C, it.L_a, it.L_b, b_of(l, it.B, it.L_b), 0});
+            } else {
+                const bool a_long = it.L_a >= it.L_b;
+                const uint32_t *Lp = a_long ? it.A : it.B, *Sp = a_long ? it.B : it.A;
+                const uint32_t L = a_long ? it.L_a : it.L_b, S = a_long ? it.L_b : it.L_a;
+                const uint32_t b = b_of(l, Sp, S);
+                host_asms.push_back(gpupi::RaggedAssembly{it.C, L + S, plans[q].s, S, m, (uint32_t)l.n, l.chunks});
+                l.chunks += gpupi::assembly_chunks(L + S);
+                for (uint32_t i = 0; i < m; ++i) host_rows.push_back(gpupi::piece_row(i, Lp, L, Sp, S, m, plans[q].s, b));
+            }
+            l.n += m;
+        }
+        l.na = host_asms.size() - l.a0;
+        ntt_bytes = std::max(ntt_bytes, gpupi::ntt_ragged_workspace_bytes((uint32_t)l.n, (uint32_t)l.nb, l.chunks, k));
         launches.push_back(l);
         g0 = g1;
     }
@@ -566,20 +564,26 @@ hipError_t ragged_launches(BatchMPContext *ctx, const GpupiMulItem *items,
     // launch's NTT scratch
     const size_t row_bytes = (host_rows.size() * sizeof(gpupi::RaggedRow) + 255) & ~(size_t)255;
     const size_t b_bytes = (host_bset.size() * sizeof(gpupi::RaggedB) + 255) & ~(size_t)255;
-    GPUPI_RETURN_IF(ensure(ctx, row_bytes + b_bytes + ntt_bytes));
+    const size_t a_bytes = (host_asms.size() * sizeof(gpupi::RaggedAssembly) + 255) & ~(size_t)255;
+    GPUPI_RETURN_IF(ensure(ctx, row_bytes + b_bytes + a_bytes + ntt_bytes));
     char *ws = reinterpret_cast<char *>(ctx->workspace);
     auto *d_rows = reinterpret_cast<gpupi::RaggedRow *>(ws);
     auto *d_bset = reinterpret_cast<gpupi::RaggedB *>(ws + row_bytes);
-    char *ntt_ws = ws + row_bytes + b_bytes;
+    auto *d_asms = reinterpret_cast<gpupi::RaggedAssembly *>(ws + row_bytes + b_bytes);
+    char *ntt_ws = ws + row_bytes + b_bytes + a_bytes;
     GPUPI_RETURN_IF(hipMemcpyAsync(d_rows, host_rows.data(), host_rows.size() * sizeof(gpupi::RaggedRow),
                                    hipMemcpyHostToDevice, ctx->stream));
     GPUPI_RETURN_IF(hipMemcpyAsync(d_bset, host_bset.data(), host_bset.size() * sizeof(gpupi::RaggedB),
                                    hipMemcpyHostToDevice, ctx->stream));
+    if (!host_asms.empty())
+        GPUPI_RETURN_IF(hipMemcpyAsync(d_asms, host_asms.data(), host_asms.size() * sizeof(gpupi::RaggedAssembly),
+                                       hipMemcpyHostToDevice, ctx->stream));
     GPUPI_RETURN_IF(hipStreamSynchronize(ctx->stream));
     for (const Launch &l : launches)
-        GPUPI_RETURN_IF(gpupi::launch_mul_ntt_ragged(d_rows + l.r0, host_rows.data() + l.r0, (uint32_t)l.n,
-                                                     d_bset + l.b0, host_bset.data() + l.b0, (uint32_t)l.nb, l.k,
-                                                     ntt_ws, ntt_bytes, ctx->ntt_tables, ctx->stream));
+        GPUPI_RETURN_IF(gpupi::launch_mul_ntt_ragged(
+            d_rows + l.r0, host_rows.data() + l.r0, (uint32_t)l.n, d_bset + l.b0, host_bset.data() + l.b0,
+            (uint32_t)l.nb, d_asms + l.a0, host_asms.data() + l.a0, (uint32_t)l.na, l.chunks, l.k, ntt_ws,
+            ntt_bytes, ctx->ntt_tables, ctx->stream));
     return hipSuccess;
 }
 }  // namespace
@@ -617,42 +621,16 @@ hipError_t gpupi_mul_ragged(BatchMPContext *ctx, const GpupiMulItem *items, uint
     if (big.empty()) return hipSuccess;
     // NTT rows, one launch sequence per transform size (and row chunk):
     // whole products, and in the batched form the pieces of the products
-    // that gain from running in pieces (split_plan).  A split product's
-    // pieces share the shorter operand's transform; even pieces land in C,
-    // odd ones in an accumulator T (T word 0 = C word s), each followed by
-    // zeros up to the next piece of its parity, and C[s..) += T finishes.
-    struct Split {
-        uint32_t *C;
-        uint32_t s, t_len;
-        size_t t_off;
-    };
-    std::vector<Split> splits;
+    // that gain from running in pieces (split_plan), which share the
+    // shorter operand's transform and are assembled from their staged
+    // products into C.
     std::vector<SplitPlan> plans(big.size());
-    size_t t_words = 0;
-    if (ctx->pipeline_form != GPUPI_FORM_PLAIN) {
+    if (ctx->pipeline_form != GPUPI_FORM_PLAIN)
         for (size_t q = 0; q < big.size(); ++q) {
             const GpupiMulItem &it = items[big[q].second];
             plans[q] = split_plan(std::max(it.L_a, it.L_b), std::min(it.L_a, it.L_b));
-            if (plans[q].m < 2) continue;
-            const uint32_t t_len = it.L_a + it.L_b - plans[q].s;
-            splits.push_back(Split{it.C, plans[q].s, t_len, t_words});
-            t_words += ((size_t)t_len + 63) & ~(size_t)63;
         }
-    }
-    void *tp = nullptr;
-    size_t tgot = 0;
-    if (t_words > 0) GPUPI_RETURN_IF(ctx->cache->acquire(ctx->stream, t_words * sizeof(uint32_t), &tp, &tgot));
-    uint32_t *T = static_cast<uint32_t *>(tp);
-    std::vector<size_t> t_offs;
-    for (const Split &sp : splits) t_offs.push_back(sp.t_off);
-    hipError_t err = ragged_launches(ctx, items, big, plans, T, t_offs);
-    for (size_t i = 0; i < splits.size() && err == hipSuccess; ++i) {
-        const Split &sp = splits[i];
-        const BatchMPArray Ch{sp.C + sp.s, sp.t_len, 1, sp.t_len}, Tf{T + sp.t_off, sp.t_len, 1, sp.t_len};
-        err = batch_mp_add(ctx, Ch, Tf, Ch);
-    }
-    if (tp != nullptr) ctx->cache->give_back(tp, tgot, ctx->stream);
-    return err;
+    return ragged_launches(ctx, items, big, plans);
 }
 
 hipError_t batch_mp_mul_small(BatchMPContext *ctx, BatchMPArray A, uint32_t B, BatchMPArray C) {

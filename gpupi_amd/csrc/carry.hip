@@ -265,6 +265,111 @@ __global__ void __launch_bounds__(256) spill_add_vec_kernel(uint32_t *C, uint32_
     if (flags != nullptr && lane == 0) flags[wave] = (uint8_t)(cin | (all_prop ? 2u : 0u));
 }
 
+// The assembly holding launch chunk g (assemblies in chunk order).
+__device__ __forceinline__ RaggedAssembly assembly_of(const RaggedAssembly *as, uint32_t n, uint32_t g) {
+    uint32_t lo = 0, hi = n;  // the last assembly with chunk0 <= g
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (as[mid].chunk0 <= g) lo = mid;
+        else hi = mid;
+    }
+    return as[lo];
+}
+
+// Ragged assembly (RaggedAssembly): one wave per 1024-limb chunk, the
+// layout of addsub_vec_kernel (flags per chunk for the fixup).  Limb x of C
+// is piece i's staged word x - i s plus, when x - i s < S, piece i - 1's
+// word x - i s + s, i = min(x / s, m - 1).
+__global__ void __launch_bounds__(256) ragged_assemble_kernel(const uint32_t *staging, uint64_t K,
+                                                              const RaggedAssembly *as, uint32_t n,
+                                                              uint32_t chunks, uint8_t *flags) {
+    constexpr int kSeg = 4;  // 4 x 256 limbs per wave
+    const uint32_t g = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    if (g >= chunks) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const RaggedAssembly a = assembly_of(as, n, g);
+    const uint32_t chunk = g - a.chunk0;
+    uint32_t cin = 0;
+    bool all_prop = true;
+#pragma unroll
+    for (int sg = 0; sg < kSeg; ++sg) {
+        const uint32_t j0 = chunk * kAssembleChunk + (uint32_t)sg * (4 * kWave) + 4 * lane;
+        uint32_t d[4], gen[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t x = j0 + t;
+            d[t] = 0;
+            gen[t] = 0;
+            if (x < a.c) {
+                const uint32_t i = min(x / a.s, a.m - 1), off = x - i * a.s;
+                const uint32_t u = staging[(size_t)(a.row0 + i) * K + off];
+                const uint32_t v = (i > 0 && off < a.S) ? staging[(size_t)(a.row0 + i - 1) * K + off + a.s] : 0u;
+                d[t] = u + v;
+                gen[t] = d[t] < u ? 1u : 0u;
+            }
+        }
+        // the lane's four limbs for a carry-in of zero
+        uint32_t r[4], c = 0;
+        bool prop = true;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            r[t] = d[t] + c;
+            c = gen[t] | (c & (d[t] == 0xffffffffu ? 1u : 0u));
+            prop = prop && r[t] == 0xffffffffu && j0 + t < a.c;
+        }
+        const bool valid = j0 < a.c;
+        const uint64_t G = ballot64(valid && c != 0), P = ballot64(valid && prop);
+        uint32_t cout;
+        const uint64_t cm = resolve_carries(G, P, cin, &cout);
+        uint32_t ci = (uint32_t)(cm >> lane) & 1u;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t v = r[t] + ci;
+            ci = ci & (r[t] == 0xffffffffu ? 1u : 0u);
+            if (j0 + t < a.c) a.C[j0 + t] = v;
+        }
+        all_prop = all_prop && (P == ~0ull);
+        cin = cout;
+    }
+    if (lane == 0) flags[g] = (uint8_t)(cin | (all_prop ? 2u : 0u));
+}
+
+// One wave per chunk >= 1 of each assembly: carry-in by look-back over the
+// assembly's flags, then an in-chunk ripple if it is 1 (carry_fixup_kernel
+// over ragged rows).
+__global__ void __launch_bounds__(256) ragged_assembly_fixup_kernel(const RaggedAssembly *as, uint32_t n,
+                                                                    uint32_t chunks, const uint8_t *flags) {
+    const uint32_t g = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    if (g >= chunks) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const RaggedAssembly a = assembly_of(as, n, g);
+    const uint32_t chunk = g - a.chunk0;
+    if (chunk == 0) return;
+    const uint8_t *rf = flags + a.chunk0;
+    uint32_t cin = 0;
+    for (int64_t hi = (int64_t)chunk - 1; hi >= 0; hi -= kWave) {
+        const int64_t k = hi - (int64_t)lane;
+        const uint32_t f = (k >= 0) ? rf[k] : 0u;  // k < 0: start of the assembly, carry-in 0
+        const uint64_t stop = ballot64((f & 2u) == 0u);
+        if (stop != 0) {
+            cin = __shfl(f, __builtin_ctzll(stop)) & 1u;
+            break;
+        }
+    }
+    if (cin == 0) return;
+    const uint32_t start = chunk * kAssembleChunk, end = min(start + kAssembleChunk, a.c);
+    for (uint32_t seg = start; seg < end; seg += kWave) {
+        const uint32_t j = seg + lane;
+        const bool valid = j < end;
+        const uint32_t v = valid ? a.C[j] : 0u;
+        const uint64_t m = ballot64(!(valid && v == 0xffffffffu));
+        const uint32_t first = m ? (uint32_t)__builtin_ctzll(m) : 64u;
+        if (valid && lane < first) a.C[j] = 0u;
+        if (valid && lane == first) a.C[j] = v + 1u;
+        if (m != 0) break;
+    }
+}
+
 // Row-aligned 16-byte access: the base is 16-byte aligned and, with more
 // than one row, so is every row.
 bool rows_aligned(const void *p, uint32_t N, uint32_t stride) {
@@ -500,6 +605,18 @@ hipError_t launch_add_spills(uint32_t *C, const uint64_t *spill, uint32_t N, uin
         GPUPI_RETURN_IF(hipGetLastError());
     }
     return hipSuccess;
+}
+
+hipError_t launch_ragged_assemble(const uint32_t *staging, uint64_t K, const RaggedAssembly *as, uint32_t n,
+                                  uint32_t chunks, uint8_t *flags, hipStream_t stream) {
+    if (n == 0 || chunks == 0) return hipSuccess;
+    if (staging == nullptr || as == nullptr || flags == nullptr) return hipErrorInvalidValue;
+    const uint32_t blocks = (chunks + 3) / 4;
+    hipLaunchKernelGGL(ragged_assemble_kernel, dim3(blocks), dim3(256), 0, stream, staging, K, as, n, chunks, flags);
+    GPUPI_RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(ragged_assembly_fixup_kernel, dim3(blocks), dim3(256), 0, stream, as, n, chunks,
+                       (const uint8_t *)flags);
+    return hipGetLastError();
 }
 
 }  // namespace gpupi

@@ -102,51 +102,65 @@ int ntt_transform_log2(uint32_t L_a, uint32_t L_b);
 // The fused 17-term group stage: N rows each of P (51 limbs), Q (68), R (83).
 hipError_t launch_group_stage(uint32_t N, uint32_t *P, uint32_t *Q, uint32_t *R, hipStream_t stream);
 // One product row of a ragged NTT launch: A (L_a limbs) times row b of the
-// launch's set of B operands (B, L_b limbs), written to C as L_a + L_b limbs
-// followed by `tail` zero limbs.  Rows sharing a B operand (the pieces of
-// one long product) have it transformed once.
+// launch's set of B operands (B, L_b limbs), written to C as L_a + L_b
+// limbs, or left staged for a RaggedAssembly when C is null.  Rows sharing
+// a B operand (the pieces of one long product) have it transformed once.
 struct RaggedRow {
     const uint32_t *A;
     const uint32_t *B;
     uint32_t *C;
-    uint32_t L_a, L_b, b, tail;
+    uint32_t L_a, L_b, b, pad;
 };
 struct RaggedB {
     const uint32_t *B;
     uint64_t L;
 };
+// A product L * S (c = |L| + |S| limbs) run in m pieces of s limbs of L (the
+// last one shorter) against S: its pieces are the launch's rows row0 ..
+// row0 + m - 1, and C word x is the sum of piece floor(x / s)'s product
+// word x - floor(x / s) s and, below it, piece floor(x / s) - 1's word
+// x - (floor(x / s) - 1) s (the last piece takes every x from (m - 1) s
+// on), with carries.  Assembled by 1024-limb chunks, numbered in the launch
+// from chunk0.
+struct RaggedAssembly {
+    uint32_t *C;
+    uint32_t c, s, S, m, row0, chunk0;
+};
+constexpr uint32_t kAssembleChunk = 1024;
 // Row i of a product L * S (LL and LS limbs) run in m pieces of s limbs of
 // L (the last one LL - (m - 1) s limbs), all against S, row b of the
-// launch's B set: piece i goes to C (even i) or to an accumulator T (odd i,
-// T word 0 = C word s) at (i - i % 2) s, and its zero tail reaches the next
-// piece of its parity (2 s on; pieces of one parity do not overlap when
-// s >= LS, or when m = 2), or the end of C or T for the last two pieces.
-// C[s..) += T then gives L * S.
+// launch's B set; its product stays staged for the assembly.
 __host__ __device__ inline RaggedRow piece_row(uint32_t i, const uint32_t *L, uint32_t LL, const uint32_t *S,
-                                               uint32_t LS, uint32_t *C, uint32_t *T, uint32_t m, uint32_t s,
-                                               uint32_t b) {
-    const uint32_t last = LL - (m - 1) * s, par = i & 1u;
-    const uint32_t len = i + 1 < m ? s : last;
-    const uint32_t tail = i + 1 == m ? 0u : i + 2 == m ? last : s - LS;
-    return RaggedRow{L + (size_t)i * s, S, (par ? T : C) + (size_t)(i - par) * s, len, LS, b, tail};
+                                               uint32_t LS, uint32_t m, uint32_t s, uint32_t b) {
+    const uint32_t len = i + 1 < m ? s : LL - (m - 1) * s;
+    return RaggedRow{L + (size_t)i * s, S, nullptr, len, LS, b, 0};
 }
-// Rows of one launch at most (N rows of 2^log2K points in one launch chunk).
+// Chunks of an assembly of c limbs.
+__host__ __device__ inline uint32_t assembly_chunks(uint32_t c) { return (c + kAssembleChunk - 1) / kAssembleChunk; }
+// Rows a ragged launch is filled to (one launch chunk of 2^log2K-point
+// rows); the pieces of one product are never split across launches, so a
+// launch holding one product's pieces may have more.
 uint32_t ntt_ragged_max_rows(int log2K);
-// L * S in m >= 2 pieces of s limbs (piece_row; s >= LS when m > 2, every
-// piece's product above one tile): the pieces go to C and T, ready for
-// C[s..) += T.  The rows are generated on the device; the workspace holds
-// them and the NTT scratch.
-size_t ntt_pieces_workspace_bytes(uint32_t LS, uint32_t m, uint32_t s);
+// C = L * S in m >= 2 pieces of s limbs (piece_row; s >= LS when m > 2,
+// every piece's product above one tile), assembled into C.  The rows are
+// generated on the device; the workspace holds them and the NTT scratch.
+size_t ntt_pieces_workspace_bytes(uint32_t LL, uint32_t LS, uint32_t m, uint32_t s);
 hipError_t launch_mul_ntt_pieces(const uint32_t *L, uint32_t LL, const uint32_t *S, uint32_t LS, uint32_t *C,
-                                 uint32_t *T, uint32_t m, uint32_t s, void *workspace, size_t workspace_bytes,
+                                 uint32_t m, uint32_t s, void *workspace, size_t workspace_bytes,
                                  const NTTPrecomputedTables &tables, hipStream_t stream);
-size_t ntt_ragged_workspace_bytes(uint32_t N, uint32_t nb, int log2K);
-// N rows over nb B operands, both as device arrays with host copies: every
-// row's L_a + L_b <= 2^log2K and b < nb, N <= ntt_ragged_max_rows(log2K),
-// and tails only above one tile (2^log2K > 4096).
+// The assemblies of a ragged launch from its staged rows (K limbs each):
+// n assemblies, `chunks` chunks in all; `flags` holds one byte per chunk.
+hipError_t launch_ragged_assemble(const uint32_t *staging, uint64_t K, const RaggedAssembly *as, uint32_t n,
+                                  uint32_t chunks, uint8_t *flags, hipStream_t stream);
+size_t ntt_ragged_workspace_bytes(uint32_t N, uint32_t nb, uint32_t chunks, int log2K);
+// N rows over nb B operands and n_asm assemblies of `chunks` chunks in all,
+// as device arrays with host copies: every row's L_a + L_b <= 2^log2K and
+// b < nb; rows without C (assembled) only above one tile (2^log2K > 4096).
 hipError_t launch_mul_ntt_ragged(const RaggedRow *rows, const RaggedRow *host_rows, uint32_t N, const RaggedB *bset,
-                                 const RaggedB *host_bset, uint32_t nb, int log2K, void *workspace,
-                                 size_t workspace_bytes, const NTTPrecomputedTables &tables, hipStream_t stream);
+                                 const RaggedB *host_bset, uint32_t nb, const RaggedAssembly *asms,
+                                 const RaggedAssembly *host_asms, uint32_t n_asm, uint32_t chunks, int log2K,
+                                 void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
+                                 hipStream_t stream);
 bool ntt_supports(uint32_t L_a, uint32_t L_b);
 // Two NTT products sharing operand B (C1 = A1 B over N1 rows, C2 = A2 B over
 // N2 rows; B has max(N1, N2) rows, or one row at stride 0): B is

@@ -1210,15 +1210,16 @@ void ntt_tables_release(NTTPrecomputedTables *tables) {
 namespace {
 
 // Copies limbs [0, L_a + L_b) of each staged product row (stride K) to its
-// row's C, followed by the row's zero tail.  Blocks: (row, 1024-limb slice).
+// row's C (rows without C stay staged for their assembly).  Blocks: (row,
+// 1024-limb slice).
 __global__ void __launch_bounds__(256) ragged_copy_kernel(const uint32_t *staging, uint64_t K,
                                                           const RaggedRow *rows, uint32_t slices) {
     const uint32_t row = blockIdx.x / slices, slice = blockIdx.x % slices;
     const RaggedRow it = rows[row];
-    const uint32_t Lc = it.L_a + it.L_b, end = Lc + it.tail;
+    if (it.C == nullptr) return;
+    const uint32_t Lc = it.L_a + it.L_b;
     const uint32_t *src = staging + (size_t)row * K;
-    for (uint32_t i = slice * 1024u + threadIdx.x; i < end && i < (slice + 1) * 1024u; i += 256u)
-        it.C[i] = i < Lc ? src[i] : 0u;
+    for (uint32_t i = slice * 1024u + threadIdx.x; i < Lc && i < (slice + 1) * 1024u; i += 256u) it.C[i] = src[i];
 }
 
 // One NTT launch sequence over products of one transform size 2^k: up to
@@ -1241,11 +1242,14 @@ struct NttJob {
     Prod prods[2];
     uint32_t n_ops = 0, n_prods = 0;
     bool ragged = false;
-    const RaggedRow *rows = nullptr;  // ragged: device rows and B set, with host copies
+    const RaggedRow *rows = nullptr;  // ragged: device rows, B set and assemblies, with host copies
     const RaggedRow *host_rows = nullptr;
     const RaggedB *bset = nullptr;
     const RaggedB *host_bset = nullptr;
     uint32_t nb = 0;
+    const RaggedAssembly *asms = nullptr;
+    const RaggedAssembly *host_asms = nullptr;
+    uint32_t n_asm = 0, asm_chunks = 0;
     int k = 0;
     size_t max_chunk = 0;  // rows per launch chunk at most (0: the default)
 };
@@ -1284,7 +1288,7 @@ bool job_ok(const NttJob &j) {
     if (j.n_ops < 1 || j.n_ops > 3 || j.n_prods < 1 || j.n_prods > 2) return false;
     if (j.ragged && (j.rows == nullptr || j.host_rows == nullptr || j.bset == nullptr || j.host_bset == nullptr ||
                      j.nb == 0 || j.nb > j.prods[0].N || j.n_ops != 2 || j.n_prods != 1 ||
-                     j.prods[0].N > ragged_max_rows(j.k)))
+                     (j.n_asm > 0 && (j.asms == nullptr || j.host_asms == nullptr))))
         return false;
     for (uint32_t q = 0; q < j.n_prods; ++q) {
         const NttJob::Prod &pr = j.prods[q];
@@ -1301,14 +1305,15 @@ struct JobLayout {
     size_t plane_elems[3] = {0, 0, 0};  // per prime, per operand (elements)
     size_t op_base[3] = {0, 0, 0};      // element offset of operand o's prime-0 plane
     size_t spill_elems = 0;             // per product
-    size_t spill_off = 0, flags_off = 0, staging_off = 0, total = 0;  // bytes
+    size_t spill_off = 0, flags_off = 0, staging_off = 0, asm_flags_off = 0, total = 0;  // bytes
     size_t nc = 0;
 };
 
 JobLayout layout_of(const NttJob &j, const JobShape &sh) {
     JobLayout l;
     const size_t K = size_t(1) << j.k;
-    l.nc = chunk_products(sh.rows, j.k);
+    // a ragged launch is one chunk (its B set and staged pieces stay resident)
+    l.nc = j.ragged ? sh.rows : chunk_products(sh.rows, j.k);
     if (j.max_chunk != 0 && l.nc > j.max_chunk) l.nc = j.max_chunk;
     size_t off = 0;
     for (uint32_t o = 0; o < j.n_ops; ++o) {
@@ -1326,6 +1331,8 @@ JobLayout layout_of(const NttJob &j, const JobShape &sh) {
     off += align256(fl);
     l.staging_off = off;
     if (j.ragged) off += align256(l.nc * K * sizeof(uint32_t));
+    l.asm_flags_off = off;
+    if (j.ragged) off += align256(j.asm_chunks);
     l.total = off;
     return l;
 }
@@ -1544,14 +1551,17 @@ hipError_t run_ntt(const NttJob &j, void *workspace, size_t workspace_bytes, con
                                               1u << final_run_log2(p), (uint32_t)runs, flags, stream));
         }
         if (j.ragged) {
-            uint64_t span = K;  // longest row with its zero tail
-            for (uint32_t i = 0; i < rows[0]; ++i)
-                span = std::max<uint64_t>(span, (uint64_t)j.host_rows[i].L_a + j.host_rows[i].L_b + j.host_rows[i].tail);
-            const uint32_t slices = (uint32_t)ceil_div64(span, 1024);
+            bool copies = false;
+            for (uint32_t i = 0; i < rows[0] && !copies; ++i) copies = j.host_rows[i].C != nullptr;
+            const uint32_t slices = (uint32_t)ceil_div64(K, 1024);
             if ((uint64_t)rows[0] * slices > 0x7fffffffull) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(ragged_copy_kernel, dim3(rows[0] * slices), dim3(256), 0, stream, staging, (uint64_t)K,
-                               a.rows, slices);
-            GPUPI_RETURN_IF(hipGetLastError());
+            if (copies) {
+                hipLaunchKernelGGL(ragged_copy_kernel, dim3(rows[0] * slices), dim3(256), 0, stream, staging,
+                                   (uint64_t)K, a.rows, slices);
+                GPUPI_RETURN_IF(hipGetLastError());
+            }
+            GPUPI_RETURN_IF(launch_ragged_assemble(staging, K, j.asms, j.n_asm, j.asm_chunks,
+                                                   reinterpret_cast<uint8_t *>(ws + l.asm_flags_off), stream));
         }
     }
     return hipSuccess;
@@ -1686,7 +1696,8 @@ int ntt_transform_log2(uint32_t L_a, uint32_t L_b) { return transform_log2(L_a, 
 
 namespace {
 NttJob ragged_job(const RaggedRow *rows, const RaggedRow *host_rows, uint32_t N, const RaggedB *bset,
-                  const RaggedB *host_bset, uint32_t nb, int log2K) {
+                  const RaggedB *host_bset, uint32_t nb, const RaggedAssembly *asms,
+                  const RaggedAssembly *host_asms, uint32_t n_asm, uint32_t asm_chunks, int log2K) {
     NttJob j;
     j.n_ops = 2;
     j.n_prods = 1;
@@ -1697,90 +1708,126 @@ NttJob ragged_job(const RaggedRow *rows, const RaggedRow *host_rows, uint32_t N,
     j.bset = bset;
     j.host_bset = host_bset;
     j.nb = nb;
+    j.asms = asms;
+    j.host_asms = host_asms;
+    j.n_asm = n_asm;
+    j.asm_chunks = asm_chunks;
     j.k = log2K;
     return j;
+}
+
+// A launch's rows, B set and assemblies agree: every row inside its
+// transform and B set, assembled rows only above one tile, each assembly's
+// pieces rows without C in its range, chunks numbered in order.
+bool ragged_ok(const RaggedRow *rows, uint32_t N, const RaggedB *bset, uint32_t nb, const RaggedAssembly *asms,
+               uint32_t n_asm, uint32_t chunks, int log2K) {
+    const uint64_t K = 1ull << log2K;
+    for (uint32_t i = 0; i < N; ++i) {
+        const RaggedRow &r = rows[i];
+        if (r.b >= nb || (uint64_t)r.L_a + r.L_b > K || bset[r.b].L != r.L_b) return false;
+        if (r.C == nullptr && log2K <= kTileLog2) return false;
+    }
+    uint32_t next = 0;
+    for (uint32_t q = 0; q < n_asm; ++q) {
+        const RaggedAssembly &x = asms[q];
+        if (x.C == nullptr || x.m < 2 || x.s == 0 || x.S == 0 || x.chunk0 != next || x.row0 > N || x.m > N - x.row0)
+            return false;
+        if (x.m > 2 && x.s < x.S) return false;  // pieces of one parity would overlap
+        const uint64_t last = (uint64_t)x.c - x.S - (uint64_t)(x.m - 1) * x.s;
+        if ((uint64_t)x.c < (uint64_t)x.S + (uint64_t)(x.m - 1) * x.s + 1 || last > x.s) return false;
+        for (uint32_t i = 0; i < x.m; ++i) {
+            const RaggedRow &r = rows[x.row0 + i];
+            if (r.C != nullptr || r.L_b != x.S || r.L_a != (i + 1 < x.m ? x.s : (uint32_t)last)) return false;
+        }
+        next += assembly_chunks(x.c);
+    }
+    return next == chunks;
 }
 }  // namespace
 
 uint32_t ntt_ragged_max_rows(int log2K) { return ragged_max_rows(log2K); }
 
 namespace {
-// The rows of a product in pieces (piece_row) and its one B entry, written
-// on the device (no host upload).
-__global__ void __launch_bounds__(256) piece_rows_kernel(RaggedRow *rows, RaggedB *bset, const uint32_t *L,
-                                                         uint32_t LL, const uint32_t *S, uint32_t LS, uint32_t *C,
-                                                         uint32_t *T, uint32_t m, uint32_t s) {
+// The rows of a product in pieces (piece_row), its one B entry and its
+// assembly, written on the device (no host upload).
+__global__ void __launch_bounds__(256) piece_rows_kernel(RaggedRow *rows, RaggedB *bset, RaggedAssembly *as,
+                                                         const uint32_t *L, uint32_t LL, const uint32_t *S,
+                                                         uint32_t LS, uint32_t *C, uint32_t m, uint32_t s) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i == 0) bset[0] = RaggedB{S, LS};
-    if (i < m) rows[i] = piece_row(i, L, LL, S, LS, C, T, m, s, 0);
+    if (i == 0) {
+        bset[0] = RaggedB{S, LS};
+        as[0] = RaggedAssembly{C, LL + LS, s, LS, m, 0, 0};
+    }
+    if (i < m) rows[i] = piece_row(i, L, LL, S, LS, m, s, 0);
 }
 
 struct PiecesLayout {
-    size_t rows_bytes, bset_bytes, ntt_bytes;
-    uint32_t per_launch;
+    size_t rows_bytes, bset_bytes, asm_bytes, ntt_bytes;
+    uint32_t chunks;
     int k;
 };
-PiecesLayout pieces_layout(uint32_t LS, uint32_t m, uint32_t s) {
+PiecesLayout pieces_layout(uint32_t LL, uint32_t LS, uint32_t m, uint32_t s) {
     PiecesLayout l;
     l.k = transform_log2(s, LS);
-    l.per_launch = std::min(m, ragged_max_rows(l.k));
+    l.chunks = assembly_chunks(LL + LS);
     l.rows_bytes = align256((size_t)m * sizeof(RaggedRow));
     l.bset_bytes = align256(sizeof(RaggedB));
-    l.ntt_bytes = job_workspace_bytes(ragged_job(nullptr, nullptr, l.per_launch, nullptr, nullptr, 1, l.k));
+    l.asm_bytes = align256(sizeof(RaggedAssembly));
+    l.ntt_bytes =
+        job_workspace_bytes(ragged_job(nullptr, nullptr, m, nullptr, nullptr, 1, nullptr, nullptr, 1, l.chunks, l.k));
     return l;
 }
 }  // namespace
 
-size_t ntt_pieces_workspace_bytes(uint32_t LS, uint32_t m, uint32_t s) {
-    const PiecesLayout l = pieces_layout(LS, m, s);
-    return l.rows_bytes + l.bset_bytes + l.ntt_bytes;
+size_t ntt_pieces_workspace_bytes(uint32_t LL, uint32_t LS, uint32_t m, uint32_t s) {
+    const PiecesLayout l = pieces_layout(LL, LS, m, s);
+    return l.rows_bytes + l.bset_bytes + l.asm_bytes + l.ntt_bytes;
 }
 
 hipError_t launch_mul_ntt_pieces(const uint32_t *L, uint32_t LL, const uint32_t *S, uint32_t LS, uint32_t *C,
-                                 uint32_t *T, uint32_t m, uint32_t s, void *workspace, size_t workspace_bytes,
+                                 uint32_t m, uint32_t s, void *workspace, size_t workspace_bytes,
                                  const NTTPrecomputedTables &tables, hipStream_t stream) {
-    if (m < 2 || s == 0 || LS == 0 || (m > 2 && s < LS) || (uint64_t)(m - 1) * s >= LL ||
-        (uint64_t)LL - (uint64_t)(m - 1) * s > s || tables.roots == nullptr)
+    if (L == nullptr || S == nullptr || C == nullptr || m < 2 || s == 0 || LS == 0 || (m > 2 && s < LS) ||
+        (uint64_t)(m - 1) * s >= LL || (uint64_t)LL - (uint64_t)(m - 1) * s > s ||
+        (uint64_t)LL + LS > 0xffffffffull || tables.roots == nullptr)
         return hipErrorInvalidValue;
-    const PiecesLayout l = pieces_layout(LS, m, s);
+    const PiecesLayout l = pieces_layout(LL, LS, m, s);
     if (l.k <= kTileLog2 || l.k > kMaxLog2 || workspace == nullptr ||
-        workspace_bytes < l.rows_bytes + l.bset_bytes + l.ntt_bytes)
+        workspace_bytes < l.rows_bytes + l.bset_bytes + l.asm_bytes + l.ntt_bytes)
         return hipErrorInvalidValue;
     char *ws = static_cast<char *>(workspace);
     auto *rows = reinterpret_cast<RaggedRow *>(ws);
     auto *bset = reinterpret_cast<RaggedB *>(ws + l.rows_bytes);
-    hipLaunchKernelGGL(piece_rows_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, rows, bset, L, LL, S, LS, C, T,
-                       m, s);
+    auto *as = reinterpret_cast<RaggedAssembly *>(ws + l.rows_bytes + l.bset_bytes);
+    hipLaunchKernelGGL(piece_rows_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, rows, bset, as, L, LL, S, LS,
+                       C, m, s);
     GPUPI_RETURN_IF(hipGetLastError());
-    // the same rows on the host, for the launch's bookkeeping
+    // the same descriptors on the host, for the launch's bookkeeping
     std::vector<RaggedRow> host_rows(m);
-    for (uint32_t i = 0; i < m; ++i) host_rows[i] = piece_row(i, L, LL, S, LS, C, T, m, s, 0);
+    for (uint32_t i = 0; i < m; ++i) host_rows[i] = piece_row(i, L, LL, S, LS, m, s, 0);
     const RaggedB host_b{S, LS};
-    for (uint32_t r0 = 0; r0 < m; r0 += l.per_launch) {
-        const uint32_t n = std::min(l.per_launch, m - r0);
-        GPUPI_RETURN_IF(run_ntt(ragged_job(rows + r0, host_rows.data() + r0, n, bset, &host_b, 1, l.k),
-                                ws + l.rows_bytes + l.bset_bytes, l.ntt_bytes, tables, stream));
-    }
-    return hipSuccess;
+    const RaggedAssembly host_as{C, LL + LS, s, LS, m, 0, 0};
+    return run_ntt(ragged_job(rows, host_rows.data(), m, bset, &host_b, 1, as, &host_as, 1, l.chunks, l.k),
+                   ws + l.rows_bytes + l.bset_bytes + l.asm_bytes, l.ntt_bytes, tables, stream);
 }
 
-size_t ntt_ragged_workspace_bytes(uint32_t N, uint32_t nb, int log2K) {
-    return job_workspace_bytes(ragged_job(nullptr, nullptr, N, nullptr, nullptr, nb, log2K));
+size_t ntt_ragged_workspace_bytes(uint32_t N, uint32_t nb, uint32_t chunks, int log2K) {
+    return job_workspace_bytes(
+        ragged_job(nullptr, nullptr, N, nullptr, nullptr, nb, nullptr, nullptr, chunks ? 1u : 0u, chunks, log2K));
 }
 
 hipError_t launch_mul_ntt_ragged(const RaggedRow *rows, const RaggedRow *host_rows, uint32_t N, const RaggedB *bset,
-                                 const RaggedB *host_bset, uint32_t nb, int log2K, void *workspace,
-                                 size_t workspace_bytes, const NTTPrecomputedTables &tables, hipStream_t stream) {
+                                 const RaggedB *host_bset, uint32_t nb, const RaggedAssembly *asms,
+                                 const RaggedAssembly *host_asms, uint32_t n_asm, uint32_t chunks, int log2K,
+                                 void *workspace, size_t workspace_bytes, const NTTPrecomputedTables &tables,
+                                 hipStream_t stream) {
     if (N == 0) return hipSuccess;
-    if (tables.roots == nullptr || log2K < kMinLog2 || log2K > kMaxLog2) return hipErrorInvalidValue;
-    for (uint32_t i = 0; i < N && host_rows != nullptr; ++i) {
-        const RaggedRow &r = host_rows[i];
-        if (r.b >= nb || (uint64_t)r.L_a + r.L_b > (1ull << log2K) || (r.tail != 0 && log2K <= kTileLog2) ||
-            (host_bset != nullptr && host_bset[r.b].L != r.L_b))
-            return hipErrorInvalidValue;
-    }
-    return run_ntt(ragged_job(rows, host_rows, N, bset, host_bset, nb, log2K), workspace, workspace_bytes, tables,
-                   stream);
+    if (tables.roots == nullptr || log2K < kMinLog2 || log2K > kMaxLog2 || rows == nullptr || host_rows == nullptr ||
+        bset == nullptr || host_bset == nullptr || (n_asm > 0 && (asms == nullptr || host_asms == nullptr)) ||
+        !ragged_ok(host_rows, N, host_bset, nb, host_asms, n_asm, chunks, log2K))
+        return hipErrorInvalidValue;
+    return run_ntt(ragged_job(rows, host_rows, N, bset, host_bset, nb, asms, host_asms, n_asm, chunks, log2K),
+                   workspace, workspace_bytes, tables, stream);
 }
 
 }  // namespace gpupi
