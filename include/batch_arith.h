@@ -66,9 +66,12 @@ hipError_t batch_mp_mul(BatchMPContext *ctx, BatchMPArray A, BatchMPArray B, Bat
 /* Ragged batched product (no counterpart in the reference, whose batch API
  * has one L_a/L_b per call): items[i].C = items[i].A * items[i].B with
  * L_a + L_b limbs written, for pairs of any lengths.  Items are grouped by
- * transform size and each group runs as one launch sequence; descriptors
- * are read from host memory before the call returns.  C must not alias its
- * own item's operands; distinct items' C must not overlap. */
+ * transform size and each group runs as one launch sequence; an unbalanced
+ * pair may run as pieces of its longer operand sharing the shorter one's
+ * transform, and items passing the same B (pointer and length) share its
+ * transform.  Descriptors are read from host memory before the call
+ * returns.  No item's C may overlap any item's operands, and distinct
+ * items' C must not overlap. */
 typedef struct GpupiMulItem {
     const uint32_t *A;
     const uint32_t *B;
