@@ -574,7 +574,9 @@ __device__ __forceinline__ uint32_t row_step0_base() {
 
 // Last level: per prime, forward on A's tile (kept in registers), forward on
 // B's tile, pointwise product, inverse; written back over A's plane.  A
-// square (B is A) multiplies A's transformed tile by itself.
+// square (B is A) multiplies A's transformed tile by itself.  The next
+// prime's A tile is loaded once A's values are multiplied in (its registers
+// are free then), so that the load runs during the inverse.
 template <int M>
 __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     __shared__ double lds[kLdsSlots];
@@ -587,18 +589,19 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     double *TA0 = a.T + oa.base + (size_t)bp.row * Kp + off;
     const uint32_t brow = a.rows ? a.rows[bp.row].b : pr.b_bcast ? 0u : bp.row;
     const double *TB0 = a.T + ob.base + (size_t)brow * Kp + off;
+    const bool square = pr.a == pr.b;  // uniform over the launch
+    double ra[16], v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) ra[j] = TA0[j * kJ];
 #pragma clang loop unroll(disable)
     for (int P = 0; P < 2; ++P) {
         const Mod m = mod_of(P);
         const double *tab = a.tables + P * kPrimeTable;
         double *TA = TA0 + P * oa.pstride;
         const double *TB = TB0 + P * ob.pstride;
-        double ra[16], v[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) ra[j] = TA[j * kJ];
         fwd_dft<M, false, kDifToPointA>(ra, lds, tab, P, m);
         __syncthreads();
-        if (pr.a == pr.b) {  // uniform over the launch
+        if (square) {
 #pragma unroll
             for (int j = 0; j < 16; ++j) v[j] = mulp(ra[j], ra[j], m);
         } else {
@@ -611,6 +614,10 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
             fwd_dft<M, false, kDifToPointB>(v, lds, tabb, P, m);
 #pragma unroll
             for (int j = 0; j < 16; ++j) v[j] = mulp(v[j], ra[j], m);
+        }
+        if (P == 0) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) ra[j] = TA[oa.pstride + j * kJ];
         }
         inv_dft<M, false>(v, lds, tab, P, m);
 #pragma unroll
