@@ -301,9 +301,11 @@ __global__ void __launch_bounds__(256) ragged_assemble_kernel(const uint32_t *st
             d[t] = 0;
             gen[t] = 0;
             if (x < a.c) {
-                const uint32_t i = min(x / a.s, a.m - 1), off = x - i * a.s;
+                uint32_t i, off;
+                bool below;
+                assembly_source(a, x, &i, &off, &below);
                 const uint32_t u = staging[(size_t)(a.row0 + i) * K + off];
-                const uint32_t v = (i > 0 && off < a.S) ? staging[(size_t)(a.row0 + i - 1) * K + off + a.s] : 0u;
+                const uint32_t v = below ? staging[(size_t)(a.row0 + i - 1) * K + off + a.s] : 0u;
                 d[t] = u + v;
                 gen[t] = d[t] < u ? 1u : 0u;
             }

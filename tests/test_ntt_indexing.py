@@ -41,3 +41,19 @@ def test_group_exact_division_step(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "hensel check: ok" in out.stdout
+
+
+def test_ragged_assembly_rules(tmp_path):
+    """Products run in pieces (gpupi_amd/csrc/ragged.hpp, the rules the
+    ragged launches and the assembly kernel use): the pieces tile the longer
+    operand, every staged word the assembly reads lies inside its piece's
+    product, and the assembled words with carries equal the schoolbook
+    product, for both kinds of split (two pieces shorter than the other
+    operand, many pieces at least as long) on random, all-ones and sparse
+    limbs."""
+    exe = tmp_path / "ragged_assembly_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "gpupi_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "native", "ragged_assembly_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "ragged assembly check: ok" in out.stdout
