@@ -456,6 +456,18 @@ __device__ __forceinline__ void col_fwd_prime(double *v, double *lds, const ColT
     if (BARRIER) __syncthreads();
 }
 
+// col_fwd_prime with the column twiddles loaded after the transform (fewer
+// registers held through it), then a barrier.
+template <int M, int P>
+__device__ __forceinline__ void col_fwd_prime_late(double *v, double *lds, const ColThread<M> &ct,
+                                                   const double *tables) {
+    const Mod m = mod_of(P);
+    const double *tab = tables + P * kPrimeTable;
+    fwd_dft<M, true, kDifToTwiddle>(v, lds, tab, P, m, ct.tid);
+    column_twiddles<M>(v, col_tw_load<M>(tab, ct.cg, ct.B0, false), m);
+    __syncthreads();
+}
+
 // Forward column level above the first: blocks cover (product, operand,
 // tile); both primes pass through the tile in turn; the inter-level twiddle
 // is applied on store.  Every global load (both primes' residues and column
@@ -519,11 +531,15 @@ __device__ __forceinline__ void load_limbs(uint32_t *x, const PassArgs &a, const
 // second prime has, the next tile block's limbs are loaded into the same
 // registers, so that their latency (and the completion of the first
 // prime's stores before them) overlaps the second prime's transform and
-// stores.  It needs 126-135 VGPRs (three waves per SIMD); held to 128 (four
-// waves) it spills 4-8 and, measured at 10^9 digits, loses 6 ms per step to
-// the three-wave build (the non-persistent kernel: 185 -> 176 ms).
+// stores.  Each prime's column twiddles are loaded after its transform (not
+// held through it), which brings the 2^6..2^8-point levels to 121-125 VGPRs:
+// four waves per SIMD without spills (at 10^9 digits 179 -> 165 ms per
+// step, against three waves).  The 2^5 and 2^9-point levels would spill
+// at four and keep three.
+constexpr int limbs_waves(int m) { return m >= 6 && m <= 8 ? 4 : 3; }
 template <int M, bool PAIR>
-__global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_limbs_kernel(PassArgs a, uint32_t total) {
+__global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) __attribute__((amdgpu_waves_per_eu(limbs_waves(M))))
+ntt_col_fwd_limbs_kernel(PassArgs a, uint32_t total) {
     using CP = ColPair<M, PAIR>;
     __shared__ double lds_tiles[CP::kLdsWords];
     double *const lds = lds_tiles + CP::half() * CP::kTileStride;
@@ -538,16 +554,12 @@ __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_limb
         const NttOp o = a.ops[bp.seg];
         double *T0 = a.T + o.base + (size_t)bp.row * plane_row_elems(a.plan);
         double *T1 = T0 + o.pstride;
-        const ColTw tw0 = col_tw_load<M>(a.tables, ct.cg, ct.B0, false);
         double v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) v[u] = (double)x[u];
-        col_fwd_prime<M, 0, true>(v, lds, ct, a.tables, tw0);
+        col_fwd_prime_late<M, 0>(v, lds, ct, a.tables);
 #pragma unroll
         for (int k = 0; k < 16; ++k) T0[ct.plast(k)] = v[k];
-        // prime 1's column twiddles: loaded here (not held through prime 0),
-        // used after its transform
-        const ColTw tw1 = col_tw_load<M>(a.tables + kPrimeTable, ct.cg, ct.B0, false);
 #pragma unroll
         for (int u = 0; u < 16; ++u) v[u] = (double)x[u];
         const uint32_t wn = w + gridDim.x;
@@ -557,7 +569,7 @@ __global__ void __launch_bounds__((ColPair<M, PAIR>::kThreads)) ntt_col_fwd_limb
             load_limbs<M>(x, a, bp, ColThread<M>(a.plan, 0, (bp.t << CP::kLog2) + CP::half(), CP::vtid()));
         }
         // the barrier at its end lets the next tile block reuse `lds`
-        col_fwd_prime<M, 1, true>(v, lds, ct, a.tables, tw1);
+        col_fwd_prime_late<M, 1>(v, lds, ct, a.tables);
 #pragma unroll
         for (int k = 0; k < 16; ++k) T1[ct.plast(k)] = v[k];
         if (!more) break;
