@@ -49,13 +49,15 @@ struct Plan {
 // k <= 12: one local level.  k <= 21: a column level and a contiguous
 // level.  Otherwise two column levels and a contiguous level.  The column
 // sizes are the fastest measured on MI355X for every k (batched products of
-// 2^28 coefficients in all, every split the kernels cover): first levels of
-// 2^8 points (256-byte runs per row), and contiguous levels of 2^8..2^12.
-// k = 29, 30 (not measured) keep a 4096-point contiguous level.
+// 2^28 coefficients in all, every split the kernels cover, single products
+// and the tree's shared-operand pairs agreeing): first levels of 2^7 or 2^8
+// points (the sizes whose first forward level runs four waves per SIMD),
+// and contiguous levels of 2^6..2^12.  k = 29, 30 (not measured) keep a
+// 4096-point contiguous level.
 GPUPI_HD Plan make_plan(int log2K) {
     //                          k: 13 14 15 16 17 18 19 20 21 22 23 24 25 26 27 28 29 30
-    constexpr int8_t kM0[18] = {5, 6, 8, 8, 8, 8, 8, 8, 9, 8, 8, 8, 8, 7, 8, 8, 8, 9};
-    constexpr int8_t kM1[18] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 6, 7, 8, 6, 8, 8, 8, 9, 9};
+    constexpr int8_t kM0[18] = {7, 7, 7, 8, 7, 7, 8, 8, 9, 7, 7, 8, 7, 7, 7, 8, 8, 9};
+    constexpr int8_t kM1[18] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 7, 8, 8, 8, 8, 8, 8, 9, 9};
     Plan p{};
     p.log2K = log2K;
     if (log2K <= kTileLog2) {
