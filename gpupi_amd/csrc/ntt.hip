@@ -339,15 +339,17 @@ struct PassArgs {
     int xcd_log2;
 };
 
-// Global loads between loads_begin() and loads_end() issue at raised wave
-// priority: a wave about to wait on memory gets its requests out ahead of
-// the waves that have arithmetic ready, so that the latency overlaps their
-// work (the loads are issued, not waited for, in between).  Used where it
-// measured faster: the row levels and the first forward level's limb loads
-// (at 10^9 digits about 1 % each); around the second column levels' and the
-// first inverse level's loads it was slower (first inverse 123 -> 137 ms).
-__device__ __forceinline__ void loads_begin() { __builtin_amdgcn_s_setprio(1); }
-__device__ __forceinline__ void loads_end() { __builtin_amdgcn_s_setprio(0); }
+// Global memory instructions between memory_first_begin() and
+// memory_first_end() issue at raised wave priority: a wave about to wait on
+// memory gets its requests out ahead of the waves that have arithmetic
+// ready, so that the latency overlaps their work (the accesses are issued,
+// not waited for, in between).  Used where it measured faster: the row
+// levels' and the first forward level's loads (at 10^9 digits about 1 %
+// each), and the first forward level's stores (1.4 % more); around the
+// second column levels' and the first inverse level's loads it was slower
+// (first inverse 123 -> 137 ms), around the row levels' stores neutral.
+__device__ __forceinline__ void memory_first_begin() { __builtin_amdgcn_s_setprio(1); }
+__device__ __forceinline__ void memory_first_end() { __builtin_amdgcn_s_setprio(0); }
 
 // The (segment, row, tile) a block works on: segments are operands (forward
 // launches) or products (the others), each `rows x tiles` blocks in turn.
@@ -527,13 +529,13 @@ __device__ __forceinline__ void load_limbs(uint32_t *x, const PassArgs &a, const
             LX = rr.L_a;
         }
     }
-    loads_begin();
+    memory_first_begin();
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         const uint64_t off = ct.off(u);
         x[u] = off < LX ? X[off] : 0u;
     }
-    loads_end();
+    memory_first_end();
 }
 
 // First forward column level, persistent: block b works through the tile
@@ -570,8 +572,10 @@ ntt_col_fwd_limbs_kernel(PassArgs a, uint32_t total) {
 #pragma unroll
         for (int u = 0; u < 16; ++u) v[u] = (double)x[u];
         col_fwd_prime_late<M, 0>(v, lds, ct, a.tables);
+        memory_first_begin();
 #pragma unroll
         for (int k = 0; k < 16; ++k) T0[ct.plast(k)] = v[k];
+        memory_first_end();
 #pragma unroll
         for (int u = 0; u < 16; ++u) v[u] = (double)x[u];
         const uint32_t wn = w + gridDim.x;
@@ -582,8 +586,10 @@ ntt_col_fwd_limbs_kernel(PassArgs a, uint32_t total) {
         }
         // the barrier at its end lets the next tile block reuse `lds`
         col_fwd_prime_late<M, 1>(v, lds, ct, a.tables);
+        memory_first_begin();
 #pragma unroll
         for (int k = 0; k < 16; ++k) T1[ct.plast(k)] = v[k];
+        memory_first_end();
         if (!more) break;
         w = wn;
     }
@@ -615,10 +621,10 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
     const double *TB0 = a.T + ob.base + (size_t)brow * Kp + off;
     const bool square = pr.a == pr.b;  // uniform over the launch
     double ra[16], v[16];
-    loads_begin();
+    memory_first_begin();
 #pragma unroll
     for (int j = 0; j < 16; ++j) ra[j] = TA0[j * kJ];
-    loads_end();
+    memory_first_end();
 #pragma clang loop unroll(disable)
     for (int P = 0; P < 2; ++P) {
         const Mod m = mod_of(P);
@@ -635,19 +641,19 @@ __global__ void __launch_bounds__(256) ntt_row_fused_kernel(PassArgs a) {
             // after A's transform (fewer registers live)
             const double *tabb = tab;
             __asm__ volatile("" : "+s"(tabb));
-            loads_begin();
+            memory_first_begin();
 #pragma unroll
             for (int j = 0; j < 16; ++j) v[j] = TB[j * kJ];
-            loads_end();
+            memory_first_end();
             fwd_dft<M, false, kDifToPointB>(v, lds, tabb, P, m);
 #pragma unroll
             for (int j = 0; j < 16; ++j) v[j] = mulp(v[j], ra[j], m);
         }
         if (P == 0) {
-            loads_begin();
+            memory_first_begin();
 #pragma unroll
             for (int j = 0; j < 16; ++j) ra[j] = TA[oa.pstride + j * kJ];
-            loads_end();
+            memory_first_end();
         }
         inv_dft<M, false>(v, lds, tab, P, m);
 #pragma unroll
@@ -682,10 +688,10 @@ __global__ void __launch_bounds__(256) ntt_row_pair_kernel(PassArgs a, uint32_t 
         const double *tab = a.tables + P * kPrimeTable;
         const double *TB = TB0 + P * ob.pstride;
         double rb[16], v[16];
-        loads_begin();
+        memory_first_begin();
 #pragma unroll
         for (int j = 0; j < 16; ++j) rb[j] = TB[j * kJ];
-        loads_end();
+        memory_first_end();
         fwd_dft<M, false, kDifToPointB>(rb, lds, tab, P, m);
 #pragma clang loop unroll(disable)
         for (int q = 0; q < 2; ++q) {
@@ -698,10 +704,10 @@ __global__ void __launch_bounds__(256) ntt_row_pair_kernel(PassArgs a, uint32_t 
             const double *tabq = tab;
             __asm__ volatile("" : "+s"(tabq));
             __syncthreads();
-            loads_begin();
+            memory_first_begin();
 #pragma unroll
             for (int j = 0; j < 16; ++j) v[j] = TA[j * kJ];
-            loads_end();
+            memory_first_end();
             fwd_dft<M, false, kDifToPointA>(v, lds, tabq, P, m);
 #pragma unroll
             for (int j = 0; j < 16; ++j) v[j] = mulp(rb[j], v[j], m);
