@@ -748,8 +748,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the pi pipeline here is single-device).", file=sys.stderr)
+        print(f"bench.py: the gpupi_amd pipeline runs on one GPU; --gpus {args.gpus} is not supported "
+              "(every stage of it is single-device).", file=sys.stderr)
         return 2
 
     import gpupi_amd as g

@@ -25,3 +25,16 @@ def test_ret_mod_hash_matches_horner_and_oracle(n):
     assert of.mod_hash(a) == want
     ones = np.full(n, 0xFFFFFFFF, dtype=np.uint32)
     assert g.ret_mod_hash(ones) == horner_hash(ones)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one process on one GPU: --gpus above 1 is refused before anything
+    # touches a device
+    import pathlib
+    import subprocess
+    import sys
+    root = pathlib.Path(__file__).resolve().parent.parent
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
